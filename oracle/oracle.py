@@ -1,0 +1,518 @@
+"""Python restatement of the reference batch renderer's host logic, driving the
+C oracle rasteriser (raster_ref.c).  TEST INFRASTRUCTURE ONLY: used by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker; the
+product never imports it.
+
+Restated from src/esp/gfx_batch/Renderer.cpp (habitat-sim):
+  - RendererConfiguration defaults ........................ :58-64
+  - create(): white texture 0, white material 0 ............ :272-309
+  - addFile(): texture/mesh/material offsets, flat
+    materials, mesh views, named templates, Whole flag ...... :354-867
+  - drawBatchId() linear search on (shader flags, mesh, tex)  :136-152
+  - addNodeHierarchy() / addEmptyNode() / clear() .......... :880-1040
+  - updateCamera(): camera = P*V, unprojection ............. :1077-1088
+  - draw(): counting sort of draws by batch id ............. :1151-1188
+    and tile -> scene mapping sceneId = y*tx + x ........... :1298-1303
+and AbstractReplayRenderer::environmentGridSize()
+(src/esp/sim/AbstractReplayRenderer.cpp:26-31).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+WHITE_TEX = 0
+NO_TEXTURES = 1
+FILE_WHOLE = 1
+FILE_GENERATE_MIPMAP = 2
+
+GL_NEAREST = 9728
+GL_NEAREST_MIPMAP_NEAREST = 9984
+GL_REPEAT = 10497
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [
+        ("pos", ctypes.c_void_p), ("uv", ctypes.c_void_p), ("col", ctypes.c_void_p),
+        ("tris", ctypes.c_void_p),
+        ("ntex", ctypes.c_int32), ("tex_desc", ctypes.c_void_p),
+        ("tex_level_off", ctypes.c_void_p), ("texels", ctypes.c_void_p),
+        ("draw_i", ctypes.c_void_p), ("draw_f", ctypes.c_void_p),
+        ("node_parent", ctypes.c_void_p), ("node_local", ctypes.c_void_p),
+        ("nenv", ctypes.c_int32), ("env_i", ctypes.c_void_p), ("env_f", ctypes.c_void_p),
+        ("out_color", ctypes.c_void_p), ("out_depth", ctypes.c_void_p),
+        ("out_lindepth", ctypes.c_void_p), ("out_semantic", ctypes.c_void_p),
+        ("out_count", ctypes.c_void_p), ("out_draw", ctypes.c_void_p),
+        ("out_tri", ctypes.c_void_p),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "_build", "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        _LIB = ctypes.CDLL(path)
+        _LIB.orc_render.argtypes = [ctypes.POINTER(_Job), ctypes.c_int]
+        _LIB.orc_mat4_mul.argtypes = [ctypes.c_void_p] * 3
+        _LIB.orc_depth_unprojection.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _LIB.orc_unproject_depth.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        _LIB.orc_log2_export.argtypes = [ctypes.c_float]
+        _LIB.orc_log2_export.restype = ctypes.c_float
+    return _LIB
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# Matrix helpers (Magnum conventions: arrays indexed [col][row])
+# ---------------------------------------------------------------------------
+
+def mat4_mul(a, b) -> np.ndarray:
+    """Magnum Matrix4 product via the oracle's C routine (float32, no FMA)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.empty((4, 4), np.float32)
+    lib().orc_mat4_mul(_p(a), _p(b), _p(out))
+    return out
+
+
+def identity() -> np.ndarray:
+    return np.eye(4, dtype=np.float32)
+
+
+def depth_unprojection(P) -> np.ndarray:
+    P = np.ascontiguousarray(P, np.float32)
+    out = np.empty(2, np.float32)
+    lib().orc_depth_unprojection(_p(P), _p(out))
+    return out
+
+
+def unproject_depth(unproj, depth: np.ndarray) -> np.ndarray:
+    d = np.ascontiguousarray(depth, np.float32).copy()
+    u = np.ascontiguousarray(unproj, np.float32)
+    lib().orc_unproject_depth(_p(u), _p(d), d.size)
+    return d
+
+
+def environment_grid_size(n: int) -> Tuple[int, int]:
+    """AbstractReplayRenderer::environmentGridSize(), AbstractReplayRenderer.cpp:26-31."""
+    x = int(math.ceil(math.sqrt(np.float32(n))))
+    return x, (n + x - 1) // x
+
+
+def generate_mipmaps(level0: np.ndarray) -> List[np.ndarray]:
+    """Stand-in for glGenerateMipmap() (Renderer.cpp:476-477,526-527; the
+    driver's filter is unspecified): levels down to min(w,h) == 1, each texel
+    the rounded mean of its 2x2 parents, (a+b+c+d+2) >> 2 per channel."""
+    out = [level0]
+    n = int(math.log2(min(level0.shape[1], level0.shape[2]))) + 1
+    cur = level0.astype(np.int64)
+    for _ in range(n - 1):
+        h2, w2 = max(cur.shape[1] // 2, 1), max(cur.shape[2] // 2, 1)
+        nxt = np.empty((cur.shape[0], h2, w2, 4), np.int64)
+        for y in range(h2):
+            for x in range(w2):
+                s = (cur[:, 2 * y, 2 * x] + cur[:, 2 * y + 1, 2 * x]
+                     + cur[:, 2 * y, 2 * x + 1] + cur[:, 2 * y + 1, 2 * x + 1])
+                nxt[:, y, x] = (s + 2) >> 2
+        cur = nxt
+        out.append(cur.astype(np.uint8))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Renderer state machine
+# ---------------------------------------------------------------------------
+
+@dataclass
+class _Draw:
+    batch: int
+    node: int
+    material: int
+    mesh: int
+    index_offset: int  # in indices
+    index_count: int
+    semantic: int
+
+
+@dataclass
+class _Scene:
+    parents: List[int] = field(default_factory=list)
+    transforms: List[np.ndarray] = field(default_factory=list)
+    batches: List[Tuple[bool, int, int]] = field(default_factory=list)
+    draws: List[_Draw] = field(default_factory=list)
+    camera: np.ndarray = field(default_factory=identity)
+    projection: np.ndarray = field(default_factory=identity)
+    unprojection: np.ndarray = field(default_factory=lambda: np.zeros(2, np.float32))
+
+
+class OracleRenderer:
+    """Mirror of esp::gfx_batch::Renderer + RendererStandalone on the CPU."""
+
+    def __init__(self, tile_size=(128, 128), tile_count=(1, 1), flags: int = 0,
+                 max_light_count: int = 0, ambient_factor: float = 0.1):
+        self.tile_size = tuple(tile_size)
+        self.tile_count = tuple(tile_count)
+        self.flags = flags
+        self.max_light_count = max_light_count
+        self.ambient_factor = np.float32(ambient_factor)
+        n = tile_count[0] * tile_count[1]
+        self.scenes = [_Scene() for _ in range(n)]
+        white = dict(width=1, height=1, layers=1,
+                     levels=[np.full((1, 1, 1, 4), 255, np.uint8)],
+                     min_filter=GL_NEAREST, mag_filter=GL_NEAREST,
+                     wrap_s=GL_REPEAT, wrap_t=GL_REPEAT)
+        self.textures: List[dict] = [white]
+        self.meshes: List[dict] = []
+        # material: ambient rgba, (texture id, layer, matrix3)
+        self.materials: List[Tuple[np.ndarray, Tuple[int, int, np.ndarray]]] = [
+            (np.ones(4, np.float32), (0, 0, np.eye(3, dtype=np.float32)))]
+        self.mesh_views: List[Tuple[int, int, int, int, np.ndarray]] = []
+        self.names: Dict[str, Tuple[int, int]] = {}
+        self._files_added = False
+
+    @property
+    def scene_count(self) -> int:
+        return len(self.scenes)
+
+    # -- addFile(), Renderer.cpp:354-867 ------------------------------------
+    def add_file_data(self, fd, file_flags: int = 0, name: Optional[str] = None,
+                      filename: str = "") -> bool:
+        tex_off = len(self.textures)
+        mesh_off = len(self.meshes)
+        view_off = len(self.mesh_views)
+        mat_off = len(self.materials)
+        if not (self.flags & NO_TEXTURES):
+            for t in fd.textures:
+                levels = list(t.levels)
+                if len(levels) == 1 and (file_flags & FILE_GENERATE_MIPMAP) and not t.compressed:
+                    levels = generate_mipmaps(levels[0])
+                self.textures.append(dict(width=t.width, height=t.height, layers=t.layers,
+                                          levels=levels, min_filter=t.min_filter,
+                                          mag_filter=t.mag_filter, wrap_s=t.wrap_s,
+                                          wrap_t=t.wrap_t))
+        for m in fd.meshes:
+            self.meshes.append(dict(data=m, vcolor=m.colors is not None))
+        for mat in fd.materials:
+            base = np.asarray(mat.base_color, np.float32)
+            if mat.flat or not self.max_light_count:
+                ambient = base
+            else:
+                ambient = (base * self.ambient_factor).astype(np.float32)
+            if mat.texture < 0:
+                tt = (0, 0, np.eye(3, dtype=np.float32))
+            else:
+                tt = (mat.texture + tex_off, mat.layer, np.asarray(mat.texture_matrix, np.float32))
+            self.materials.append((ambient, tt))
+        self._files_added = True
+        # addFile() re-creates the per-scene camera matrices (identity),
+        # Renderer.cpp:630-631; the depth unprojection is kept.
+        for sc in self.scenes:
+            sc.camera = identity()
+
+        if not fd.has_scene:
+            if len(fd.meshes) != 1 or fd.materials:
+                return False
+            if not (file_flags & FILE_WHOLE):
+                return False
+            cnt = len(fd.meshes[0].indices)
+            self.mesh_views.append((mesh_off, 0, cnt, 0, identity()))
+            used = name or filename
+            if used in self.names:
+                return False
+            self.names[used] = (view_off, view_off + 1)
+            return True
+
+        entries = fd.mesh_entries
+        has_views = any(e.view is not None for e in entries)
+        views = []
+        for e in entries:
+            mesh_id = e.mesh + mesh_off
+            if has_views:
+                off_b, cnt, mat = e.view
+                off = off_b // 4
+            else:
+                off, cnt, mat = 0, len(fd.meshes[e.mesh].indices), e.material
+            mat = 0 if mat == -1 else mat + mat_off
+            views.append([mesh_id, off, cnt, mat, None])
+        nodes = fd.nodes
+        if not (file_flags & FILE_WHOLE):
+            for v, e in zip(views, entries):
+                v[4] = np.asarray(nodes[e.node].transform, np.float32)
+            self.mesh_views.extend(tuple(v) for v in views)
+            offset = view_off
+            roots = [i for i, n in enumerate(nodes) if n.parent == -1]
+            for r in roots:
+                children = [i for i, n in enumerate(nodes) if n.parent == r]
+                nm = nodes[r].name
+                if not nm or nm in self.names:
+                    return False
+                self.names[nm] = (offset, offset + len(children))
+                offset += len(children)
+            assert offset == len(self.mesh_views)
+        else:
+            absolute: Dict[int, np.ndarray] = {}
+
+            def absf(i):
+                if i not in absolute:
+                    par = nodes[i].parent
+                    pm = identity() if par < 0 else absf(par)
+                    absolute[i] = mat4_mul(pm, nodes[i].transform)
+                return absolute[i]
+
+            for v, e in zip(views, entries):
+                v[4] = absf(e.node)
+            self.mesh_views.extend(tuple(v) for v in views)
+            used = name or filename
+            if used in self.names:
+                return False
+            self.names[used] = (view_off, view_off + len(views))
+        return True
+
+    def add_file(self, path: str, file_flags: int = 0, name: Optional[str] = None) -> bool:
+        from habitat_sim_amd.assets import load_gltf
+
+        return self.add_file_data(load_gltf(path), file_flags, name, path)
+
+    def has_node_hierarchy(self, name: str) -> bool:
+        return name in self.names
+
+    # -- scene population ---------------------------------------------------
+    def _batch_id(self, scene: _Scene, vcolor: bool, mesh: int, tex: int) -> int:
+        for i, b in enumerate(scene.batches):
+            if b == (vcolor, mesh, tex):
+                return i
+        scene.batches.append((vcolor, mesh, tex))
+        return len(scene.batches) - 1
+
+    def add_node_hierarchy(self, scene_id: int, name: str, bake=None,
+                           semantic_id: int = 0) -> int:
+        sc = self.scenes[scene_id]
+        start, end = self.names[name]
+        bake = identity() if bake is None else np.asarray(bake, np.float32)
+        top = len(sc.transforms)
+        sc.parents.append(-1)
+        sc.transforms.append(identity())
+        for i in range(start, end):
+            mesh, off, cnt, mat, xf = self.mesh_views[i]
+            nid = len(sc.transforms)
+            sc.parents.append(top)
+            sc.transforms.append(mat4_mul(bake, xf))
+            tex = self.materials[mat][1][0]
+            b = self._batch_id(sc, self.meshes[mesh]["vcolor"], mesh, tex)
+            sc.draws.append(_Draw(b, nid, mat, mesh, off, cnt, semantic_id))
+        return top
+
+    def add_empty_node(self, scene_id: int) -> int:
+        sc = self.scenes[scene_id]
+        sc.parents.append(-1)
+        sc.transforms.append(identity())
+        return len(sc.transforms) - 1
+
+    def clear(self, scene_id: int) -> None:
+        self.scenes[scene_id] = _Scene(camera=self.scenes[scene_id].camera,
+                                       projection=self.scenes[scene_id].projection,
+                                       unprojection=self.scenes[scene_id].unprojection)
+
+    def update_camera(self, scene_id: int, projection, view) -> None:
+        sc = self.scenes[scene_id]
+        sc.projection = np.asarray(projection, np.float32)
+        sc.camera = mat4_mul(projection, view)
+        sc.unprojection = depth_unprojection(projection)
+
+    def camera(self, scene_id: int) -> np.ndarray:
+        return self.scenes[scene_id].camera
+
+    def camera_depth_unprojection(self, scene_id: int) -> np.ndarray:
+        return self.scenes[scene_id].unprojection
+
+    def transformations(self, scene_id: int) -> List[np.ndarray]:
+        return self.scenes[scene_id].transforms
+
+    def scene_stats(self, scene_id: int) -> Tuple[int, int, int]:
+        sc = self.scenes[scene_id]
+        return len(sc.transforms), len(sc.draws), len(sc.batches)
+
+    def sorted_draws(self, scene_id: int) -> List[_Draw]:
+        """Stable counting sort by batch id, Renderer.cpp:1151-1188."""
+        sc = self.scenes[scene_id]
+        out: List[_Draw] = []
+        for b in range(len(sc.batches)):
+            out.extend(d for d in sc.draws if d.batch == b)
+        return out
+
+    # -- draw ----------------------------------------------------------------
+    def draw(self, nthreads: int = 1) -> None:
+        """RendererStandalone::draw(): render all scenes into the framebuffer."""
+        self.planes = self.render(nthreads)
+
+    def color_image(self) -> np.ndarray:
+        return self.framebuffer(self.planes, "color")
+
+    def depth_image(self) -> np.ndarray:
+        return self.framebuffer(self.planes, "depth")
+
+    def linear_depth_image(self) -> np.ndarray:
+        return self.framebuffer(self.planes, "lindepth")
+
+    def semantic_image(self) -> np.ndarray:
+        return self.framebuffer(self.planes, "semantic")
+
+    def render(self, nthreads: int = 1, want_ids: bool = False, want_count: bool = False,
+               scenes: Optional[List[int]] = None) -> Dict[str, np.ndarray]:
+        """Render every scene (or the listed ones).  Returns env-major planes
+        of shape (n, th, tw[, 4]) with rows bottom-up."""
+        tw, th = self.tile_size
+        ids = list(range(self.scene_count)) if scenes is None else list(scenes)
+        job = build_job(self, ids)
+        n = len(ids)
+        out = dict(color=np.empty((n, th, tw, 4), np.uint8),
+                   depth=np.empty((n, th, tw), np.float32),
+                   lindepth=np.empty((n, th, tw), np.float32),
+                   semantic=np.empty((n, th, tw), np.uint32))
+        if want_count:
+            out["count"] = np.empty((n, th, tw), np.uint32)
+        if want_ids:
+            out["draw"] = np.empty((n, th, tw), np.int32)
+            out["tri"] = np.empty((n, th, tw), np.int32)
+        run_job(job, out, nthreads)
+        return out
+
+    def framebuffer(self, planes: Dict[str, np.ndarray], key: str = "color") -> np.ndarray:
+        """Assemble env-major planes into the tile-grid framebuffer layout
+        (rows bottom-up), RendererStandalone::colorImage() equivalent."""
+        tw, th = self.tile_size
+        tx, ty = self.tile_count
+        p = planes[key]
+        fb = np.empty((ty * th, tx * tw) + p.shape[3:], p.dtype)
+        for s in range(self.scene_count):
+            x, y = s % tx, s // tx
+            fb[y * th:(y + 1) * th, x * tw:(x + 1) * tw] = p[s]
+        return fb
+
+
+# ---------------------------------------------------------------------------
+# Job construction
+# ---------------------------------------------------------------------------
+
+class _Built:
+    pass
+
+
+def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
+    tw, th = r.tile_size
+    b = _Built()
+    # geometry: every mesh, triangles with absolute vertex indices
+    pos, uv, col, tris = [], [], [], []
+    vbase, tbase = [], []
+    nv = nt = 0
+    for m in r.meshes:
+        d = m["data"]
+        v = len(d.positions)
+        pos.append(np.asarray(d.positions, np.float32))
+        if d.texcoords is not None:
+            t = np.asarray(d.texcoords, np.float32)
+            if d.texcoord_half:
+                t = t.astype(np.float16).astype(np.float32)
+            uv.append(t)
+        else:
+            uv.append(np.zeros((v, 2), np.float32))
+        if d.colors is not None:
+            c = d.colors
+            if c.dtype == np.uint8:
+                c = (c.astype(np.float32) / np.float32(255.0)).astype(np.float32)
+            col.append(np.asarray(c, np.float32))
+        else:
+            col.append(np.ones((v, 4), np.float32))
+        tris.append(np.asarray(d.indices, np.uint32) + np.uint32(nv))
+        vbase.append(nv)
+        tbase.append(nt)
+        nv += v
+        nt += len(d.indices) // 3
+    b.pos = np.ascontiguousarray(np.concatenate(pos) if pos else np.zeros((1, 3), np.float32))
+    b.uv = np.ascontiguousarray(np.concatenate(uv) if uv else np.zeros((1, 2), np.float32))
+    b.col = np.ascontiguousarray(np.concatenate(col) if col else np.ones((1, 4), np.float32))
+    b.tris = np.ascontiguousarray(np.concatenate(tris) if tris else np.zeros(3, np.uint32))
+
+    # textures
+    desc, offs, blobs = [], [], []
+    off = 0
+    for t in r.textures:
+        desc.append([t["width"], t["height"], t["layers"], len(t["levels"]), t["min_filter"],
+                     t["mag_filter"], t["wrap_s"], t["wrap_t"]])
+        lo = np.zeros(16, np.int64)
+        for i, lv in enumerate(t["levels"]):
+            lo[i] = off
+            blobs.append(np.ascontiguousarray(lv, np.uint8).reshape(-1))
+            off += lv.size
+        offs.append(lo)
+    b.tex_desc = np.ascontiguousarray(np.array(desc, np.int32))
+    b.tex_off = np.ascontiguousarray(np.array(offs, np.int64))
+    b.texels = np.ascontiguousarray(np.concatenate(blobs))
+
+    # per scene nodes and sorted draws
+    di, df, npar, nloc, ei, ef = [], [], [], [], [], []
+    node_start = draw_start = 0
+    pix = 0
+    for s in scene_ids:
+        sc = r.scenes[s]
+        sd = r.sorted_draws(s)
+        for d in sd:
+            mesh = r.meshes[d.mesh]
+            amb, (tex, layer, tm) = r.materials[d.material]
+            assert d.index_offset % 3 == 0 and d.index_count % 3 == 0
+            di.append([tbase[d.mesh] + d.index_offset // 3, d.index_count // 3, d.node,
+                       tex, layer, d.semantic, 1 if mesh["vcolor"] else 0, 0])
+            f = np.zeros(16, np.float32)
+            f[:4] = amb
+            f[4:13] = np.asarray(tm, np.float32).reshape(-1)
+            df.append(f)
+        npar.extend(sc.parents)
+        nloc.extend(np.asarray(t, np.float32).reshape(16) for t in sc.transforms)
+        ei.append([tw, th, node_start, len(sc.parents), draw_start, len(sd),
+                   NO_TEXTURES if (r.flags & NO_TEXTURES) else 0, pix])
+        e = np.zeros(32, np.float32)
+        e[:16] = sc.camera.reshape(-1)
+        e[16:] = sc.projection.reshape(-1)
+        ef.append(e)
+        node_start += len(sc.parents)
+        draw_start += len(sd)
+        pix += tw * th
+    b.draw_i = np.ascontiguousarray(np.array(di, np.int32).reshape(-1, 8) if di else np.zeros((1, 8), np.int32))
+    b.draw_f = np.ascontiguousarray(np.array(df, np.float32).reshape(-1, 16) if df else np.zeros((1, 16), np.float32))
+    b.node_parent = np.ascontiguousarray(np.array(npar, np.int32) if npar else np.zeros(1, np.int32))
+    b.node_local = np.ascontiguousarray(np.array(nloc, np.float32).reshape(-1, 16) if nloc else np.zeros((1, 16), np.float32))
+    b.env_i = np.ascontiguousarray(np.array(ei, np.int32).reshape(-1, 8))
+    b.env_f = np.ascontiguousarray(np.array(ef, np.float32).reshape(-1, 32))
+    b.nenv = len(scene_ids)
+    return b
+
+
+def run_job(b: _Built, out: Dict[str, np.ndarray], nthreads: int = 1) -> None:
+    j = _Job()
+    j.pos, j.uv, j.col, j.tris = _p(b.pos), _p(b.uv), _p(b.col), _p(b.tris)
+    j.ntex = len(b.tex_desc)
+    j.tex_desc, j.tex_level_off, j.texels = _p(b.tex_desc), _p(b.tex_off), _p(b.texels)
+    j.draw_i, j.draw_f = _p(b.draw_i), _p(b.draw_f)
+    j.node_parent, j.node_local = _p(b.node_parent), _p(b.node_local)
+    j.nenv, j.env_i, j.env_f = b.nenv, _p(b.env_i), _p(b.env_f)
+    j.out_color = _p(out.get("color"))
+    j.out_depth = _p(out.get("depth"))
+    j.out_lindepth = _p(out.get("lindepth"))
+    j.out_semantic = _p(out.get("semantic"))
+    j.out_count = _p(out.get("count"))
+    j.out_draw = _p(out.get("draw"))
+    j.out_tri = _p(out.get("tri"))
+    lib().orc_render(ctypes.byref(j), int(nthreads))

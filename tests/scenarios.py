@@ -1,0 +1,322 @@
+"""Restatements of habitat-sim's src/tests/GfxBatchRendererTest.cpp render
+cases, written against the renderer interface shared by the product
+(habitat_sim_amd.RendererStandalone) and the oracle (oracle.OracleRenderer):
+
+    add_file(path, flags=0, name=None) -> bool, has_node_hierarchy(name),
+    add_node_hierarchy(scene, name, bake=None, semantic_id=0) -> id,
+    add_empty_node(scene), clear(scene), update_camera(scene, P, V),
+    transformations(scene) (indexable, assignable), scene_stats(scene),
+    draw(), color_image(), depth_image()
+
+Each function takes ``make(tile_size, tile_count, flags=0)`` returning a fresh
+renderer, performs the reference test's calls and returns a list of checks
+(name, actual, expected[, tolerance]) plus the images to compare against the
+PNG goldens.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+from habitat_sim_amd import math3d as m3
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SCENES = os.path.join(GOLDEN, "scenes")
+SHOTS = os.path.join(GOLDEN, "screenshots")
+
+WHOLE = 1
+GENERATE_MIPMAP = 2
+NO_TEXTURES = 1
+
+# FileData, GfxBatchRendererTest.cpp:88-164: files, single-scene batch count,
+# multiple-scenes batch counts, (max, mean) image thresholds
+FILE_DATA = {
+    "": ([("batch.gltf", 0, None)], 1, (1, 1, 0, 1), (0.0, 0.0)),
+    "generate mipmap": ([("batch.gltf", GENERATE_MIPMAP, None)], 1, (1, 1, 0, 1), (0.0, 0.0)),
+    "multiple meshes": ([("batch-multiple-meshes.gltf", 0, None)], 3, (4, 2, 0, 1), (0.0, 0.0)),
+    "multiple textures": ([("batch-multiple-textures.gltf", 0, None)], 3, (4, 2, 0, 1), (0.0, 0.0)),
+    "multiple textures, generate mipmap": (
+        [("batch-multiple-textures.gltf", GENERATE_MIPMAP, None)], 3, (4, 2, 0, 1), (0.0, 0.0)),
+    "multiple files": ([("batch-square-circle-triangle.gltf", 0, None),
+                        ("batch-four-squares.gltf", 0, None)], 2, (4, 2, 0, 1), (0.0, 0.0)),
+    "multiple files, compressed textures": (
+        [("batch-square-circle-triangle-compressed.gltf", 0, None),
+         ("batch-four-squares-compressed.gltf", 0, None)], 2, (4, 2, 0, 1), (1.5, 0.5)),
+    "multiple files, compressed textures, generate mipmap": (
+        [("batch-square-circle-triangle-compressed.gltf", GENERATE_MIPMAP, None),
+         ("batch-four-squares-compressed.gltf", GENERATE_MIPMAP, None)], 2, (4, 2, 0, 1), (1.5, 0.5)),
+    "multiple files, some whole-file": (
+        [("batch-square-circle-triangle.gltf", 0, None),
+         ("batch-four-squares-deep-hierarchy-whole-file.gltf", WHOLE, "four squares")],
+        2, (4, 2, 0, 1), (0.0, 0.0)),
+}
+
+# MeshHierarchyData, GfxBatchRendererTest.cpp:166-220
+MESH_HIERARCHY_DATA = {
+    "": ([("batch.gltf", 0, None)], 0, (5, 4, 1), 0xcc / 255.0, "GfxBatchRendererTestMeshHierarchy.png"),
+    "multiple meshes": ([("batch-multiple-meshes.gltf", 0, None)], 0, (5, 4, 4), 0xcc / 255.0,
+                        "GfxBatchRendererTestMeshHierarchy.png"),
+    "multiple textures": ([("batch-multiple-textures.gltf", 0, None)], 0, (5, 4, 4), 0xcc / 255.0,
+                          "GfxBatchRendererTestMeshHierarchy.png"),
+    "multiple files": ([("batch-square-circle-triangle.gltf", 0, None),
+                        ("batch-four-squares.gltf", 0, None)], 0, (5, 4, 4), 0xcc / 255.0,
+                       "GfxBatchRendererTestMeshHierarchy.png"),
+    "multiple files with deep hierarchy as a whole": (
+        [("batch-square-circle-triangle.gltf", WHOLE, None),
+         ("batch-four-squares-deep-hierarchy-whole-file.gltf", WHOLE, "four squares")],
+        0, (5, 4, 4), 0xcc / 255.0, "GfxBatchRendererTestMeshHierarchy.png"),
+    "multiple files, no materials": (
+        [("batch-square-circle-triangle.gltf", 0, None),
+         ("batch-four-squares-no-materials.gltf", 0, None)], 0, (5, 4, 4), 1.0,
+        "GfxBatchRendererTestMeshHierarchyNoTextures.png"),
+    "no textures": ([("batch.gltf", 0, None)], NO_TEXTURES, (5, 4, 1), 1.0,
+                    "GfxBatchRendererTestMeshHierarchyNoTextures.png"),
+    "multiple meshes, no textures": ([("batch-multiple-meshes.gltf", 0, None)], NO_TEXTURES,
+                                     (5, 4, 4), 1.0, "GfxBatchRendererTestMeshHierarchyNoTextures.png"),
+    "multiple files, scene-less, as a whole": (
+        [("batch-square-circle-triangle.gltf", WHOLE, None),
+         ("batch-four-squares-no-scene.gltf", WHOLE, "four squares")], 0, (2, 1, 1), 1.0,
+        "GfxBatchRendererTestMeshHierarchyNoTextures.png"),
+}
+
+
+def load_png(name: str) -> np.ndarray:
+    """Golden PNG -> (h, w, 4) uint8 with rows bottom-up (Magnum order)."""
+    from PIL import Image
+
+    a = np.asarray(Image.open(os.path.join(SHOTS, name)).convert("RGBA"), np.uint8)
+    return np.ascontiguousarray(a[::-1])
+
+
+def image_delta(actual: np.ndarray, expected: np.ndarray):
+    """Magnum DebugTools::CompareImage statistic: per-pixel delta is the mean
+    absolute channel difference; returns (max, mean) over pixels."""
+    d = np.abs(actual.astype(np.float64) - expected.astype(np.float64)).mean(axis=-1)
+    return float(d.max()), float(d.mean())
+
+
+def add_files(r, files):
+    for fname, flags, name in files:
+        ok = r.add_file(os.path.join(SCENES, fname), flags, name)
+        assert ok, f"add_file({fname}) failed"
+
+
+def ortho_4_3():
+    return (m3.orthographic_projection((2.0 * 4.0 / 3.0, 2.0), 0.1, 10.0),
+            m3.inverted(m3.translation((0.0, 0.0, 1.0))))
+
+
+def defaults(make, files):
+    """GfxBatchRendererTest::defaults(), :1459-1519."""
+    r = make((48, 32), (2, 3))
+    checks = [("scene_count", r.scene_count, 6)]
+    for i in range(6):
+        checks.append((f"camera{i}", np.asarray(r.camera(i)), m3.identity()))
+        checks.append((f"stats{i}", tuple(r.scene_stats(i)), (0, 0, 0)))
+    add_files(r, files)
+    r.draw()
+    color = r.color_image()
+    depth = r.depth_image()
+    checks += [("color.shape", color.shape, (96, 96, 4)),
+               ("color[48][48]", tuple(color[48, 48]), (0x1f, 0x1f, 0x1f, 0xff)),
+               ("depth[48][48]", float(depth[48, 48]), 1.0)]
+    return checks, [(color, "GfxBatchRendererTestDefaults.png", (0.0, 0.0))]
+
+
+def single_mesh(make, files, thresholds):
+    """GfxBatchRendererTest::singleMesh(), :1521-1585."""
+    r = make((128, 96), (1, 1))
+    add_files(r, files)
+    P, V = ortho_4_3()
+    r.update_camera(0, P, V)
+    checks = [("has square", r.has_node_hierarchy("square"), True),
+              ("has squares", r.has_node_hierarchy("squares"), False),
+              ("has ''", r.has_node_hierarchy(""), False)]
+    checks.append(("add square", r.add_node_hierarchy(0, "square", m3.scaling(0.4)), 0))
+    checks.append(("stats", tuple(r.scene_stats(0)), (2, 1, 1)))
+    t = r.transformations(0)
+    checks.append(("len(transformations)", len(t), 2))
+    checks.append(("transformations[0]", np.asarray(t[0]), m3.identity()))
+    t[0] = m3.scaling(2.0)
+    r.draw()
+    color = r.color_image()
+    depth = r.depth_image()
+    checks.append(("color.shape", color.shape, (96, 128, 4)))
+    if thresholds[0] == 0.0:
+        checks.append(("color[75][35]", tuple(color[75, 35]), (0xcc, 0xcc, 0xcc, 0xff)))
+        checks.append(("color[20][38]", tuple(color[20, 38]), (0x99, 0x00, 0x00, 0xff)))
+    checks += [("depth[0][0]", float(depth[0, 0]), 1.0),
+               ("depth[95][127]", float(depth[95, 127]), 1.0),
+               ("depth[64][48]", float(depth[64, 48]), 0.0909091, 1e-6)]
+    return checks, [(color, "GfxBatchRendererTestSingleMesh.png", thresholds)]
+
+
+def mesh_hierarchy(make, files, flags, stats, mult, golden):
+    """GfxBatchRendererTest::meshHierarchy(), :1587-1660."""
+    r = make((128, 96), (1, 1), flags)
+    add_files(r, files)
+    P, V = ortho_4_3()
+    r.update_camera(0, P, V)
+    checks = [("add four squares", r.add_node_hierarchy(0, "four squares"), 0),
+              ("stats", tuple(r.scene_stats(0)), stats)]
+    t = r.transformations(0)
+    checks.append(("len(transformations)", len(t), stats[0]))
+    t[0] = m3.scaling(0.8)
+    r.draw()
+    color = r.color_image()
+
+    def scaled(rgb):
+        return tuple(int(np.float32(c / 255.0) * np.float32(mult) * 255.0 + 0.5) for c in rgb)
+
+    checks += [("color[29][87]", tuple(color[29, 87][:3]), scaled((0x00, 0xff, 0xff))),
+               ("color[68][50]", tuple(color[68, 50][:3]), scaled((0xff, 0x00, 0xff))),
+               ("color[67][78]", tuple(color[67, 78][:3]), scaled((0xff, 0xff, 0x00)))]
+    return checks, [(color, golden, (0.0, 0.0))]
+
+
+def multiple_meshes(make, files, batch_count, thresholds):
+    """GfxBatchRendererTest::multipleMeshes(), :1662-1718."""
+    r = make((128, 96), (1, 1))
+    add_files(r, files)
+    P, V = ortho_4_3()
+    r.update_camera(0, P, V)
+    checks = [("add square", r.add_node_hierarchy(0, "square"), 0)]
+    r.transformations(0)[0] = m3.mul(m3.translation((0.0, 0.5, 0.0)), m3.scaling(0.4))
+    checks.append(("add circle", r.add_node_hierarchy(0, "circle"), 2))
+    r.transformations(0)[2] = m3.mul(m3.translation((-0.5, -0.5, 0.0)), m3.scaling(0.4))
+    checks.append(("add triangle", r.add_node_hierarchy(0, "triangle"), 4))
+    r.transformations(0)[4] = m3.mul(m3.translation((0.5, -0.5, 0.0)), m3.scaling(0.4))
+    checks.append(("stats", tuple(r.scene_stats(0)), (6, 3, batch_count)))
+    r.draw()
+    color = r.color_image()
+    checks += [("color[18][44]", tuple(color[18, 44][:3]), (0x00, 0xcc, 0xcc)),
+               ("color[24][88]", tuple(color[24, 88][:3]), (0xcc, 0x00, 0xcc))]
+    return checks, [(color, "GfxBatchRendererTestMultipleMeshes.png", thresholds)]
+
+
+def _populate_multiple(r, batch_counts):
+    ident = m3.identity()
+    checks = [("add0 four squares", r.add_node_hierarchy(0, "four squares"), 0),
+              ("add1 circle", r.add_node_hierarchy(1, "circle"), 0),
+              ("add1 square", r.add_node_hierarchy(1, "square"), 2),
+              ("add3 triangle", r.add_node_hierarchy(3, "triangle"), 0)]
+    checks += [("stats0", tuple(r.scene_stats(0)), (5, 4, batch_counts[0])),
+               ("stats1", tuple(r.scene_stats(1)), (4, 2, batch_counts[1])),
+               ("stats2", tuple(r.scene_stats(2)), (0, 0, batch_counts[2])),
+               ("stats3", tuple(r.scene_stats(3)), (2, 1, batch_counts[3]))]
+    r.update_camera(0, ident, m3.inverted(m3.translation((0.0, 0.0, 1.0))))
+    r.transformations(0)[0] = m3.translation((0.0, 0.0, 0.0))
+    r.update_camera(1, ident, m3.inverted(m3.translation((0.0, 0.5, 1.0))))
+    r.transformations(1)[0] = m3.mul(m3.translation((0.5, 0.0, 0.0)), m3.scaling(0.5))
+    r.transformations(1)[2] = m3.mul(m3.translation((-0.5, 1.0, 0.0)), m3.scaling(0.5))
+    r.update_camera(3, ident, m3.inverted(m3.translation((0.0, -0.5, 1.0))))
+    r.transformations(3)[0] = m3.mul(m3.translation((0.5, 0.0, 0.0)), m3.scaling(0.5))
+    return checks
+
+
+def multiple_scenes(make, files, batch_counts, thresholds):
+    """GfxBatchRendererTest::multipleScenes(), :1734-1823."""
+    r = make((64, 48), (2, 2))
+    add_files(r, files)
+    r.update_camera(0, m3.orthographic_projection((2.0, 2.0 * 4.0 / 3.0), 0.1, 10.0),
+                    m3.inverted(m3.translation((0.0, 0.0, 1.0))))
+    checks = _populate_multiple(r, batch_counts)
+    r.draw()
+    return checks, [(r.color_image(), "GfxBatchRendererTestMultipleScenes.png", thresholds)]
+
+
+def clear_scene(make, files, thresholds):
+    """GfxBatchRendererTest::clearScene(), :1825-1904."""
+    r = make((64, 48), (2, 2))
+    add_files(r, files)
+    r.update_camera(0, m3.orthographic_projection((2.0, 2.0 * 4.0 / 3.0), 0.1, 10.0),
+                    m3.inverted(m3.translation((0.0, 0.0, 1.0))))
+    checks = [("add3 triangle", r.add_node_hierarchy(3, "triangle"), 0),
+              ("add1 square", r.add_node_hierarchy(1, "square"), 0),
+              ("add1 circle", r.add_node_hierarchy(1, "circle"), 2),
+              ("add1 triangle", r.add_node_hierarchy(1, "triangle"), 4),
+              ("add0 four squares", r.add_node_hierarchy(0, "four squares"), 0)]
+    ident = m3.identity()
+    r.update_camera(0, ident, m3.inverted(m3.translation((0.0, 0.0, 1.0))))
+    r.transformations(0)[0] = m3.translation((0.0, 0.0, 0.0))
+    r.update_camera(1, ident, m3.inverted(m3.translation((0.0, 0.5, 1.0))))
+    r.update_camera(3, ident, m3.inverted(m3.translation((0.0, -0.5, 1.0))))
+    r.transformations(3)[0] = m3.mul(m3.translation((0.5, 0.0, 0.0)), m3.scaling(0.5))
+    r.clear(1)
+    checks.append(("stats1 after clear", tuple(r.scene_stats(1))[:2], (0, 0)))
+    r.draw()
+    images = [(r.color_image(), "GfxBatchRendererTestMultipleScenesClearScene1.png", thresholds)]
+    checks += [("re-add1 circle", r.add_node_hierarchy(1, "circle"), 0),
+               ("re-add1 square", r.add_node_hierarchy(1, "square"), 2)]
+    r.transformations(1)[0] = m3.mul(m3.translation((0.5, 0.0, 0.0)), m3.scaling(0.5))
+    r.transformations(1)[2] = m3.mul(m3.translation((-0.5, 1.0, 0.0)), m3.scaling(0.5))
+    r.draw()
+    images.append((r.color_image(), "GfxBatchRendererTestMultipleScenes.png", thresholds))
+    return checks, images
+
+
+def lights_disabled(make):
+    """GfxBatchRendererTest::lights() instance "no lights enabled", :1906-1972:
+    maxLightCount 0 makes every material flat; perspective camera, textured
+    UV spheres with nearest sampling (fuzzy thresholds 0.75 / 0.005)."""
+    r = make((128, 96), (1, 1))
+    add_files(r, [("batch.gltf", 0, None)])
+    r.update_camera(0, m3.perspective_projection(math.radians(60.0), 4.0 / 3.0, 0.1, 10.0),
+                    m3.inverted(m3.translation((0.0, 0.0, 5.0))))
+    checks = [("add first", r.add_node_hierarchy(0, "flat checkerboard sphere"), 0),
+              ("add second", r.add_node_hierarchy(0, "shaded yellow sphere"), 2),
+              ("add third", r.add_node_hierarchy(0, "shaded checkerboard sphere"), 4)]
+    r.transformations(0)[0] = m3.translation((-1.5, -1.0, 0.0))
+    r.transformations(0)[2] = m3.translation((0.0, 1.0, 0.0))
+    r.transformations(0)[4] = m3.translation((1.5, -1.0, 0.0))
+    checks.append(("stats", tuple(r.scene_stats(0)), (6, 3, 2)))
+    r.draw()
+    return checks, [(r.color_image(), "GfxBatchRendererTestLightsDisabled.png", (0.75, 0.005))]
+
+
+def depth_unprojection(make, unproject):
+    """GfxBatchRendererTest::depthUnprojection(), :2175-2246."""
+    r = make((64, 64), (2, 2))
+    add_files(r, [("batch.gltf", 0, None)])
+    P = m3.perspective_projection(math.radians(60.0), 1.0, 0.001, 10.0)
+    for i, z in enumerate((2.5, 5.0, 7.5, 20.0)):
+        r.update_camera(i, P, m3.inverted(m3.translation((0.0, 0.0, z))))
+    checks = [("has square", r.has_node_hierarchy("square"), True)]
+    for i in range(4):
+        checks.append((f"add{i}", r.add_node_hierarchy(i, "square"), 0))
+    r.draw()
+    depth = np.array(r.depth_image())
+    for y in range(2):
+        for x in range(2):
+            s = y * 2 + x
+            tile = depth[y * 64:(y + 1) * 64, x * 64:(x + 1) * 64]
+            depth[y * 64:(y + 1) * 64, x * 64:(x + 1) * 64] = unproject(
+                r.camera_depth_unprojection(s), tile)
+    checks += [("d[32][32]", float(depth[32, 32]), 2.5, 0.01),
+               ("d[32][96]", float(depth[32, 96]), 5.0, 0.01),
+               ("d[96][32]", float(depth[96, 32]), 7.5, 0.01),
+               ("d[96][96]", float(depth[96, 96]), 0.0)]
+    return checks, []
+
+
+def assert_checks(checks):
+    for c in checks:
+        name, actual, expected = c[:3]
+        if len(c) > 3:
+            assert abs(actual - expected) <= c[3], f"{name}: {actual} vs {expected}"
+        elif isinstance(expected, np.ndarray):
+            np.testing.assert_array_equal(np.asarray(actual), expected, err_msg=name)
+        else:
+            assert actual == expected, f"{name}: {actual!r} != {expected!r}"
+
+
+def assert_images(images):
+    for img, golden, (tmax, tmean) in images:
+        exp = load_png(golden)
+        assert img.shape == exp.shape, f"{golden}: shape {img.shape} vs {exp.shape}"
+        dmax, dmean = image_delta(img, exp)
+        assert dmax <= tmax and dmean <= tmean, (
+            f"{golden}: max delta {dmax} (<= {tmax}), mean {dmean} (<= {tmean}); "
+            f"{int((np.abs(img.astype(int) - exp.astype(int)).max(-1) > 0).sum())} pixels differ")
