@@ -1,0 +1,225 @@
+"""Benchmark: env-frames/s of the batch renderer on the 512-environment
+apartment workload (128x128 RGB-D + semantic per env, ~200k triangles per
+scene, 20 shared textured templates, trilinear sampling).
+
+A step = one draw() of all environments after a per-step camera update (a
+seeded random walk), with every input already resident in HBM; outputs stay
+in HBM (colour RGBA8, linear depth FP32, semantic uint32 planes).
+
+Prints ONE JSON line (rank 0).  ``roofline`` uses the algorithmic bytes of a
+frame,
+
+  B_frame = sum_env [ sum_inst (12 T + 16 V + 64) + sum_sensor W H 4 + 8 W H ]
+
+(T triangles and V vertices of each drawn instance; 3 sensors), divided by
+the frame's GPU time measured with HIP events recorded on the renderer's own
+stream.  ``cpu_baseline`` times the CPU oracle (the project's C restatement of
+the GL pipeline, OpenMP over environments) on a bounded sample of the same
+workload on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "habitat-sim_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "frames/s @128x128 RGB-D, 512 envs; learned-cull vs plain speedup at 0 px diff"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(spec, fd, sensors=3) -> float:
+    tv = {}
+    for name, (lo, hi) in _template_ranges(fd).items():
+        t = v = 0
+        for mi in range(lo, hi):
+            m = fd.meshes[fd.mesh_entries[mi].mesh]
+            t += len(m.indices) // 3
+            v += len(m.positions)
+        tv[name] = (t, v)
+    w, h = spec.tile
+    total = 0.0
+    for env in spec.envs:
+        for inst in env.instances:
+            t, v = tv[inst.template]
+            total += 12 * t + 16 * v + 64
+        total += sensors * w * h * 4 + 8 * w * h
+    return total
+
+
+def _template_ranges(fd):
+    """template name -> range of mesh entries (children of the named root)."""
+    out = {}
+    for i, n in enumerate(fd.nodes):
+        if n.parent == -1 and n.name:
+            kids = [k for k, e in enumerate(fd.mesh_entries) if fd.nodes[e.node].parent == i]
+            out[n.name] = (min(kids), max(kids) + 1) if kids else (0, 0)
+    return out
+
+
+def camera_walk(spec, steps, seed=7):
+    """Per step and env a view matrix: the env's pose turned by a seeded
+    random yaw walk (10 degree steps) and moved 0.25 m forward, clamped to the
+    apartment."""
+    import numpy as np
+    from habitat_sim_amd import math3d as m3
+
+    rng = np.random.default_rng(seed)
+    n = len(spec.envs)
+    poses = [env.camera_pose.copy() for env in spec.envs]
+    views = np.empty((steps, n, 4, 4), np.float32)
+    for s in range(steps):
+        acts = rng.integers(0, 3, n)
+        for e in range(n):
+            p = poses[e]
+            if acts[e] == 1:
+                p = m3.mul(p, m3.rotation_y(math.radians(10)))
+            elif acts[e] == 2:
+                p = m3.mul(p, m3.rotation_y(math.radians(-10)))
+            else:
+                fwd = -p[2, :3]
+                pos = np.clip(p[3, :3] + 0.25 * fwd, 0.3, 11.7)
+                p = p.copy()
+                p[3, :3] = pos.astype(np.float32)
+                p[3, 1] = 1.5
+            poses[e] = p
+            views[s, e] = m3.inverted_rigid(p)
+    return views
+
+
+def cpu_baseline(spec, seconds=12.0, max_seconds=30.0):
+    """Oracle env-frames/s on a sample of the workload on this host."""
+    import numpy as np
+    from habitat_sim_amd import math3d as m3
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    sample = list(range(min(64, len(spec.envs))))
+    o = O.OracleRenderer(spec.tile, spec.grid)
+    o.add_file_data(spec.file)
+    for e in sample:
+        env = spec.envs[e]
+        for inst in env.instances:
+            o.add_node_hierarchy(e, inst.template, inst.bake, inst.semantic)
+        o.update_camera(e, env.projection, m3.inverted_rigid(env.camera_pose))
+    job = O.build_job(o, sample)
+    w, h = spec.tile
+    n = len(sample)
+    out = dict(color=np.empty((n, h, w, 4), np.uint8), depth=np.empty((n, h, w), np.float32),
+               lindepth=np.empty((n, h, w), np.float32), semantic=np.empty((n, h, w), np.uint32))
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        O.run_job(job, out, threads)
+        frames += n
+        el = time.perf_counter() - t0
+        if el >= seconds or el >= max_seconds:
+            break
+    return {"value": frames / el, "unit": "env-frames/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of the {len(spec.envs)} apartment envs (~{spec.triangles(0) // 1000}k "
+                      f"tris each), {frames // n} passes, {el:.1f} s, all planes"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--envs", type=int, default=512)
+    ap.add_argument("--res", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "environments shard across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__
+    __graft_entry__.ensure_built()
+    import numpy as np
+    from habitat_sim_amd import RendererConfiguration, RendererStandalone, hip, scenes
+    from habitat_sim_amd import _native as N
+
+    spec = scenes.apartments(n_envs=args.envs, tile=args.res)
+    gx, gy = spec.grid
+    cfg = RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy))
+    planes = N.PLANE_COLOR | N.PLANE_LINEAR_DEPTH | N.PLANE_SEMANTIC
+    r = RendererStandalone(cfg, planes=planes, top_down=True)
+    scenes.populate(r, spec)
+    views = camera_walk(spec, args.warmup + args.steps)
+    proj = np.stack([env.projection for env in spec.envs]).astype(np.float32)
+    n = len(spec.envs)
+
+    stream = r.stream()
+    for k in range(args.warmup):
+        r.update_cameras(0, proj, views[k])
+        r.draw()
+    r.synchronize()
+    ev = [(hip.Event(), hip.Event()) for _ in range(args.steps)]
+    hip.hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r.update_cameras(0, proj, views[args.warmup + k])
+        ev[k][0].record(stream)
+        r.draw()
+        ev[k][1].record(stream)
+    r.synchronize()
+    hip.hip().hipDeviceSynchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    gpu_ms = [a.elapsed_ms(b) for a, b in ev]
+    frame_ms = sum(gpu_ms) / len(gpu_ms)
+    stats = r.frame_stats()
+
+    bytes_frame = algorithmic_bytes(spec, spec.file)
+    achieved = bytes_frame / (frame_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_apartments512.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_frame")
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(spec, args.cpu_seconds)
+    value = n * args.steps / wall
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "env-frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"apartments-{n}env-{args.res}x{args.res}-rgbd-semantic",
+                   "envs": n, "resolution": [args.res, args.res],
+                   "triangles_per_frame": int(stats["triangles"]),
+                   "sensors": ["rgba8", "linear_depth_f32", "semantic_u32"],
+                   "cull": "none", "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "frame (k_nodes + k_setup_bin + k_raster)",
+                     "algorithmic_bytes_per_frame": bytes_frame,
+                     "gpu_ms_per_frame": frame_ms},
+        "cpu_baseline": cpu,
+        "batch_steps_per_s": value / n,
+        "learned_cull_speedup": None,
+        "frame_stats": stats,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line))
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+// Device-side data layout of the batch renderer (shared by the host code and
+// the HIP kernels).  All structs are POD, 16-byte aligned where they are read
+// as vectors.
+#pragma once
+
+#include <stdint.h>
+
+namespace gbr {
+
+constexpr int kBlock = 256;        // threads per workgroup, all kernels
+constexpr int kMaxBin = 32;        // largest bin edge (LDS visibility tile)
+constexpr int kVertexCache = 1024; // vertices per setup work item held in LDS
+constexpr int kBigQueue = 192;     // cooperative-raster queue entries per bin
+constexpr int kDrawCache = 1024;   // draw prim bases cached in LDS per bin
+constexpr float kSubpix = 256.0f;  // 8 sub-pixel bits
+constexpr float kGuardPx = 8192.0f;
+
+// per-vertex attribute formats
+enum : uint32_t {
+  kColorNone = 0,
+  kColorU8 = 1,   // RGBA8 in col8[]
+  kColorF32 = 2,  // float4 in col32[]
+  kUvF32 = 0,     // float2 in uv32[]
+  kUvF16 = 1      // half2 in uv16[]
+};
+
+// draw flags
+enum : uint32_t {
+  kDrawColorShift = 0,  // 2 bits: color format
+  kDrawUvHalf = 1u << 2,
+};
+
+struct alignas(16) DrawDesc {
+  uint32_t index_base;  // first index of the mesh view in indices[]
+  uint32_t tri_count;
+  uint32_t node;        // global node index (absolute transform)
+  uint32_t material;
+  uint32_t semantic;
+  uint32_t flags;       // color format | uv format
+  uint32_t prim_base;   // scene-local primitive order of the first triangle
+  uint32_t pad;
+};
+
+struct alignas(16) SceneDesc {
+  int32_t width, height;
+  int32_t bins_x, bins_y;
+  uint32_t bin_base;    // first bin of this scene in the bin arrays
+  uint32_t bin_cap;     // entries per bin
+  uint32_t draw_begin;  // first draw in draws[]
+  uint32_t draw_count;
+  uint32_t plane_offset;  // pixel offset of this scene's planes
+  uint32_t prim_count;
+  unsigned long long entry_base;  // first bin entry of this scene (bins x bin_cap)
+};
+
+struct alignas(16) WorkItem {
+  uint32_t scene;
+  uint32_t draw;        // global draw index
+  uint32_t tri_start;   // within the draw
+  uint32_t tri_count;
+  uint32_t vmin;        // global vertex index range referenced by the chunk
+  uint32_t vcount;
+  uint32_t pad[2];
+};
+
+struct alignas(16) RasterItem {
+  uint32_t scene;
+  uint32_t bin;         // scene-local bin index
+  uint32_t pad[2];
+};
+
+struct alignas(16) MaterialDesc {
+  float ambient[4];
+  int32_t texture;      // -1 = untextured
+  uint32_t layer;
+  uint32_t pad[2];
+  float texmat[9];      // column-major
+  float pad2[3];
+};
+
+struct alignas(16) TextureDesc {
+  int32_t width, height, layers, levels;
+  int32_t min_filter, mag_filter, wrap_s, wrap_t;
+  int64_t level_offset[16];  // byte offsets into texels[]
+};
+
+struct FrameParams {
+  // geometry
+  const float* pos_x;
+  const float* pos_y;
+  const float* pos_z;
+  const float2* uv32;
+  const uint32_t* uv16;
+  const uint32_t* col8;
+  const float4* col32;
+  const uint32_t* indices;
+  // materials / textures
+  const MaterialDesc* materials;
+  const TextureDesc* textures;
+  const uint8_t* texels;
+  const float* unorm_lut;  // i/255.0f, 256 entries
+  // scenes
+  const DrawDesc* draws;
+  const SceneDesc* scenes;
+  const float* cameras;        // [scene][16] projection*view
+  const float* unprojections;  // [scene][2]
+  const float* abs_nodes;      // [node][16]
+  // binning
+  uint32_t* bin_count;
+  uint32_t* bin_entries;
+  // outputs (env-major planes), any may be null
+  uint32_t* out_color;
+  float* out_depth;
+  float* out_lindepth;
+  uint32_t* out_semantic;
+  // counters
+  unsigned long long* stats;  // [0] bin entries, [1] overflowed bins
+  int32_t bin_size;
+  int32_t bin_shift;
+  int32_t top_down;
+  int32_t no_textures;
+};
+
+struct GatherArgs {
+  const uint8_t* plane;
+  uint8_t* dst;
+  long long dst_stride;
+  int x0, y0, w, h;       // rectangle in framebuffer pixels
+  int tile_w, tile_h, tiles_x, scene_count;
+  int top_down;
+  int src_bytes;          // 4
+  int dst_bytes;          // 4 or 3 (RGB8 from RGBA8)
+  int fill;               // value for pixels of tiles beyond scene_count
+};
+
+}  // namespace gbr

@@ -1,0 +1,1237 @@
+// Host side of the MI355X batch renderer: the esp::gfx_batch::Renderer /
+// RendererStandalone logic (src/esp/gfx_batch/Renderer.cpp,
+// RendererStandalone.cpp) restated natively, device buffer management, and
+// the C ABI declared in include/gbr.h.
+//
+// Scene state mirrors the reference: per scene a flat node list with
+// parents[i] < i, local transforms, draws keyed into draw batches by
+// (vertex-colour shader variant, mesh, texture) (Renderer.cpp:136-152,
+// 880-961).  draw() sorts draws by batch (Renderer.cpp:1151-1188) to fix the
+// primitive submission order that decides depth ties, uploads what changed,
+// and launches the HIP kernels on the renderer's stream.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "gbr.h"
+#include "gbr_device.h"
+
+#pragma STDC FP_CONTRACT OFF
+
+namespace gbr {
+hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
+                        hipStream_t s);
+hipError_t launch_setup(const FrameParams& p, const WorkItem* items, uint32_t n, hipStream_t s);
+hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n,
+                         hipStream_t s);
+hipError_t launch_gather(const GatherArgs& g, hipStream_t s);
+}  // namespace gbr
+
+using namespace gbr;
+
+namespace {
+
+thread_local std::string g_error;
+
+gbr_status fail(gbr_status s, const std::string& msg) {
+  g_error = msg;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? GBR_ERR_OOM : GBR_ERR_DEVICE,            \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+#define TRY(expr)                    \
+  do {                               \
+    gbr_status s_ = (expr);          \
+    if (s_ != GBR_OK) return s_;     \
+  } while (0)
+
+struct Mat4 {
+  float m[16];
+};
+
+Mat4 identity4() {
+  Mat4 r;
+  for (int i = 0; i < 16; ++i) r.m[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  return r;
+}
+
+// Magnum Matrix4 product (accumulated from +0 over k), Renderer.cpp:1217-1220
+Mat4 mul4(const Mat4& a, const Mat4& b) {
+  Mat4 t;
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r) {
+      float s = 0.0f;
+      for (int k = 0; k < 4; ++k) {
+        float p = a.m[k * 4 + r] * b.m[c * 4 + k];
+        s = s + p;
+      }
+      t.m[c * 4 + r] = s;
+    }
+  return t;
+}
+
+Mat4 load4(const float* p) {
+  Mat4 r;
+  std::memcpy(r.m, p, sizeof r.m);
+  return r;
+}
+
+template <class T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t cap = 0;
+  gbr_status reserve(size_t n, bool keep = false, hipStream_t s = nullptr) {
+    if (n <= cap && ptr) return GBR_OK;
+    size_t ncap = std::max<size_t>(n + n / 2, 64);
+    T* np = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&np), ncap * sizeof(T));
+    if (e != hipSuccess)
+      return fail(GBR_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (keep && ptr && cap) {
+      (void)hipMemcpyAsync(np, ptr, cap * sizeof(T), hipMemcpyDeviceToDevice, s);
+      (void)hipStreamSynchronize(s);
+    }
+    if (ptr) (void)hipFree(ptr);
+    ptr = np;
+    cap = ncap;
+    return GBR_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+};
+
+template <class T>
+gbr_status upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+  TRY(b.reserve(std::max<size_t>(v.size(), 1)));
+  if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return GBR_OK;
+}
+
+struct MeshInfo {
+  uint32_t vbase, vcount;
+  uint32_t ibase, icount;
+  uint32_t color_fmt;
+  uint32_t uv_half;
+};
+
+struct MeshView {
+  uint32_t mesh;
+  uint32_t index_offset;  // in indices, relative to the mesh
+  uint32_t index_count;
+  uint32_t material;
+  Mat4 xf;
+  uint32_t ranges_chunk = 0;              // chunk size the ranges were computed for
+  std::vector<uint32_t> ranges;           // (vmin, vcount) per chunk
+};
+
+struct Draw {
+  uint32_t batch, node, material, view, semantic;
+};
+
+struct Batch {
+  uint32_t vcolor, mesh, texture;
+};
+
+struct Light {
+  uint64_t node;
+  int32_t type;
+  float color[3];
+  float range;
+};
+
+struct Scene {
+  std::vector<int32_t> parents;
+  std::vector<Mat4> transforms;
+  std::vector<Batch> batches;
+  std::vector<Draw> draws;
+  std::vector<Light> lights;
+  Mat4 camera = identity4();
+  float unproj[2] = {0.0f, 0.0f};
+  bool xform_dirty = true;
+};
+
+float unorm(int i) { return (float)i / 255.0f; }
+
+}  // namespace
+
+struct gbr_renderer {
+  gbr_config cfg{};
+  gbr_tuning tun{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // assets (host mirrors)
+  std::vector<float> pos_x, pos_y, pos_z;
+  std::vector<float2> uv32;
+  std::vector<uint32_t> uv16;
+  std::vector<uint32_t> col8;
+  std::vector<float4> col32;
+  std::vector<uint32_t> indices;
+  std::vector<uint8_t> texels;
+  std::vector<TextureDesc> textures;
+  std::vector<MaterialDesc> materials;
+  std::vector<MeshInfo> meshes;
+  std::vector<MeshView> views;
+  std::unordered_map<std::string, std::pair<uint32_t, uint32_t>> names;
+  bool files_added = false;
+  bool assets_dirty = true;
+
+  std::vector<Scene> scenes;
+  bool structure_dirty = true;
+
+  // frame layout (rebuilt on structural change)
+  std::vector<uint32_t> scene_node_base;
+  std::vector<SceneDesc> scene_desc;
+  uint32_t n_nodes = 0, n_items = 0, n_ritems = 0;
+  uint64_t n_tris = 0;
+  size_t n_bins = 0, entries_cap = 0;
+
+  // device buffers
+  DevBuf<float> d_pos_x, d_pos_y, d_pos_z;
+  DevBuf<float2> d_uv32;
+  DevBuf<uint32_t> d_uv16, d_col8;
+  DevBuf<float4> d_col32;
+  DevBuf<uint32_t> d_indices;
+  DevBuf<uint8_t> d_texels;
+  DevBuf<TextureDesc> d_textures;
+  DevBuf<MaterialDesc> d_materials;
+  DevBuf<float> d_lut;
+  DevBuf<DrawDesc> d_draws;
+  DevBuf<SceneDesc> d_scenes;
+  DevBuf<WorkItem> d_items;
+  DevBuf<RasterItem> d_ritems;
+  DevBuf<int32_t> d_parents;
+  DevBuf<float> d_local, d_abs;
+  DevBuf<float> d_cameras, d_unproj;
+  DevBuf<uint32_t> d_bin_count, d_bin_entries;
+  DevBuf<unsigned long long> d_stats;
+  DevBuf<uint32_t> d_color, d_sem;
+  DevBuf<float> d_depth, d_lin;
+  DevBuf<uint8_t> d_fb_color, d_fb_depth, d_stage;
+  // pinned per-frame staging for cameras
+  float* h_cam = nullptr;
+  size_t h_cam_cap = 0;
+  hipEvent_t cam_event = nullptr;
+  bool cam_event_pending = false;
+
+  uint64_t tile_pixels() const { return (uint64_t)cfg.tile_size[0] * cfg.tile_size[1]; }
+  uint32_t scene_count() const { return (uint32_t)scenes.size(); }
+};
+
+namespace {
+
+gbr_status check_scene(const gbr_renderer* r, uint32_t scene, const char* fn) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, std::string(fn) + ": null renderer");
+  if (scene >= r->scenes.size())
+    return fail(GBR_ERR_RANGE, std::string(fn) + ": index " + std::to_string(scene) +
+                                   " out of range for " + std::to_string(r->scenes.size()) +
+                                   " scenes");
+  return GBR_OK;
+}
+
+gbr_status set_device(const gbr_renderer* r) {
+  HIP_TRY(hipSetDevice(r->device));
+  return GBR_OK;
+}
+
+// glGenerateMipmap() stand-in: 2x2 rounded box filter down to min(w,h) == 1
+std::vector<std::vector<uint8_t>> generate_mips(const uint8_t* level0, int w, int h, int layers,
+                                                int& levels_out) {
+  int n = 1;
+  while ((1 << n) <= std::min(w, h)) ++n;  // log2(min)+1
+  std::vector<std::vector<uint8_t>> out;
+  out.emplace_back(level0, level0 + (size_t)w * h * layers * 4);
+  int cw = w, ch = h;
+  for (int l = 1; l < n; ++l) {
+    int nw = std::max(cw / 2, 1), nh = std::max(ch / 2, 1);
+    const std::vector<uint8_t>& src = out.back();
+    std::vector<uint8_t> dst((size_t)nw * nh * layers * 4);
+    for (int la = 0; la < layers; ++la)
+      for (int y = 0; y < nh; ++y)
+        for (int x = 0; x < nw; ++x)
+          for (int c = 0; c < 4; ++c) {
+            auto at = [&](int xx, int yy) {
+              return (int)src[(((size_t)la * ch + yy) * cw + xx) * 4 + c];
+            };
+            int s = at(2 * x, 2 * y) + at(2 * x, 2 * y + 1) + at(2 * x + 1, 2 * y) +
+                    at(2 * x + 1, 2 * y + 1);
+            dst[(((size_t)la * nh + y) * nw + x) * 4 + c] = (uint8_t)((s + 2) >> 2);
+          }
+    out.push_back(std::move(dst));
+    cw = nw;
+    ch = nh;
+  }
+  levels_out = n;
+  return out;
+}
+
+uint32_t batch_id(Scene& sc, uint32_t vcolor, uint32_t mesh, uint32_t tex) {
+  for (size_t i = 0; i < sc.batches.size(); ++i)
+    if (sc.batches[i].vcolor == vcolor && sc.batches[i].mesh == mesh && sc.batches[i].texture == tex)
+      return (uint32_t)i;
+  sc.batches.push_back({vcolor, mesh, tex});
+  return (uint32_t)sc.batches.size() - 1;
+}
+
+// Per-chunk vertex ranges of a mesh view (for the LDS vertex cache)
+const std::vector<uint32_t>& view_ranges(gbr_renderer* r, MeshView& v, uint32_t chunk) {
+  if (v.ranges_chunk == chunk) return v.ranges;
+  v.ranges.clear();
+  const MeshInfo& m = r->meshes[v.mesh];
+  const uint32_t ntri = v.index_count / 3;
+  for (uint32_t t0 = 0; t0 < ntri; t0 += chunk) {
+    uint32_t t1 = std::min(ntri, t0 + chunk);
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint32_t i = m.ibase + v.index_offset + 3 * t0; i < m.ibase + v.index_offset + 3 * t1; ++i) {
+      lo = std::min(lo, r->indices[i]);
+      hi = std::max(hi, r->indices[i]);
+    }
+    if (lo > hi) lo = hi = 0;
+    v.ranges.push_back(lo);
+    v.ranges.push_back(hi - lo + 1);
+  }
+  v.ranges_chunk = chunk;
+  return v.ranges;
+}
+
+gbr_status upload_assets(gbr_renderer* r) {
+  hipStream_t s = r->stream;
+  TRY(upload(r->d_pos_x, r->pos_x, s));
+  TRY(upload(r->d_pos_y, r->pos_y, s));
+  TRY(upload(r->d_pos_z, r->pos_z, s));
+  TRY(upload(r->d_uv32, r->uv32, s));
+  TRY(upload(r->d_uv16, r->uv16, s));
+  TRY(upload(r->d_col8, r->col8, s));
+  TRY(upload(r->d_col32, r->col32, s));
+  TRY(upload(r->d_indices, r->indices, s));
+  TRY(upload(r->d_texels, r->texels, s));
+  TRY(upload(r->d_textures, r->textures, s));
+  TRY(upload(r->d_materials, r->materials, s));
+  std::vector<float> lut(256);
+  for (int i = 0; i < 256; ++i) lut[i] = unorm(i);
+  TRY(upload(r->d_lut, lut, s));
+  HIP_TRY(hipStreamSynchronize(s));  // host vectors may change after return
+  r->assets_dirty = false;
+  return GBR_OK;
+}
+
+gbr_status rebuild_structure(gbr_renderer* r) {
+  hipStream_t s = r->stream;
+  const uint32_t nsc = r->scene_count();
+  const int W = r->cfg.tile_size[0], H = r->cfg.tile_size[1];
+  const int B = r->tun.bin_size;
+  const int bins_x = (W + B - 1) / B, bins_y = (H + B - 1) / B;
+  const uint32_t chunk = (uint32_t)r->tun.chunk_triangles;
+
+  std::vector<int32_t> parents;
+  std::vector<float> local;
+  std::vector<DrawDesc> draws;
+  std::vector<WorkItem> items;
+  std::vector<RasterItem> ritems;
+  r->scene_node_base.assign(nsc, 0);
+  r->scene_desc.assign(nsc, SceneDesc{});
+  uint64_t bins = 0, entries = 0, tris = 0;
+  for (uint32_t si = 0; si < nsc; ++si) {
+    Scene& sc = r->scenes[si];
+    const uint32_t nbase = (uint32_t)parents.size();
+    r->scene_node_base[si] = nbase;
+    for (size_t i = 0; i < sc.parents.size(); ++i) {
+      parents.push_back(sc.parents[i] < 0 ? -1 : (int32_t)(nbase + sc.parents[i]));
+      local.insert(local.end(), sc.transforms[i].m, sc.transforms[i].m + 16);
+    }
+    sc.xform_dirty = false;
+    // counting sort of draws by batch id, stable (Renderer.cpp:1151-1188)
+    std::vector<uint32_t> order;
+    order.reserve(sc.draws.size());
+    for (uint32_t b = 0; b < sc.batches.size(); ++b)
+      for (uint32_t i = 0; i < sc.draws.size(); ++i)
+        if (sc.draws[i].batch == b) order.push_back(i);
+    SceneDesc& sd = r->scene_desc[si];
+    sd.width = W;
+    sd.height = H;
+    sd.bins_x = bins_x;
+    sd.bins_y = bins_y;
+    sd.draw_begin = (uint32_t)draws.size();
+    sd.draw_count = (uint32_t)order.size();
+    sd.plane_offset = (uint32_t)((uint64_t)si * W * H);
+    uint32_t prim = 0;
+    for (uint32_t oi : order) {
+      const Draw& dr = sc.draws[oi];
+      MeshView& v = r->views[dr.view];
+      const MeshInfo& m = r->meshes[v.mesh];
+      DrawDesc dd{};
+      dd.index_base = m.ibase + v.index_offset;
+      dd.tri_count = v.index_count / 3;
+      dd.node = nbase + dr.node;
+      dd.material = dr.material;
+      dd.semantic = dr.semantic;
+      dd.flags = (m.color_fmt << kDrawColorShift) | (m.uv_half ? kDrawUvHalf : 0u);
+      dd.prim_base = prim;
+      const uint32_t gdraw = (uint32_t)draws.size();
+      draws.push_back(dd);
+      const std::vector<uint32_t>& rg = view_ranges(r, v, chunk);
+      for (uint32_t t0 = 0, c = 0; t0 < dd.tri_count; t0 += chunk, ++c) {
+        WorkItem wi{};
+        wi.scene = si;
+        wi.draw = gdraw;
+        wi.tri_start = t0;
+        wi.tri_count = std::min(chunk, dd.tri_count - t0);
+        wi.vmin = rg[2 * c];  // global vertex indices
+        wi.vcount = rg[2 * c + 1];
+        items.push_back(wi);
+      }
+      prim += dd.tri_count;
+    }
+    sd.prim_count = prim;
+    tris += prim;
+    const uint32_t nb = (uint32_t)(bins_x * bins_y);
+    sd.bin_base = (uint32_t)bins;
+    double per_bin = (double)prim / nb * r->tun.bin_capacity_factor;
+    uint32_t cap = (uint32_t)std::min<double>(std::max<double>(per_bin, r->tun.min_bin_capacity), 1u << 24);
+    cap = (cap + 63) & ~63u;
+    sd.bin_cap = cap;
+    sd.entry_base = entries;
+    bins += nb;
+    entries += (uint64_t)nb * cap;
+    for (uint32_t b = 0; b < nb; ++b) ritems.push_back(RasterItem{si, b, {0, 0}});
+  }
+  r->n_nodes = (uint32_t)parents.size();
+  r->n_items = (uint32_t)items.size();
+  r->n_ritems = (uint32_t)ritems.size();
+  r->n_tris = tris;
+  TRY(upload(r->d_parents, parents, s));
+  TRY(upload(r->d_local, local, s));
+  TRY(r->d_abs.reserve(std::max<size_t>(local.size(), 16)));
+  TRY(upload(r->d_draws, draws, s));
+  TRY(upload(r->d_scenes, r->scene_desc, s));
+  TRY(upload(r->d_items, items, s));
+  TRY(upload(r->d_ritems, ritems, s));
+  if (bins > r->n_bins || r->d_bin_count.ptr == nullptr) {
+    TRY(r->d_bin_count.reserve(bins + 1));
+    HIP_TRY(hipMemsetAsync(r->d_bin_count.ptr, 0, r->d_bin_count.cap * sizeof(uint32_t), s));
+  }
+  r->n_bins = bins;
+  TRY(r->d_bin_entries.reserve(std::max<uint64_t>(entries, 1)));
+  r->entries_cap = entries;
+  HIP_TRY(hipStreamSynchronize(s));
+  r->structure_dirty = false;
+  return GBR_OK;
+}
+
+gbr_status upload_dirty_transforms(gbr_renderer* r) {
+  for (uint32_t si = 0; si < r->scene_count(); ++si) {
+    Scene& sc = r->scenes[si];
+    if (!sc.xform_dirty) continue;
+    sc.xform_dirty = false;
+    if (sc.transforms.empty()) continue;
+    HIP_TRY(hipMemcpyAsync(r->d_local.ptr + (size_t)r->scene_node_base[si] * 16, sc.transforms.data(),
+                           sc.transforms.size() * sizeof(Mat4), hipMemcpyHostToDevice, r->stream));
+  }
+  return GBR_OK;
+}
+
+gbr_status upload_cameras(gbr_renderer* r) {
+  const size_t n = r->scene_count();
+  const size_t need = n * 18;
+  if (r->cam_event_pending) {
+    HIP_TRY(hipEventSynchronize(r->cam_event));
+    r->cam_event_pending = false;
+  }
+  if (r->h_cam_cap < need) {
+    if (r->h_cam) (void)hipHostFree(r->h_cam);
+    r->h_cam = nullptr;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r->h_cam), need * sizeof(float)));
+    r->h_cam_cap = need;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    std::memcpy(r->h_cam + i * 16, r->scenes[i].camera.m, 64);
+    r->h_cam[n * 16 + 2 * i] = r->scenes[i].unproj[0];
+    r->h_cam[n * 16 + 2 * i + 1] = r->scenes[i].unproj[1];
+  }
+  TRY(r->d_cameras.reserve(n * 16 + 16));
+  TRY(r->d_unproj.reserve(n * 2 + 2));
+  HIP_TRY(hipMemcpyAsync(r->d_cameras.ptr, r->h_cam, n * 64, hipMemcpyHostToDevice, r->stream));
+  HIP_TRY(hipMemcpyAsync(r->d_unproj.ptr, r->h_cam + n * 16, n * 8, hipMemcpyHostToDevice, r->stream));
+  HIP_TRY(hipEventRecord(r->cam_event, r->stream));
+  r->cam_event_pending = true;
+  return GBR_OK;
+}
+
+gbr_status ensure_planes(gbr_renderer* r) {
+  const size_t px = (size_t)r->scene_count() * r->tile_pixels();
+  const uint32_t pl = r->cfg.planes;
+  if (pl & GBR_PLANE_COLOR) TRY(r->d_color.reserve(px));
+  if (pl & GBR_PLANE_DEPTH) TRY(r->d_depth.reserve(px));
+  if (pl & GBR_PLANE_LINEAR_DEPTH) TRY(r->d_lin.reserve(px));
+  if (pl & GBR_PLANE_SEMANTIC) TRY(r->d_sem.reserve(px));
+  return GBR_OK;
+}
+
+gbr_status do_draw(gbr_renderer* r) {
+  TRY(set_device(r));
+  TRY(ensure_planes(r));
+  if (r->assets_dirty) TRY(upload_assets(r));
+  if (r->structure_dirty) TRY(rebuild_structure(r));
+  else TRY(upload_dirty_transforms(r));
+  TRY(upload_cameras(r));
+  TRY(r->d_stats.reserve(4));
+  HIP_TRY(hipMemsetAsync(r->d_stats.ptr, 0, 4 * sizeof(unsigned long long), r->stream));
+
+  FrameParams p{};
+  p.pos_x = r->d_pos_x.ptr;
+  p.pos_y = r->d_pos_y.ptr;
+  p.pos_z = r->d_pos_z.ptr;
+  p.uv32 = r->d_uv32.ptr;
+  p.uv16 = r->d_uv16.ptr;
+  p.col8 = r->d_col8.ptr;
+  p.col32 = r->d_col32.ptr;
+  p.indices = r->d_indices.ptr;
+  p.materials = r->d_materials.ptr;
+  p.textures = r->d_textures.ptr;
+  p.texels = r->d_texels.ptr;
+  p.unorm_lut = r->d_lut.ptr;
+  p.draws = r->d_draws.ptr;
+  p.scenes = r->d_scenes.ptr;
+  p.cameras = r->d_cameras.ptr;
+  p.unprojections = r->d_unproj.ptr;
+  p.abs_nodes = r->d_abs.ptr;
+  p.bin_count = r->d_bin_count.ptr;
+  p.bin_entries = r->d_bin_entries.ptr;
+  const uint32_t pl = r->cfg.planes;
+  p.out_color = (pl & GBR_PLANE_COLOR) ? r->d_color.ptr : nullptr;
+  p.out_depth = (pl & GBR_PLANE_DEPTH) ? r->d_depth.ptr : nullptr;
+  p.out_lindepth = (pl & GBR_PLANE_LINEAR_DEPTH) ? r->d_lin.ptr : nullptr;
+  p.out_semantic = (pl & GBR_PLANE_SEMANTIC) ? r->d_sem.ptr : nullptr;
+  p.stats = r->d_stats.ptr;
+  p.bin_size = r->tun.bin_size;
+  int sh = 0;
+  while ((1 << sh) < r->tun.bin_size) ++sh;
+  p.bin_shift = sh;
+  p.top_down = r->cfg.top_down ? 1 : 0;
+  p.no_textures = (r->cfg.flags & GBR_FLAG_NO_TEXTURES) ? 1 : 0;
+
+  HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, r->stream));
+  HIP_TRY(launch_setup(p, r->d_items.ptr, r->n_items, r->stream));
+  HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, r->stream));
+  return GBR_OK;
+}
+
+gbr_status gather_plane(gbr_renderer* r, uint32_t plane, const int32_t* rect, void* dst,
+                        int64_t row_stride, int dst_bytes, int32_t memory) {
+  if (!r || !rect || !dst) return fail(GBR_ERR_ARGUMENT, "image_into: null argument");
+  TRY(set_device(r));
+  const int fw = r->cfg.tile_size[0] * r->cfg.tile_count[0];
+  const int fh = r->cfg.tile_size[1] * r->cfg.tile_count[1];
+  if (rect[0] < 0 || rect[1] < 0 || rect[2] > fw || rect[3] > fh || rect[2] < rect[0] ||
+      rect[3] < rect[1])
+    return fail(GBR_ERR_SIZE, "image_into: rectangle {" + std::to_string(rect[0]) + ", " +
+                                  std::to_string(rect[1]) + ", " + std::to_string(rect[2]) + ", " +
+                                  std::to_string(rect[3]) + "} doesn't fit in a size of {" +
+                                  std::to_string(fw) + ", " + std::to_string(fh) + "}");
+  if (!(r->cfg.planes & plane))
+    return fail(GBR_ERR_STATE, "image_into: the plane is not produced by this renderer (gbr_config.planes)");
+  const int w = rect[2] - rect[0], h = rect[3] - rect[1];
+  if (w == 0 || h == 0) return GBR_OK;
+  const uint8_t* src = nullptr;
+  uint32_t fill = 0;
+  switch (plane) {
+    case GBR_PLANE_COLOR: src = reinterpret_cast<const uint8_t*>(r->d_color.ptr); fill = 0xFF1F1F1Fu; break;
+    case GBR_PLANE_DEPTH: src = reinterpret_cast<const uint8_t*>(r->d_depth.ptr); fill = 0x3F800000u; break;
+    case GBR_PLANE_LINEAR_DEPTH: src = reinterpret_cast<const uint8_t*>(r->d_lin.ptr); break;
+    default: src = reinterpret_cast<const uint8_t*>(r->d_sem.ptr); break;
+  }
+  if (!src) return fail(GBR_ERR_STATE, "image_into: nothing drawn yet");
+  GatherArgs g{};
+  g.plane = src;
+  g.x0 = rect[0];
+  g.y0 = rect[1];
+  g.w = w;
+  g.h = h;
+  g.tile_w = r->cfg.tile_size[0];
+  g.tile_h = r->cfg.tile_size[1];
+  g.tiles_x = r->cfg.tile_count[0];
+  g.scene_count = (int)r->scene_count();
+  g.top_down = r->cfg.top_down ? 1 : 0;
+  g.src_bytes = 4;
+  g.dst_bytes = dst_bytes;
+  g.fill = (int)fill;
+  const int64_t tight = (int64_t)w * dst_bytes;
+  if (row_stride == 0) row_stride = tight;
+  if (row_stride < tight) return fail(GBR_ERR_SIZE, "image_into: row stride smaller than a row");
+  if (memory == GBR_MEMORY_DEVICE) {
+    g.dst = reinterpret_cast<uint8_t*>(dst);
+    g.dst_stride = row_stride;
+    HIP_TRY(launch_gather(g, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    return GBR_OK;
+  }
+  TRY(r->d_stage.reserve((size_t)tight * h));
+  g.dst = r->d_stage.ptr;
+  g.dst_stride = tight;
+  HIP_TRY(launch_gather(g, r->stream));
+  HIP_TRY(hipMemcpy2DAsync(dst, row_stride, r->d_stage.ptr, tight, tight, h, hipMemcpyDeviceToHost,
+                           r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return GBR_OK;
+}
+
+gbr_status framebuffer_pointer(gbr_renderer* r, uint32_t plane, DevBuf<uint8_t>& buf,
+                               const void** out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "device_pointer: null argument");
+  const int fw = r->cfg.tile_size[0] * r->cfg.tile_count[0];
+  const int fh = r->cfg.tile_size[1] * r->cfg.tile_count[1];
+  TRY(buf.reserve((size_t)fw * fh * 4));
+  int32_t rect[4] = {0, 0, fw, fh};
+  TRY(gather_plane(r, plane, rect, buf.ptr, (int64_t)fw * 4, 4, GBR_MEMORY_DEVICE));
+  *out = buf.ptr;
+  return GBR_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+int32_t gbr_abi_version(void) { return GBR_ABI_VERSION; }
+
+const char* gbr_last_error(void) { return g_error.c_str(); }
+
+gbr_status gbr_config_init(gbr_config* c) {
+  if (!c) return fail(GBR_ERR_ARGUMENT, "gbr_config_init: null");
+  std::memset(c, 0, sizeof *c);
+  c->tile_size[0] = c->tile_size[1] = 128;
+  c->tile_count[0] = c->tile_count[1] = 1;
+  c->ambient_factor = 0.1f;
+  c->device = -1;
+  c->planes = GBR_PLANE_ALL;
+  return GBR_OK;
+}
+
+gbr_status gbr_tuning_init(gbr_tuning* t) {
+  if (!t) return fail(GBR_ERR_ARGUMENT, "gbr_tuning_init: null");
+  std::memset(t, 0, sizeof *t);
+  t->bin_size = 32;
+  t->chunk_triangles = 256;
+  t->bin_capacity_factor = 2.0f;
+  t->min_bin_capacity = 256;
+  t->cull_mode = GBR_CULL_NONE;
+  return GBR_OK;
+}
+
+static gbr_status validate_tuning(const gbr_tuning* t) {
+  if (t->bin_size != 8 && t->bin_size != 16 && t->bin_size != 32)
+    return fail(GBR_ERR_ARGUMENT, "tuning: bin_size must be 8, 16 or 32");
+  if (t->chunk_triangles < 1 || t->chunk_triangles > kBlock)
+    return fail(GBR_ERR_ARGUMENT, "tuning: chunk_triangles must be in [1, 256]");
+  if (!(t->bin_capacity_factor > 0.0f) || t->min_bin_capacity < 1)
+    return fail(GBR_ERR_ARGUMENT, "tuning: capacity must be positive");
+  if (t->cull_mode != GBR_CULL_NONE)
+    return fail(GBR_ERR_UNSUPPORTED, "tuning: only GBR_CULL_NONE is available");
+  return GBR_OK;
+}
+
+gbr_status gbr_create(const gbr_config* config, gbr_renderer** out) {
+  if (!config || !out) return fail(GBR_ERR_ARGUMENT, "gbr_create: null argument");
+  *out = nullptr;
+  if (config->tile_size[0] < 1 || config->tile_size[1] < 1 || config->tile_count[0] < 1 ||
+      config->tile_count[1] < 1)
+    return fail(GBR_ERR_ARGUMENT, "gbr_create: tile size and count must be positive");
+  if (config->max_light_count != 0)
+    return fail(GBR_ERR_UNSUPPORTED, "gbr_create: lights (max_light_count > 0) are not supported");
+  if (config->planes & ~GBR_PLANE_ALL) return fail(GBR_ERR_ARGUMENT, "gbr_create: unknown plane bits");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  int dev = config->device;
+  if (dev < 0) HIP_TRY(hipGetDevice(&dev));
+  if (dev >= ndev) return fail(GBR_ERR_ARGUMENT, "gbr_create: device ordinal out of range");
+  HIP_TRY(hipSetDevice(dev));
+  gbr_renderer* r = new (std::nothrow) gbr_renderer();
+  if (!r) return fail(GBR_ERR_OOM, "gbr_create: out of host memory");
+  r->cfg = *config;
+  r->device = dev;
+  gbr_tuning_init(&r->tun);
+  const uint64_t n = (uint64_t)config->tile_count[0] * config->tile_count[1];
+  r->scenes.resize(n);
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&r->cam_event, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    delete r;
+    return fail(GBR_ERR_DEVICE, std::string("gbr_create: ") + hipGetErrorString(e));
+  }
+  // texture 0 = white pixel, material 0 = white ambient (Renderer.cpp:287-304)
+  TextureDesc white{};
+  white.width = white.height = white.layers = white.levels = 1;
+  white.min_filter = 9728;
+  white.mag_filter = 9728;
+  white.wrap_s = white.wrap_t = 10497;
+  white.level_offset[0] = 0;
+  r->textures.push_back(white);
+  r->texels.insert(r->texels.end(), {255, 255, 255, 255});
+  MaterialDesc m0{};
+  for (int i = 0; i < 4; ++i) m0.ambient[i] = 1.0f;
+  m0.texture = 0;
+  m0.texmat[0] = m0.texmat[4] = m0.texmat[8] = 1.0f;
+  r->materials.push_back(m0);
+  *out = r;
+  return GBR_OK;
+}
+
+gbr_status gbr_destroy(gbr_renderer* r) {
+  if (!r) return GBR_OK;
+  (void)hipSetDevice(r->device);
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  DevBuf<float>* fb[] = {&r->d_pos_x, &r->d_pos_y, &r->d_pos_z, &r->d_lut, &r->d_local, &r->d_abs,
+                         &r->d_cameras, &r->d_unproj, &r->d_depth, &r->d_lin};
+  for (auto* b : fb) b->release();
+  DevBuf<uint32_t>* ub[] = {&r->d_uv16, &r->d_col8, &r->d_indices, &r->d_bin_count,
+                            &r->d_bin_entries, &r->d_color, &r->d_sem};
+  for (auto* b : ub) b->release();
+  r->d_uv32.release();
+  r->d_col32.release();
+  r->d_texels.release();
+  r->d_textures.release();
+  r->d_materials.release();
+  r->d_draws.release();
+  r->d_scenes.release();
+  r->d_items.release();
+  r->d_ritems.release();
+  r->d_parents.release();
+  r->d_stats.release();
+  r->d_fb_color.release();
+  r->d_fb_depth.release();
+  r->d_stage.release();
+  if (r->h_cam) (void)hipHostFree(r->h_cam);
+  if (r->cam_event) (void)hipEventDestroy(r->cam_event);
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+  return GBR_OK;
+}
+
+gbr_status gbr_get_config(const gbr_renderer* r, gbr_config* out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "gbr_get_config: null argument");
+  *out = r->cfg;
+  return GBR_OK;
+}
+
+gbr_status gbr_scene_count(const gbr_renderer* r, uint64_t* out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "gbr_scene_count: null argument");
+  *out = r->scenes.size();
+  return GBR_OK;
+}
+
+gbr_status gbr_add_file(gbr_renderer* r, const char* filename, uint32_t, const char*) {
+  if (!r || !filename) return fail(GBR_ERR_ARGUMENT, "gbr_add_file: null argument");
+  return fail(GBR_ERR_UNSUPPORTED,
+              "gbr_add_file: native file import is not available; parse the file and call "
+              "gbr_add_file_data()");
+}
+
+// Renderer::addFile(), Renderer.cpp:354-867, on a parsed file
+gbr_status gbr_add_file_data(gbr_renderer* r, const gbr_file_desc* f, uint32_t flags,
+                             const char* name, const char* filename) {
+  if (!r || !f) return fail(GBR_ERR_ARGUMENT, "gbr_add_file_data: null argument");
+  if (flags & ~(GBR_FILE_WHOLE | GBR_FILE_GENERATE_MIPMAP))
+    return fail(GBR_ERR_ARGUMENT, "gbr_add_file_data: unknown file flags");
+  const std::string fname = filename ? filename : "";
+  const uint32_t tex_off = (uint32_t)r->textures.size();
+  const uint32_t mesh_off = (uint32_t)r->meshes.size();
+  const uint32_t view_off = (uint32_t)r->views.size();
+  const uint32_t mat_off = (uint32_t)r->materials.size();
+
+  // validate references up front
+  for (uint32_t i = 0; i < f->material_count; ++i)
+    if (f->materials[i].texture >= (int32_t)f->texture_count)
+      return fail(GBR_ERR_IO, "addFile(): material references a missing texture in " + fname);
+  for (uint32_t i = 0; i < f->mesh_count; ++i) {
+    const gbr_mesh_desc& m = f->meshes[i];
+    if (!m.positions || (m.index_count && !m.indices) || m.index_count % 3)
+      return fail(GBR_ERR_IO, "addFile(): can't import mesh " + std::to_string(i) + " of " + fname);
+    for (uint32_t k = 0; k < m.index_count; ++k)
+      if (m.indices[k] >= m.vertex_count)
+        return fail(GBR_ERR_IO, "addFile(): mesh " + std::to_string(i) + " index out of range in " + fname);
+  }
+
+  // textures (Renderer.cpp:417-534)
+  if (!(r->cfg.flags & GBR_FLAG_NO_TEXTURES)) {
+    for (uint32_t i = 0; i < f->texture_count; ++i) {
+      const gbr_texture_desc& t = f->textures[i];
+      if (!t.texels || t.width < 1 || t.height < 1 || t.layers < 1 || t.levels < 1 || t.levels > 16)
+        return fail(GBR_ERR_IO, "addFile(): can't import texture " + std::to_string(i) + " of " + fname);
+      TextureDesc td{};
+      td.width = (int)t.width;
+      td.height = (int)t.height;
+      td.layers = (int)t.layers;
+      td.min_filter = (int)t.min_filter;
+      td.mag_filter = (int)t.mag_filter;
+      td.wrap_s = (int)t.wrap_s;
+      td.wrap_t = (int)t.wrap_t;
+      const bool gen = t.levels == 1 && (flags & GBR_FILE_GENERATE_MIPMAP) && !t.compressed;
+      if (gen) {
+        int nl = 1;
+        auto lv = generate_mips(t.texels, td.width, td.height, td.layers, nl);
+        td.levels = nl;
+        for (int l = 0; l < nl; ++l) {
+          td.level_offset[l] = (int64_t)r->texels.size();
+          r->texels.insert(r->texels.end(), lv[l].begin(), lv[l].end());
+        }
+      } else {
+        td.levels = (int)t.levels;
+        size_t off = 0;
+        for (int l = 0; l < td.levels; ++l) {
+          const size_t sz = (size_t)std::max(td.width >> l, 1) * std::max(td.height >> l, 1) * td.layers * 4;
+          td.level_offset[l] = (int64_t)r->texels.size();
+          r->texels.insert(r->texels.end(), t.texels + off, t.texels + off + sz);
+          off += sz;
+        }
+      }
+      r->textures.push_back(td);
+    }
+  }
+
+  // meshes (Renderer.cpp:536-557)
+  for (uint32_t i = 0; i < f->mesh_count; ++i) {
+    const gbr_mesh_desc& m = f->meshes[i];
+    MeshInfo mi{};
+    mi.vbase = (uint32_t)r->pos_x.size();
+    mi.vcount = m.vertex_count;
+    mi.ibase = (uint32_t)r->indices.size();
+    mi.icount = m.index_count;
+    mi.uv_half = m.texcoords_half ? 1u : 0u;
+    mi.color_fmt = m.colors ? (m.colors_u8 ? kColorU8 : kColorF32) : kColorNone;
+    for (uint32_t v = 0; v < m.vertex_count; ++v) {
+      r->pos_x.push_back(m.positions[3 * v]);
+      r->pos_y.push_back(m.positions[3 * v + 1]);
+      r->pos_z.push_back(m.positions[3 * v + 2]);
+      float u = m.texcoords ? m.texcoords[2 * v] : 0.0f;
+      float vv = m.texcoords ? m.texcoords[2 * v + 1] : 0.0f;
+      r->uv32.push_back(make_float2(u, vv));
+      __half hu = __float2half(u), hv = __float2half(vv);
+      r->uv16.push_back((uint32_t)__half_as_ushort(hu) | ((uint32_t)__half_as_ushort(hv) << 16));
+      if (mi.color_fmt == kColorU8) {
+        const uint8_t* c = static_cast<const uint8_t*>(m.colors) + 4 * v;
+        r->col8.push_back((uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) |
+                          ((uint32_t)c[3] << 24));
+        r->col32.push_back(make_float4(0, 0, 0, 0));
+      } else if (mi.color_fmt == kColorF32) {
+        const float* c = static_cast<const float*>(m.colors) + 4 * v;
+        r->col8.push_back(0);
+        r->col32.push_back(make_float4(c[0], c[1], c[2], c[3]));
+      } else {
+        r->col8.push_back(0);
+        r->col32.push_back(make_float4(1, 1, 1, 1));
+      }
+    }
+    for (uint32_t k = 0; k < m.index_count; ++k) r->indices.push_back(mi.vbase + m.indices[k]);
+    r->meshes.push_back(mi);
+  }
+
+  // materials (Renderer.cpp:559-626): flat if marked flat or no lights
+  for (uint32_t i = 0; i < f->material_count; ++i) {
+    const gbr_material_desc& md = f->materials[i];
+    MaterialDesc m{};
+    for (int c = 0; c < 4; ++c) {
+      float b = md.base_color[c];
+      m.ambient[c] = (md.flat || !r->cfg.max_light_count) ? b : b * r->cfg.ambient_factor;
+    }
+    if (md.texture < 0) {
+      m.texture = 0;
+      m.layer = 0;
+      m.texmat[0] = m.texmat[4] = m.texmat[8] = 1.0f;
+    } else {
+      m.texture = (int32_t)(md.texture + tex_off);
+      m.layer = md.layer;
+      std::memcpy(m.texmat, md.texture_matrix, sizeof m.texmat);
+    }
+    // with NoTextures nothing is imported: the id only keys draw batches and
+    // the kernels never sample (Renderer.cpp:417-418, 842-849)
+    r->materials.push_back(m);
+  }
+  r->files_added = true;
+  r->assets_dirty = true;
+  // cameras are re-created by addFile(), Renderer.cpp:630-631
+  for (Scene& sc : r->scenes) sc.camera = identity4();
+
+  if (!f->has_scene) {
+    if (f->mesh_count != 1 || f->material_count)
+      return fail(GBR_ERR_IO, "addFile(): expected exactly one mesh and no material for a scene-less file " + fname);
+    if (!(flags & GBR_FILE_WHOLE))
+      return fail(GBR_ERR_ARGUMENT, "addFile(): scene-less file " + fname + " has to be added with GBR_FILE_WHOLE");
+    MeshView v;
+    v.mesh = mesh_off;
+    v.index_offset = 0;
+    v.index_count = f->meshes[0].index_count;
+    v.material = 0;
+    v.xf = identity4();
+    r->views.push_back(std::move(v));
+    const std::string used = (name && *name) ? name : fname;
+    if (!r->names.emplace(used, std::make_pair(view_off, view_off + 1)).second)
+      return fail(GBR_ERR_STATE, "addFile(): node name " + used + " in " + fname + " already exists");
+    return GBR_OK;
+  }
+
+  bool has_views = false;
+  for (uint32_t i = 0; i < f->mesh_entry_count; ++i) has_views |= f->mesh_entries[i].has_view != 0;
+  std::vector<MeshView> nv;
+  for (uint32_t i = 0; i < f->mesh_entry_count; ++i) {
+    const gbr_mesh_entry_desc& e = f->mesh_entries[i];
+    if (e.mesh >= f->mesh_count || e.node >= f->node_count)
+      return fail(GBR_ERR_IO, "addFile(): mesh field out of range in " + fname);
+    const gbr_mesh_desc& m = f->meshes[e.mesh];
+    MeshView v;
+    v.mesh = e.mesh + mesh_off;
+    int32_t mat;
+    if (has_views) {
+      const uint32_t isz = m.source_index_size ? m.source_index_size : 4;
+      v.index_offset = e.view_index_offset_bytes / isz;
+      v.index_count = e.view_index_count;
+      mat = e.view_material;
+    } else {
+      v.index_offset = 0;
+      v.index_count = m.index_count;
+      mat = e.material;
+    }
+    if (v.index_offset + v.index_count > m.index_count || v.index_offset % 3 || v.index_count % 3)
+      return fail(GBR_ERR_IO, "addFile(): mesh view out of range in " + fname);
+    if (mat >= (int32_t)f->material_count)
+      return fail(GBR_ERR_IO, "addFile(): mesh view material out of range in " + fname);
+    v.material = mat == -1 ? 0 : (uint32_t)mat + mat_off;
+    nv.push_back(std::move(v));
+  }
+  if (!(flags & GBR_FILE_WHOLE)) {
+    for (uint32_t i = 0; i < f->mesh_entry_count; ++i)
+      nv[i].xf = load4(f->nodes[f->mesh_entries[i].node].transform);
+    for (auto& v : nv) r->views.push_back(std::move(v));
+    uint32_t offset = view_off;
+    for (uint32_t root = 0; root < f->node_count; ++root) {
+      if (f->nodes[root].parent != -1) continue;
+      uint32_t children = 0;
+      for (uint32_t c = 0; c < f->node_count; ++c) children += f->nodes[c].parent == (int32_t)root;
+      const char* nm = f->nodes[root].name;
+      if (!nm || !*nm)
+        return fail(GBR_ERR_IO, "addFile(): node " + std::to_string(root) + " in " + fname + " has no name");
+      if (!r->names.emplace(nm, std::make_pair(offset, offset + children)).second)
+        return fail(GBR_ERR_STATE, "addFile(): node name " + std::string(nm) + " in " + fname + " already exists");
+      offset += children;
+    }
+    if (offset != r->views.size())
+      return fail(GBR_ERR_IO, "addFile(): mesh views don't match the template children in " + fname);
+  } else {
+    std::vector<int> state(f->node_count, 0);
+    std::vector<Mat4> absm(f->node_count);
+    for (uint32_t i = 0; i < f->node_count; ++i) {
+      // parents precede children in a glTF-ordered file only by convention:
+      // resolve recursively with an explicit stack
+      std::vector<uint32_t> stack{i};
+      while (!stack.empty()) {
+        uint32_t n = stack.back();
+        if (state[n] == 2) { stack.pop_back(); continue; }
+        int32_t p = f->nodes[n].parent;
+        if (p >= (int32_t)f->node_count) return fail(GBR_ERR_IO, "addFile(): bad parent in " + fname);
+        if (p >= 0 && state[p] != 2) {
+          if (state[p] == 1) return fail(GBR_ERR_IO, "addFile(): cyclic hierarchy in " + fname);
+          state[n] = 1;
+          stack.push_back((uint32_t)p);
+          continue;
+        }
+        absm[n] = mul4(p >= 0 ? absm[p] : identity4(), load4(f->nodes[n].transform));
+        state[n] = 2;
+        stack.pop_back();
+      }
+    }
+    for (uint32_t i = 0; i < f->mesh_entry_count; ++i) nv[i].xf = absm[f->mesh_entries[i].node];
+    const uint32_t cnt = (uint32_t)nv.size();
+    for (auto& v : nv) r->views.push_back(std::move(v));
+    const std::string used = (name && *name) ? name : fname;
+    if (!r->names.emplace(used, std::make_pair(view_off, view_off + cnt)).second)
+      return fail(GBR_ERR_STATE, "addFile(): node name " + used + " already exists");
+  }
+  return GBR_OK;
+}
+
+gbr_status gbr_has_node_hierarchy(const gbr_renderer* r, const char* name, int32_t* out) {
+  if (!r || !name || !out) return fail(GBR_ERR_ARGUMENT, "gbr_has_node_hierarchy: null argument");
+  *out = r->names.count(name) ? 1 : 0;
+  return GBR_OK;
+}
+
+// Renderer::addNodeHierarchy(), Renderer.cpp:880-961
+gbr_status gbr_add_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, const char* name,
+                                           const float* bake, uint32_t semantic,
+                                           uint64_t* out_node) {
+  TRY(check_scene(r, scene, "addNodeHierarchy()"));
+  if (!name) return fail(GBR_ERR_ARGUMENT, "addNodeHierarchy(): null name");
+  auto found = r->names.find(name);
+  if (found == r->names.end())
+    return fail(GBR_ERR_NOT_FOUND, std::string("Renderer::add(): name ") + name + " not found");
+  Scene& sc = r->scenes[scene];
+  const Mat4 bk = bake ? load4(bake) : identity4();
+  const uint32_t top = (uint32_t)sc.transforms.size();
+  sc.parents.push_back(-1);
+  sc.transforms.push_back(identity4());
+  for (uint32_t i = found->second.first; i != found->second.second; ++i) {
+    const MeshView& v = r->views[i];
+    const uint32_t id = (uint32_t)sc.transforms.size();
+    sc.parents.push_back((int32_t)top);
+    sc.transforms.push_back(mul4(bk, v.xf));
+    const MaterialDesc& m = r->materials[v.material];
+    const uint32_t b = batch_id(sc, r->meshes[v.mesh].color_fmt != kColorNone ? 1u : 0u, v.mesh,
+                                (uint32_t)m.texture);
+    sc.draws.push_back(Draw{b, id, v.material, i, semantic});
+  }
+  sc.xform_dirty = true;
+  r->structure_dirty = true;
+  if (out_node) *out_node = top;
+  return GBR_OK;
+}
+
+gbr_status gbr_add_node_hierarchy(gbr_renderer* r, uint32_t scene, const char* name,
+                                  const float* bake, uint64_t* out_node) {
+  return gbr_add_node_hierarchy_semantic(r, scene, name, bake, 0u, out_node);
+}
+
+gbr_status gbr_add_empty_node(gbr_renderer* r, uint32_t scene, uint64_t* out_node) {
+  TRY(check_scene(r, scene, "addEmptyNode()"));
+  Scene& sc = r->scenes[scene];
+  sc.parents.push_back(-1);
+  sc.transforms.push_back(identity4());
+  sc.xform_dirty = true;
+  r->structure_dirty = true;  // node arrays grew
+  if (out_node) *out_node = sc.transforms.size() - 1;
+  return GBR_OK;
+}
+
+gbr_status gbr_add_light(gbr_renderer* r, uint32_t scene, uint64_t node, int32_t type,
+                         uint64_t* out) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "addLight(): null renderer");
+  if (!r->cfg.max_light_count) return fail(GBR_ERR_STATE, "Renderer::addLight(): max light count is zero");
+  TRY(check_scene(r, scene, "addLight()"));
+  (void)node;
+  (void)type;
+  (void)out;
+  return fail(GBR_ERR_UNSUPPORTED, "addLight(): lights are not supported");
+}
+
+// Renderer::clear(), Renderer.cpp:1017-1040 (camera kept)
+gbr_status gbr_clear(gbr_renderer* r, uint32_t scene) {
+  TRY(check_scene(r, scene, "Renderer::clear()"));
+  Scene& sc = r->scenes[scene];
+  sc.parents.clear();
+  sc.transforms.clear();
+  sc.lights.clear();
+  sc.draws.clear();
+  sc.batches.clear();
+  r->structure_dirty = true;
+  return GBR_OK;
+}
+
+gbr_status gbr_clear_lights(gbr_renderer* r, uint32_t scene) {
+  TRY(check_scene(r, scene, "Renderer::clearLights()"));
+  r->scenes[scene].lights.clear();
+  return GBR_OK;
+}
+
+gbr_status gbr_camera(const gbr_renderer* r, uint32_t scene, float* out16) {
+  TRY(check_scene(r, scene, "Renderer::camera()"));
+  if (!out16) return fail(GBR_ERR_ARGUMENT, "camera(): null output");
+  std::memcpy(out16, r->scenes[scene].camera.m, 64);
+  return GBR_OK;
+}
+
+gbr_status gbr_camera_depth_unprojection(const gbr_renderer* r, uint32_t scene, float* out2) {
+  TRY(check_scene(r, scene, "Renderer::cameraDepthUnprojection()"));
+  if (!out2) return fail(GBR_ERR_ARGUMENT, "cameraDepthUnprojection(): null output");
+  out2[0] = r->scenes[scene].unproj[0];
+  out2[1] = r->scenes[scene].unproj[1];
+  return GBR_OK;
+}
+
+gbr_status gbr_calculate_depth_unprojection(const float* P, float* out2) {
+  if (!P || !out2) return fail(GBR_ERR_ARGUMENT, "calculateDepthUnprojection(): null argument");
+  float a = P[2 * 4 + 2] - 1.0f;
+  float b = P[3 * 4 + 2];
+  out2[0] = a * 0.5f;
+  out2[1] = b * 0.5f;
+  return GBR_OK;
+}
+
+// Renderer::updateCamera(), Renderer.cpp:1077-1088
+gbr_status gbr_update_camera(gbr_renderer* r, uint32_t scene, const float* projection16,
+                             const float* view16) {
+  TRY(check_scene(r, scene, "Renderer::updateCamera()"));
+  if (!projection16 || !view16) return fail(GBR_ERR_ARGUMENT, "updateCamera(): null matrix");
+  Scene& sc = r->scenes[scene];
+  sc.camera = mul4(load4(projection16), load4(view16));
+  gbr_calculate_depth_unprojection(projection16, sc.unproj);
+  return GBR_OK;
+}
+
+gbr_status gbr_update_cameras(gbr_renderer* r, uint32_t first, uint32_t count,
+                              const float* projections16, const float* views16) {
+  if (!r || !projections16 || !views16) return fail(GBR_ERR_ARGUMENT, "updateCameras(): null argument");
+  if ((uint64_t)first + count > r->scenes.size())
+    return fail(GBR_ERR_RANGE, "updateCameras(): scene range out of range");
+  for (uint32_t i = 0; i < count; ++i)
+    TRY(gbr_update_camera(r, first + i, projections16 + 16 * (size_t)i, views16 + 16 * (size_t)i));
+  return GBR_OK;
+}
+
+gbr_status gbr_transformations(gbr_renderer* r, uint32_t scene, float** out_data,
+                               uint64_t* out_count) {
+  TRY(check_scene(r, scene, "Renderer::transformations()"));
+  if (!out_data || !out_count) return fail(GBR_ERR_ARGUMENT, "transformations(): null output");
+  Scene& sc = r->scenes[scene];
+  *out_data = sc.transforms.empty() ? nullptr : sc.transforms.data()->m;
+  *out_count = sc.transforms.size();
+  sc.xform_dirty = true;
+  return GBR_OK;
+}
+
+gbr_status gbr_mark_transformations_dirty(gbr_renderer* r, uint32_t scene) {
+  TRY(check_scene(r, scene, "markTransformationsDirty()"));
+  r->scenes[scene].xform_dirty = true;
+  return GBR_OK;
+}
+
+gbr_status gbr_scene_stats(const gbr_renderer* r, uint32_t scene, gbr_scene_stats_info* out) {
+  TRY(check_scene(r, scene, "Renderer::sceneStats()"));
+  if (!out) return fail(GBR_ERR_ARGUMENT, "sceneStats(): null output");
+  const Scene& sc = r->scenes[scene];
+  out->node_count = sc.transforms.size();
+  out->draw_count = sc.draws.size();
+  out->draw_batch_count = sc.batches.size();
+  return GBR_OK;
+}
+
+gbr_status gbr_draw(gbr_renderer* r) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "draw(): null renderer");
+  return do_draw(r);
+}
+
+gbr_status gbr_synchronize(gbr_renderer* r) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "synchronize(): null renderer");
+  TRY(set_device(r));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return GBR_OK;
+}
+
+gbr_status gbr_stream(gbr_renderer* r, void** out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "stream(): null argument");
+  *out = r->stream;
+  return GBR_OK;
+}
+
+gbr_status gbr_color_image_into(gbr_renderer* r, const int32_t* rect, void* dst, int64_t stride,
+                                int32_t format, int32_t memory) {
+  if (format != GBR_FORMAT_RGBA8 && format != GBR_FORMAT_RGB8)
+    return fail(GBR_ERR_ARGUMENT, "colorImageInto(): format must be RGBA8 or RGB8");
+  return gather_plane(r, GBR_PLANE_COLOR, rect, dst, stride, format == GBR_FORMAT_RGB8 ? 3 : 4, memory);
+}
+
+gbr_status gbr_depth_image_into(gbr_renderer* r, const int32_t* rect, float* dst, int64_t stride,
+                                int32_t memory) {
+  return gather_plane(r, GBR_PLANE_DEPTH, rect, dst, stride, 4, memory);
+}
+
+gbr_status gbr_linear_depth_image_into(gbr_renderer* r, const int32_t* rect, float* dst,
+                                       int64_t stride, int32_t memory) {
+  return gather_plane(r, GBR_PLANE_LINEAR_DEPTH, rect, dst, stride, 4, memory);
+}
+
+gbr_status gbr_semantic_image_into(gbr_renderer* r, const int32_t* rect, uint32_t* dst,
+                                   int64_t stride, int32_t memory) {
+  return gather_plane(r, GBR_PLANE_SEMANTIC, rect, dst, stride, 4, memory);
+}
+
+gbr_status gbr_color_device_pointer(gbr_renderer* r, const void** out) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "color_device_pointer: null renderer");
+  return framebuffer_pointer(r, GBR_PLANE_COLOR, r->d_fb_color, out);
+}
+
+gbr_status gbr_depth_device_pointer(gbr_renderer* r, const void** out) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "depth_device_pointer: null renderer");
+  return framebuffer_pointer(r, GBR_PLANE_DEPTH, r->d_fb_depth, out);
+}
+
+gbr_status gbr_plane_device_pointer(gbr_renderer* r, uint32_t plane, const void** out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "plane_device_pointer: null argument");
+  TRY(ensure_planes(r));
+  if (!(r->cfg.planes & plane)) return fail(GBR_ERR_STATE, "plane_device_pointer: plane not produced");
+  switch (plane) {
+    case GBR_PLANE_COLOR: *out = r->d_color.ptr; break;
+    case GBR_PLANE_DEPTH: *out = r->d_depth.ptr; break;
+    case GBR_PLANE_LINEAR_DEPTH: *out = r->d_lin.ptr; break;
+    case GBR_PLANE_SEMANTIC: *out = r->d_sem.ptr; break;
+    default: return fail(GBR_ERR_ARGUMENT, "plane_device_pointer: unknown plane");
+  }
+  return GBR_OK;
+}
+
+// DepthUnprojection.cpp:107-130
+gbr_status gbr_unproject_depth(const float* un, float* depth, int64_t rows, int64_t cols,
+                               int64_t row_stride) {
+  if (!un || (!depth && rows * cols)) return fail(GBR_ERR_ARGUMENT, "unprojectDepth(): null argument");
+  if (row_stride == 0) row_stride = cols * 4;
+  const float far_depth = un[1] / (1.0f + un[0]);
+  for (int64_t y = 0; y < rows; ++y) {
+    float* row = reinterpret_cast<float*>(reinterpret_cast<char*>(depth) + y * row_stride);
+    for (int64_t x = 0; x < cols; ++x) {
+      float d = un[1] / (row[x] + un[0]);
+      row[x] = d == far_depth ? 0.0f : d;
+    }
+  }
+  return GBR_OK;
+}
+
+gbr_status gbr_get_tuning(const gbr_renderer* r, gbr_tuning* out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "get_tuning: null argument");
+  *out = r->tun;
+  return GBR_OK;
+}
+
+gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* t) {
+  if (!r || !t) return fail(GBR_ERR_ARGUMENT, "set_tuning: null argument");
+  TRY(validate_tuning(t));
+  r->tun = *t;
+  r->structure_dirty = true;
+  return GBR_OK;
+}
+
+gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "frame_stats: null argument");
+  std::memset(out, 0, sizeof *out);
+  out->work_items = r->n_items;
+  out->triangles = r->n_tris;
+  out->raster_items = r->n_ritems;
+  if (r->d_stats.ptr) {
+    unsigned long long s[4] = {0, 0, 0, 0};
+    TRY(set_device(r));
+    HIP_TRY(hipMemcpyAsync(s, r->d_stats.ptr, sizeof s, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    out->bin_entries = s[0];
+    out->overflowed_bins = s[1];
+  }
+  return GBR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,7 @@
+"""MI355X-native batch sensor renderer: drop-in for habitat-sim's
+esp::gfx_batch::Renderer / RendererStandalone path."""
+from .renderer import (RendererConfiguration, RendererStandalone,  # noqa: F401
+                       RendererStandaloneConfiguration, SceneStats,
+                       calculate_depth_unprojection, unproject_depth)
+
+__version__ = "0.1.0"

@@ -1,0 +1,454 @@
+"""Python mirror of esp::gfx_batch::RendererConfiguration / Renderer /
+RendererStandalone (src/esp/gfx_batch/Renderer.h, RendererStandalone.h) over
+the C ABI.  Method names follow the reference in snake_case; argument meaning
+and error behaviour are the reference's (out-of-range scene ids raise, a file
+that cannot be imported makes add_file() return False).
+"""
+from __future__ import annotations
+
+import ctypes
+from collections import namedtuple
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .assets import FileData, ImportError_, load_gltf
+
+SceneStats = namedtuple("SceneStats", "node_count draw_count draw_batch_count")
+
+RendererFlags_NO_TEXTURES = N.FLAG_NO_TEXTURES
+RendererFileFlags_WHOLE = N.FILE_WHOLE
+RendererFileFlags_GENERATE_MIPMAP = N.FILE_GENERATE_MIPMAP
+
+
+class RendererConfiguration:
+    """RendererConfiguration, Renderer.h:51-113 (defaults Renderer.cpp:58-64)."""
+
+    def __init__(self):
+        self.flags = 0
+        self.tile_size = (128, 128)
+        self.tile_count = (1, 1)
+        self.max_light_count = 0
+        self.ambient_factor = 0.1
+
+    def set_flags(self, flags: int) -> "RendererConfiguration":
+        self.flags = int(flags)
+        return self
+
+    def set_tile_size_count(self, tile_size, tile_count) -> "RendererConfiguration":
+        self.tile_size = (int(tile_size[0]), int(tile_size[1]))
+        self.tile_count = (int(tile_count[0]), int(tile_count[1]))
+        return self
+
+    def set_max_light_count(self, count: int) -> "RendererConfiguration":
+        self.max_light_count = int(count)
+        return self
+
+    def set_ambient_factor(self, factor: float) -> "RendererConfiguration":
+        self.ambient_factor = float(factor)
+        return self
+
+
+class RendererStandaloneConfiguration:
+    """RendererStandaloneConfiguration, RendererStandalone.h."""
+
+    def __init__(self):
+        self.device = -1
+        self.flags = 0
+
+    def set_cuda_device(self, device: int) -> "RendererStandaloneConfiguration":
+        """Device ordinal (the reference's CUDA device; a HIP device here)."""
+        self.device = int(device)
+        return self
+
+    def set_flags(self, flags: int) -> "RendererStandaloneConfiguration":
+        self.flags = int(flags)
+        return self
+
+
+def _mat16(m) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(m, np.float32).reshape(16))
+    return a
+
+
+class _TransformView:
+    """Mutable view of a scene's node transformations (Renderer.h:682-683):
+    ``view[i]`` is a (4, 4) float32 [col][row] array aliasing the renderer's
+    host storage; assignment writes through."""
+
+    def __init__(self, array: np.ndarray):
+        self._a = array
+
+    def __len__(self):
+        return self._a.shape[0]
+
+    def __getitem__(self, i):
+        return self._a[i]
+
+    def __setitem__(self, i, value):
+        self._a[i] = np.asarray(value, np.float32).reshape(4, 4)
+
+    def __array__(self, dtype=None):
+        return self._a if dtype is None else self._a.astype(dtype)
+
+
+class RendererStandalone:
+    """RendererStandalone (RendererStandalone.h:120-241): renderer + its own
+    framebuffer of tile_size * tile_count pixels on one MI355X."""
+
+    def __init__(self, configuration: Optional[RendererConfiguration] = None,
+                 standalone_configuration: Optional[RendererStandaloneConfiguration] = None,
+                 planes: int = N.PLANE_ALL, top_down: bool = False):
+        configuration = configuration or RendererConfiguration()
+        standalone_configuration = standalone_configuration or RendererStandaloneConfiguration()
+        lib = N.lib()
+        cfg = N.Config()
+        N.check(lib.gbr_config_init(ctypes.byref(cfg)))
+        cfg.tile_size[0], cfg.tile_size[1] = configuration.tile_size
+        cfg.tile_count[0], cfg.tile_count[1] = configuration.tile_count
+        cfg.flags = configuration.flags
+        cfg.max_light_count = configuration.max_light_count
+        cfg.ambient_factor = configuration.ambient_factor
+        cfg.device = standalone_configuration.device
+        cfg.standalone_flags = standalone_configuration.flags
+        cfg.planes = planes
+        cfg.top_down = 1 if top_down else 0
+        h = ctypes.c_void_p()
+        N.check(lib.gbr_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._cfg = cfg
+        self._keep = []
+
+    # -- lifecycle -----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            N.lib().gbr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- getters (Renderer.h:458-491) ---------------------------------------
+    @property
+    def flags(self) -> int:
+        return self._cfg.flags
+
+    @property
+    def standalone_flags(self) -> int:
+        return self._cfg.standalone_flags
+
+    @property
+    def tile_size(self):
+        return (self._cfg.tile_size[0], self._cfg.tile_size[1])
+
+    @property
+    def tile_count(self):
+        return (self._cfg.tile_count[0], self._cfg.tile_count[1])
+
+    @property
+    def scene_count(self) -> int:
+        n = ctypes.c_uint64()
+        N.call("gbr_scene_count", self._h, ctypes.byref(n))
+        return n.value
+
+    @property
+    def max_light_count(self) -> int:
+        return self._cfg.max_light_count
+
+    @property
+    def framebuffer_size(self):
+        return (self.tile_size[0] * self.tile_count[0], self.tile_size[1] * self.tile_count[1])
+
+    # -- assets ----------------------------------------------------------------
+    def add_file(self, filename: str, flags: int = 0, name: Optional[str] = None) -> bool:
+        """Renderer::addFile(); returns False if the file can't be imported."""
+        try:
+            fd = load_gltf(filename)
+        except (ImportError_, OSError, ValueError, KeyError) as e:
+            import sys
+            print(f"Renderer::addFile(): can't open the file {filename}: {e}", file=sys.stderr)
+            return False
+        return self.add_file_data(fd, flags, name, filename)
+
+    def add_file_data(self, fd: FileData, flags: int = 0, name: Optional[str] = None,
+                      filename: str = "") -> bool:
+        keep = []
+
+        def ptr(a):
+            keep.append(a)
+            return a.ctypes.data
+
+        meshes = (N.MeshDesc * max(len(fd.meshes), 1))()
+        for i, m in enumerate(fd.meshes):
+            d = meshes[i]
+            pos = np.ascontiguousarray(m.positions, np.float32)
+            d.vertex_count = len(pos)
+            d.positions = ptr(pos)
+            if m.texcoords is not None:
+                d.texcoords = ptr(np.ascontiguousarray(m.texcoords, np.float32))
+            if m.colors is not None:
+                c = m.colors
+                if c.dtype == np.uint8:
+                    d.colors = ptr(np.ascontiguousarray(c, np.uint8))
+                    d.colors_u8 = 1
+                else:
+                    d.colors = ptr(np.ascontiguousarray(c, np.float32))
+            d.texcoords_half = 1 if m.texcoord_half else 0
+            idx = np.ascontiguousarray(m.indices, np.uint32)
+            d.index_count = len(idx)
+            d.indices = ptr(idx)
+            d.source_index_size = m.index_size
+        textures = (N.TextureDesc * max(len(fd.textures), 1))()
+        for i, t in enumerate(fd.textures):
+            d = textures[i]
+            d.width, d.height, d.layers, d.levels = t.width, t.height, t.layers, len(t.levels)
+            blob = np.concatenate([np.ascontiguousarray(lv, np.uint8).reshape(-1) for lv in t.levels])
+            d.texels = ptr(blob)
+            d.min_filter, d.mag_filter = t.min_filter, t.mag_filter
+            d.wrap_s, d.wrap_t = t.wrap_s, t.wrap_t
+            d.compressed = 1 if t.compressed else 0
+        materials = (N.MaterialDesc * max(len(fd.materials), 1))()
+        for i, m in enumerate(fd.materials):
+            d = materials[i]
+            d.flat = 1 if m.flat else 0
+            for k in range(4):
+                d.base_color[k] = m.base_color[k]
+            d.texture = m.texture
+            d.layer = m.layer
+            tm = np.asarray(m.texture_matrix, np.float32).reshape(9)
+            for k in range(9):
+                d.texture_matrix[k] = tm[k]
+        nodes = (N.NodeDesc * max(len(fd.nodes), 1))()
+        names = []
+        for i, n in enumerate(fd.nodes):
+            d = nodes[i]
+            d.parent = n.parent
+            tf = np.asarray(n.transform, np.float32).reshape(16)
+            for k in range(16):
+                d.transform[k] = tf[k]
+            if n.name is not None:
+                b = n.name.encode()
+                names.append(b)
+                d.name = b
+        entries = (N.MeshEntryDesc * max(len(fd.mesh_entries), 1))()
+        for i, e in enumerate(fd.mesh_entries):
+            d = entries[i]
+            d.node, d.mesh, d.material = e.node, e.mesh, e.material
+            if e.view is not None:
+                d.has_view = 1
+                d.view_index_offset_bytes, d.view_index_count, d.view_material = e.view
+        f = N.FileDesc()
+        f.mesh_count, f.meshes = len(fd.meshes), meshes
+        f.texture_count, f.textures = len(fd.textures), textures
+        f.material_count, f.materials = len(fd.materials), materials
+        f.node_count, f.nodes = len(fd.nodes), nodes
+        f.mesh_entry_count, f.mesh_entries = len(fd.mesh_entries), entries
+        f.has_scene = 1 if fd.has_scene else 0
+        st = N.lib().gbr_add_file_data(self._h, ctypes.byref(f), flags,
+                                       name.encode() if name else None,
+                                       filename.encode() if filename else None)
+        if st in (N.ERR_IO, N.ERR_STATE, N.ERR_ARGUMENT):
+            import sys
+            print(N.lib().gbr_last_error().decode(), file=sys.stderr)
+            return False
+        N.check(st)
+        return True
+
+    def has_node_hierarchy(self, name: str) -> bool:
+        out = ctypes.c_int32()
+        N.call("gbr_has_node_hierarchy", self._h, name.encode(), ctypes.byref(out))
+        return bool(out.value)
+
+    # -- scenes ------------------------------------------------------------------
+    def add_node_hierarchy(self, scene: int, name: str, bake=None, semantic_id: int = 0) -> int:
+        out = ctypes.c_uint64()
+        b = None if bake is None else _mat16(bake)
+        N.call("gbr_add_node_hierarchy_semantic", self._h, scene, name.encode(),
+               None if b is None else b.ctypes.data, semantic_id, ctypes.byref(out))
+        return out.value
+
+    def add_empty_node(self, scene: int) -> int:
+        out = ctypes.c_uint64()
+        N.call("gbr_add_empty_node", self._h, scene, ctypes.byref(out))
+        return out.value
+
+    def add_light(self, scene: int, node: int, light_type: int) -> int:
+        out = ctypes.c_uint64()
+        N.call("gbr_add_light", self._h, scene, node, light_type, ctypes.byref(out))
+        return out.value
+
+    def clear(self, scene: int) -> None:
+        N.call("gbr_clear", self._h, scene)
+
+    def clear_lights(self, scene: int) -> None:
+        N.call("gbr_clear_lights", self._h, scene)
+
+    def camera(self, scene: int) -> np.ndarray:
+        out = np.empty(16, np.float32)
+        N.call("gbr_camera", self._h, scene, out.ctypes.data)
+        return out.reshape(4, 4)
+
+    def camera_depth_unprojection(self, scene: int) -> np.ndarray:
+        out = np.empty(2, np.float32)
+        N.call("gbr_camera_depth_unprojection", self._h, scene, out.ctypes.data)
+        return out
+
+    def update_camera(self, scene: int, projection, view) -> None:
+        p, v = _mat16(projection), _mat16(view)
+        N.call("gbr_update_camera", self._h, scene, p.ctypes.data, v.ctypes.data)
+
+    def update_cameras(self, first: int, projections, views) -> None:
+        p = np.ascontiguousarray(projections, np.float32).reshape(-1, 16)
+        v = np.ascontiguousarray(views, np.float32).reshape(-1, 16)
+        assert len(p) == len(v)
+        N.call("gbr_update_cameras", self._h, first, len(p), p.ctypes.data, v.ctypes.data)
+
+    def transformations(self, scene: int) -> _TransformView:
+        data = N.c_float_p()
+        count = ctypes.c_uint64()
+        N.call("gbr_transformations", self._h, scene, ctypes.byref(data), ctypes.byref(count))
+        if count.value == 0:
+            return _TransformView(np.zeros((0, 4, 4), np.float32))
+        arr = np.ctypeslib.as_array(data, shape=(count.value, 4, 4))
+        return _TransformView(arr)
+
+    def scene_stats(self, scene: int) -> SceneStats:
+        s = N.SceneStatsInfo()
+        N.call("gbr_scene_stats", self._h, scene, ctypes.byref(s))
+        return SceneStats(s.node_count, s.draw_count, s.draw_batch_count)
+
+    # -- rendering ---------------------------------------------------------------
+    def draw(self) -> None:
+        N.call("gbr_draw", self._h)
+
+    def synchronize(self) -> None:
+        N.call("gbr_synchronize", self._h)
+
+    def stream(self) -> int:
+        out = ctypes.c_void_p()
+        N.call("gbr_stream", self._h, ctypes.byref(out))
+        return out.value or 0
+
+    def _rect(self, rect):
+        if rect is None:
+            w, h = self.framebuffer_size
+            rect = (0, 0, w, h)
+        return np.asarray(rect, np.int32)
+
+    def color_image_into(self, rect, image: np.ndarray) -> None:
+        r = self._rect(rect)
+        fmt = N.FORMAT_RGB8 if image.shape[-1] == 3 else N.FORMAT_RGBA8
+        self._check_size(r, image)
+        N.call("gbr_color_image_into", self._h, r.ctypes.data, image.ctypes.data,
+               image.strides[0], fmt, N.MEMORY_HOST)
+
+    def depth_image_into(self, rect, image: np.ndarray) -> None:
+        r = self._rect(rect)
+        self._check_size(r, image)
+        N.call("gbr_depth_image_into", self._h, r.ctypes.data, image.ctypes.data,
+               image.strides[0], N.MEMORY_HOST)
+
+    def linear_depth_image_into(self, rect, image: np.ndarray) -> None:
+        r = self._rect(rect)
+        self._check_size(r, image)
+        N.call("gbr_linear_depth_image_into", self._h, r.ctypes.data, image.ctypes.data,
+               image.strides[0], N.MEMORY_HOST)
+
+    def semantic_image_into(self, rect, image: np.ndarray) -> None:
+        r = self._rect(rect)
+        self._check_size(r, image)
+        N.call("gbr_semantic_image_into", self._h, r.ctypes.data, image.ctypes.data,
+               image.strides[0], N.MEMORY_HOST)
+
+    @staticmethod
+    def _check_size(r, image):
+        if image.shape[0] != r[3] - r[1] or image.shape[1] != r[2] - r[0]:
+            raise N.GbrError(N.ERR_SIZE, f"expected image size of {(r[2]-r[0], r[3]-r[1])} pixels "
+                                         f"but got {(image.shape[1], image.shape[0])}")
+        if not image.flags.writeable or image.strides[-1] != image.itemsize and image.ndim == 2:
+            raise N.GbrError(N.ERR_ARGUMENT, "image must be writeable with contiguous rows")
+
+    def color_image(self) -> np.ndarray:
+        w, h = self.framebuffer_size
+        out = np.empty((h, w, 4), np.uint8)
+        self.color_image_into(None, out)
+        return out
+
+    def depth_image(self) -> np.ndarray:
+        w, h = self.framebuffer_size
+        out = np.empty((h, w), np.float32)
+        self.depth_image_into(None, out)
+        return out
+
+    def linear_depth_image(self) -> np.ndarray:
+        w, h = self.framebuffer_size
+        out = np.empty((h, w), np.float32)
+        self.linear_depth_image_into(None, out)
+        return out
+
+    def semantic_image(self) -> np.ndarray:
+        w, h = self.framebuffer_size
+        out = np.empty((h, w), np.uint32)
+        self.semantic_image_into(None, out)
+        return out
+
+    def color_device_pointer(self) -> int:
+        out = ctypes.c_void_p()
+        N.call("gbr_color_device_pointer", self._h, ctypes.byref(out))
+        return out.value
+
+    def depth_device_pointer(self) -> int:
+        out = ctypes.c_void_p()
+        N.call("gbr_depth_device_pointer", self._h, ctypes.byref(out))
+        return out.value
+
+    def plane_device_pointer(self, plane: int) -> int:
+        out = ctypes.c_void_p()
+        N.call("gbr_plane_device_pointer", self._h, plane, ctypes.byref(out))
+        return out.value
+
+    # -- tuning / counters ---------------------------------------------------------
+    def tuning(self) -> N.Tuning:
+        t = N.Tuning()
+        N.call("gbr_get_tuning", self._h, ctypes.byref(t))
+        return t
+
+    def set_tuning(self, **kw) -> None:
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        N.call("gbr_set_tuning", self._h, ctypes.byref(t))
+
+    def frame_stats(self) -> dict:
+        s = N.FrameStatsInfo()
+        N.call("gbr_frame_stats", self._h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+def calculate_depth_unprojection(projection) -> np.ndarray:
+    """calculateDepthUnprojection(), DepthUnprojection.cpp:98-101."""
+    p = _mat16(projection)
+    out = np.empty(2, np.float32)
+    N.call("gbr_calculate_depth_unprojection", p.ctypes.data, out.ctypes.data)
+    return out
+
+
+def unproject_depth(unprojection, depth: np.ndarray) -> np.ndarray:
+    """unprojectDepth(), DepthUnprojection.cpp:107-130 (in place on host
+    memory; returns the array)."""
+    u = np.ascontiguousarray(unprojection, np.float32)
+    if depth.dtype != np.float32 or depth.ndim != 2 or depth.strides[1] != 4:
+        raise ValueError("depth must be a 2D float32 array with contiguous rows")
+    N.call("gbr_unproject_depth", u.ctypes.data, depth.ctypes.data, depth.shape[0],
+           depth.shape[1], depth.strides[0])
+    return depth

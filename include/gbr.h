@@ -1,0 +1,299 @@
+/*
+ * gbr.h -- C ABI of the MI355X batch sensor renderer.
+ *
+ * Drop-in for habitat-sim's esp::gfx_batch::Renderer / RendererStandalone
+ * path (src/esp/gfx_batch/Renderer.h, RendererStandalone.h,
+ * DepthUnprojection.h).  Every entry point below names the reference method it
+ * replaces (file:line relative to the habitat-sim source tree).  Entry points
+ * marked "new" have no counterpart in the reference.
+ *
+ * Conventions
+ *  - All functions return a gbr_status; 0 is success.  gbr_last_error()
+ *    returns a thread-local message for the last failure.  No C++ exception
+ *    crosses this boundary.
+ *  - Matrices are 16 floats, column-major ([col*4 + row]), Magnum layout.
+ *  - Images are bottom-up (first row = GL window y 0), like the reference's
+ *    framebuffer reads, unless gbr_config.top_down is set for the per-scene
+ *    planes.
+ *  - A renderer owns one HIP stream; calls on one renderer must be serialised
+ *    by the caller.  gbr_draw() is asynchronous; the *_into() readbacks
+ *    synchronise the stream (the reference's "stalls the CPU" contract,
+ *    RendererStandalone.h:174,196).
+ */
+#ifndef GBR_H
+#define GBR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBR_ABI_VERSION 1
+
+typedef int32_t gbr_status;
+enum {
+  GBR_OK = 0,
+  GBR_ERR_ARGUMENT = 1,    /* null pointer, invalid flag or value */
+  GBR_ERR_RANGE = 2,       /* scene / node / light index out of range */
+  GBR_ERR_NOT_FOUND = 3,   /* unknown node hierarchy name */
+  GBR_ERR_IO = 4,          /* file can't be opened or imported */
+  GBR_ERR_SIZE = 5,        /* rectangle / image size mismatch */
+  GBR_ERR_DEVICE = 6,      /* HIP runtime failure */
+  GBR_ERR_OOM = 7,         /* device or host allocation failed */
+  GBR_ERR_UNSUPPORTED = 8, /* feature not provided by this build */
+  GBR_ERR_STATE = 9        /* conflicting state, e.g. duplicate name */
+};
+
+/* RendererFlag, Renderer.h:20-30 */
+#define GBR_FLAG_NO_TEXTURES 0x1u
+/* RendererStandaloneFlag, RendererStandalone.h */
+#define GBR_STANDALONE_QUIET_LOG 0x1u
+/* RendererFileFlag, Renderer.h:120-154 */
+#define GBR_FILE_WHOLE 0x1u
+#define GBR_FILE_GENERATE_MIPMAP 0x2u
+/* RendererLightType, Renderer.h:170-173 */
+#define GBR_LIGHT_DIRECTIONAL 0
+#define GBR_LIGHT_POINT 1
+
+/* Output planes produced by gbr_draw() (new; the reference has colour and
+   window depth only, RendererStandalone.cpp:123-124) */
+#define GBR_PLANE_COLOR 0x1u        /* RGBA8 */
+#define GBR_PLANE_DEPTH 0x2u        /* window depth, Depth32F */
+#define GBR_PLANE_LINEAR_DEPTH 0x4u /* unprojectDepth() result, far -> 0 */
+#define GBR_PLANE_SEMANTIC 0x8u     /* uint32 semantic id, 0 = background */
+#define GBR_PLANE_ALL 0xfu
+
+/* pixel formats accepted by gbr_color_image_into() */
+#define GBR_FORMAT_RGBA8 0
+#define GBR_FORMAT_RGB8 1
+
+/* where a destination pointer lives */
+#define GBR_MEMORY_HOST 0
+#define GBR_MEMORY_DEVICE 1
+
+/* culling modes (new) */
+#define GBR_CULL_NONE 0    /* the reference's method: every triangle set up */
+#define GBR_CULL_FRUSTUM 1 /* per-chunk frustum test before setup (exact) */
+
+typedef struct gbr_renderer gbr_renderer;
+
+/* RendererConfiguration (Renderer.h:51-113, defaults Renderer.cpp:58-64)
+   + RendererStandaloneConfiguration (RendererStandalone.h) */
+typedef struct gbr_config {
+  int32_t tile_size[2];   /* default {128, 128} */
+  int32_t tile_count[2];  /* default {1, 1}; scene count = product */
+  uint32_t flags;         /* GBR_FLAG_* */
+  uint32_t max_light_count; /* only 0 (flat shading) is supported */
+  float ambient_factor;   /* default 0.1 */
+  int32_t device;         /* HIP device ordinal; -1 = current device */
+  uint32_t standalone_flags; /* GBR_STANDALONE_* */
+  uint32_t planes;        /* GBR_PLANE_* produced by draw(); default ALL */
+  uint32_t top_down;      /* new: per-scene planes stored top-down */
+  uint32_t reserved[7];
+} gbr_config;
+
+/* SceneStats, Renderer.h:750-774 */
+typedef struct gbr_scene_stats_info {
+  uint64_t node_count;
+  uint64_t draw_count;
+  uint64_t draw_batch_count;
+} gbr_scene_stats_info;
+
+/* new: launch parameters, chosen by the autotuner */
+typedef struct gbr_tuning {
+  int32_t bin_size;           /* screen bin edge in pixels: 8, 16 or 32 */
+  int32_t chunk_triangles;    /* triangles per setup work item: 64..256 */
+  float bin_capacity_factor;  /* per-bin entry capacity relative to tris/bins */
+  int32_t min_bin_capacity;   /* lower bound of the per-bin capacity */
+  int32_t cull_mode;          /* GBR_CULL_* */
+  int32_t reserved[3];
+} gbr_tuning;
+
+/* new: per-frame counters of the last gbr_draw() (read after a sync) */
+typedef struct gbr_frame_stats_info {
+  uint64_t work_items;        /* setup work items launched */
+  uint64_t triangles;         /* triangles submitted */
+  uint64_t bin_entries;       /* triangle-bin pairs written */
+  uint64_t overflowed_bins;   /* bins that exceeded capacity (exact fallback) */
+  uint64_t raster_items;      /* bins rasterised */
+} gbr_frame_stats_info;
+
+/* ---- parsed-file description consumed by gbr_add_file_data() (new) ---- */
+typedef struct gbr_mesh_desc {
+  uint32_t vertex_count;
+  const float* positions;   /* [vertex_count*3] */
+  const float* texcoords;   /* [vertex_count*2] or NULL; Y-up convention */
+  const void* colors;       /* [vertex_count*4] float or uint8, or NULL */
+  uint32_t colors_u8;       /* colors are uint8 (normalised) */
+  uint32_t texcoords_half;  /* store texture coordinates as fp16 */
+  uint32_t index_count;     /* triangle-list indices */
+  const uint32_t* indices;
+  uint32_t source_index_size; /* bytes per index in the source file */
+} gbr_mesh_desc;
+
+typedef struct gbr_texture_desc {
+  uint32_t width, height, layers, levels;  /* level 0 size */
+  const uint8_t* texels;    /* RGBA8, levels concatenated, each [layers][h][w] */
+  uint32_t min_filter, mag_filter, wrap_s, wrap_t; /* GL enums */
+  uint32_t compressed;      /* source was block-compressed: no mip generation */
+} gbr_texture_desc;
+
+typedef struct gbr_material_desc {
+  uint32_t flat;            /* Trade::MaterialType::Flat */
+  float base_color[4];
+  int32_t texture;          /* file-local texture id or -1 */
+  uint32_t layer;
+  float texture_matrix[9];  /* column-major 3x3 */
+} gbr_material_desc;
+
+typedef struct gbr_node_desc {
+  int32_t parent;           /* file-local node id or -1 */
+  float transform[16];
+  const char* name;         /* NULL if unnamed */
+} gbr_node_desc;
+
+typedef struct gbr_mesh_entry_desc {
+  uint32_t node;
+  uint32_t mesh;
+  int32_t material;         /* material of the mesh primitive, -1 none */
+  uint32_t has_view;        /* meshViewIndexOffset/Count/Material present */
+  uint32_t view_index_offset_bytes;
+  uint32_t view_index_count;
+  int32_t view_material;
+} gbr_mesh_entry_desc;
+
+typedef struct gbr_file_desc {
+  uint32_t mesh_count;
+  const gbr_mesh_desc* meshes;
+  uint32_t texture_count;
+  const gbr_texture_desc* textures;
+  uint32_t material_count;
+  const gbr_material_desc* materials;
+  uint32_t node_count;
+  const gbr_node_desc* nodes;
+  uint32_t mesh_entry_count;
+  const gbr_mesh_entry_desc* mesh_entries;
+  uint32_t has_scene;
+} gbr_file_desc;
+
+/* ---- library ------------------------------------------------------------ */
+int32_t gbr_abi_version(void);
+const char* gbr_last_error(void);
+gbr_status gbr_config_init(gbr_config* config);           /* RendererConfiguration() defaults */
+gbr_status gbr_tuning_init(gbr_tuning* tuning);
+
+/* ---- lifecycle: Renderer(const RendererConfiguration&) Renderer.h:445-448,
+   RendererStandalone ctor RendererStandalone.cpp:115-135, destroy() :735-737 */
+gbr_status gbr_create(const gbr_config* config, gbr_renderer** out);
+gbr_status gbr_destroy(gbr_renderer* r);
+
+/* getters: flags() tileSize() tileCount() sceneCount() maxLightCount()
+   Renderer.h:458-491, standaloneFlags() RendererStandalone.h */
+gbr_status gbr_get_config(const gbr_renderer* r, gbr_config* out);
+gbr_status gbr_scene_count(const gbr_renderer* r, uint64_t* out);
+
+/* ---- assets: addFile() Renderer.h:517-556 / Renderer.cpp:354-867 ---------- */
+/* Parses glTF 2.0 (+MAGNUMX_mesh_views, KHR_texture_ktx) natively is not yet
+   provided: returns GBR_ERR_UNSUPPORTED; use gbr_add_file_data(). */
+gbr_status gbr_add_file(gbr_renderer* r, const char* filename, uint32_t file_flags,
+                        const char* name);
+/* new: addFile() on an already-parsed file; `filename` is the name used for
+   GBR_FILE_WHOLE when `name` is NULL or empty.  GBR_ERR_IO / GBR_ERR_STATE
+   correspond to the reference returning false. */
+gbr_status gbr_add_file_data(gbr_renderer* r, const gbr_file_desc* file, uint32_t file_flags,
+                             const char* name, const char* filename);
+/* hasNodeHierarchy() Renderer.h:564 */
+gbr_status gbr_has_node_hierarchy(const gbr_renderer* r, const char* name, int32_t* out);
+
+/* ---- scenes ---------------------------------------------------------------- */
+/* addNodeHierarchy(scene, name, bakeTransformation) Renderer.h:586-596;
+   bake may be NULL (identity). */
+gbr_status gbr_add_node_hierarchy(gbr_renderer* r, uint32_t scene, const char* name,
+                                  const float* bake, uint64_t* out_node);
+/* new: same, every draw of the hierarchy carries `semantic_id` */
+gbr_status gbr_add_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, const char* name,
+                                           const float* bake, uint32_t semantic_id,
+                                           uint64_t* out_node);
+/* addEmptyNode() Renderer.h:598 */
+gbr_status gbr_add_empty_node(gbr_renderer* r, uint32_t scene, uint64_t* out_node);
+/* addLight() Renderer.h:619-621: needs max_light_count > 0 (not supported) */
+gbr_status gbr_add_light(gbr_renderer* r, uint32_t scene, uint64_t node, int32_t type,
+                         uint64_t* out_light);
+/* clear() Renderer.h:631, clearLights() Renderer.h:640 */
+gbr_status gbr_clear(gbr_renderer* r, uint32_t scene);
+gbr_status gbr_clear_lights(gbr_renderer* r, uint32_t scene);
+
+/* ---- cameras and transformations ----------------------------------------- */
+/* camera() Renderer.h:647: combined projection*view */
+gbr_status gbr_camera(const gbr_renderer* r, uint32_t scene, float* out16);
+/* cameraDepthUnprojection() Renderer.h:653 */
+gbr_status gbr_camera_depth_unprojection(const gbr_renderer* r, uint32_t scene, float* out2);
+/* updateCamera(scene, projection, view) Renderer.h:665-667 */
+gbr_status gbr_update_camera(gbr_renderer* r, uint32_t scene, const float* projection16,
+                             const float* view16);
+/* new: updateCamera() for `count` consecutive scenes from arrays of matrices */
+gbr_status gbr_update_cameras(gbr_renderer* r, uint32_t first_scene, uint32_t count,
+                              const float* projections16, const float* views16);
+/* transformations(scene) Renderer.h:682-683: mutable host view of the local
+   node matrices (16 floats each, contiguous), valid until the next structural
+   change of the scene (add*, clear).  Obtaining the view marks the scene's
+   transformations for upload at the next gbr_draw(). */
+gbr_status gbr_transformations(gbr_renderer* r, uint32_t scene, float** out_data,
+                               uint64_t* out_count);
+/* new: mark transformations modified through a view obtained earlier */
+gbr_status gbr_mark_transformations_dirty(gbr_renderer* r, uint32_t scene);
+/* sceneStats() Renderer.h:730 */
+gbr_status gbr_scene_stats(const gbr_renderer* r, uint32_t scene, gbr_scene_stats_info* out);
+
+/* ---- rendering ------------------------------------------------------------- */
+/* RendererStandalone::draw() RendererStandalone.cpp:155-159 (clear + draw of
+   every scene), asynchronous on the renderer's stream */
+gbr_status gbr_draw(gbr_renderer* r);
+/* new: wait for the renderer's stream */
+gbr_status gbr_synchronize(gbr_renderer* r);
+/* new: the renderer's hipStream_t */
+gbr_status gbr_stream(gbr_renderer* r, void** out_stream);
+
+/* colorImageInto() RendererStandalone.cpp:168-180: rect = {x0, y0, x1, y1}
+   in framebuffer pixels (tile grid, bottom-up); dst has rect size, rows
+   `row_stride` bytes apart.  format GBR_FORMAT_RGBA8 or RGB8. */
+gbr_status gbr_color_image_into(gbr_renderer* r, const int32_t* rect, void* dst,
+                                int64_t row_stride, int32_t format, int32_t memory);
+/* depthImageInto() RendererStandalone.cpp:189-201 (window depth) */
+gbr_status gbr_depth_image_into(gbr_renderer* r, const int32_t* rect, float* dst,
+                                int64_t row_stride, int32_t memory);
+/* new: unprojected depth (depthImageInto() + unprojectDepth(), as
+   BatchReplayRenderer::doRender() does, BatchReplayRenderer.cpp:139-148) */
+gbr_status gbr_linear_depth_image_into(gbr_renderer* r, const int32_t* rect, float* dst,
+                                       int64_t row_stride, int32_t memory);
+/* new: semantic ids (classic R32UI convention, RenderTarget.cpp:78,190) */
+gbr_status gbr_semantic_image_into(gbr_renderer* r, const int32_t* rect, uint32_t* dst,
+                                   int64_t row_stride, int32_t memory);
+/* colorCudaBufferDevicePointer() / depthCudaBufferDevicePointer()
+   RendererStandalone.h:225-235: linearised framebuffer (tile grid, bottom-up)
+   in device memory, valid until the next gbr_draw() / gbr_destroy() */
+gbr_status gbr_color_device_pointer(gbr_renderer* r, const void** out);
+gbr_status gbr_depth_device_pointer(gbr_renderer* r, const void** out);
+/* new: per-scene planes, [scene][tile_h][tile_w] (RGBA8 / float / uint32),
+   written in place by gbr_draw(); no copy */
+gbr_status gbr_plane_device_pointer(gbr_renderer* r, uint32_t plane, const void** out);
+
+/* ---- depth unprojection: DepthUnprojection.h:137-153 ----------------------- */
+/* calculateDepthUnprojection() DepthUnprojection.cpp:98-101 */
+gbr_status gbr_calculate_depth_unprojection(const float* projection16, float* out2);
+/* unprojectDepth() DepthUnprojection.cpp:107-130 on host memory */
+gbr_status gbr_unproject_depth(const float* unprojection2, float* depth, int64_t rows,
+                               int64_t cols, int64_t row_stride);
+
+/* ---- tuning and counters (new) --------------------------------------------- */
+gbr_status gbr_get_tuning(const gbr_renderer* r, gbr_tuning* out);
+gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* tuning);
+gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBR_H */

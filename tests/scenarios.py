@@ -138,7 +138,7 @@ def single_mesh(make, files, thresholds):
     checks.append(("stats", tuple(r.scene_stats(0)), (2, 1, 1)))
     t = r.transformations(0)
     checks.append(("len(transformations)", len(t), 2))
-    checks.append(("transformations[0]", np.asarray(t[0]), m3.identity()))
+    checks.append(("transformations[0]", np.array(t[0]), m3.identity()))
     t[0] = m3.scaling(2.0)
     r.draw()
     color = r.color_image()
