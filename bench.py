@@ -125,6 +125,18 @@ def cpu_baseline(spec, seconds=12.0, max_seconds=30.0):
                       f"tris each), {frames // n} passes, {el:.1f} s, all planes"}
 
 
+def copy_probe(nbytes: int) -> None:
+    """Device-to-device copies of a known size: reads nbytes, writes nbytes."""
+    from habitat_sim_amd import hip
+    a, b = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+    for _ in range(3):
+        hip.memcpy_d2d(b.value, a.value, nbytes)
+    hip.hip().hipDeviceSynchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}))
+    a.close()
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +146,9 @@ def main():
     ap.add_argument("--res", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (the reference kernel "
+                         "for scaling hardware byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -142,6 +157,9 @@ def main():
 
     import __graft_entry__
     __graft_entry__.ensure_built()
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
+        return
     import numpy as np
     from habitat_sim_amd import RendererConfiguration, RendererStandalone, hip, scenes
     from habitat_sim_amd import _native as N

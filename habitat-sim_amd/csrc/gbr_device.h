@@ -9,7 +9,10 @@ namespace gbr {
 
 constexpr int kBlock = 256;        // threads per workgroup, all kernels
 constexpr int kMaxBin = 32;        // largest bin edge (LDS visibility tile)
-constexpr int kVertexCache = 1024; // vertices per setup work item held in LDS
+constexpr int kChunkTris = 1024;   // triangles per setup work item (4 per lane)
+constexpr int kVertexCache = 640;  // vertices per setup work item held in LDS
+constexpr int kMaxSceneBins = 1024;  // bins per scene (LDS counters in setup, <= 2^11)
+constexpr int kPairCap = 2048;     // (bin, triangle) pairs aggregated in LDS per item (<= 2^11)
 constexpr int kBigQueue = 192;     // cooperative-raster queue entries per bin
 constexpr int kDrawCache = 1024;   // draw prim bases cached in LDS per bin
 constexpr float kSubpix = 256.0f;  // 8 sub-pixel bits
@@ -38,7 +41,7 @@ struct alignas(16) DrawDesc {
   uint32_t semantic;
   uint32_t flags;       // color format | uv format
   uint32_t prim_base;   // scene-local primitive order of the first triangle
-  uint32_t pad;
+  uint32_t scene;
 };
 
 struct alignas(16) SceneDesc {
@@ -51,17 +54,64 @@ struct alignas(16) SceneDesc {
   uint32_t plane_offset;  // pixel offset of this scene's planes
   uint32_t prim_count;
   unsigned long long entry_base;  // first bin entry of this scene (bins x bin_cap)
+  unsigned long long rec_base;    // first primitive record of this scene
+  unsigned long long pad3;
 };
 
+// One chunk of <= kChunkTris consecutive triangles of a draw; everything a
+// setup workgroup needs comes from this one 32-byte load.
 struct alignas(16) WorkItem {
   uint32_t scene;
-  uint32_t draw;        // global draw index
-  uint32_t tri_start;   // within the draw
+  uint32_t draw;         // global draw index
+  uint32_t first_index;  // index of the chunk's first triangle in indices[]
   uint32_t tri_count;
-  uint32_t vmin;        // global vertex index range referenced by the chunk
+  uint32_t prim;         // scene-local primitive order of the first triangle
+  uint32_t vmin;         // global vertex index range referenced by the chunk
   uint32_t vcount;
-  uint32_t pad[2];
+  uint32_t node;         // global node index of the draw (absolute transform)
 };
+
+// Per-primitive records written by setup for triangles that reach a bin,
+// indexed by (scene record base + primitive order).  Bin entries are the
+// primitive order, or kSubTri | index of a clipped fan triangle in the
+// sub-triangle buffer.
+constexpr uint32_t kSubTri = 0x80000000u;
+
+struct alignas(16) RasterRec {
+  int32_t X0, Y0, X1, Y1;
+  int32_t X2, Y2;
+  uint32_t prim;    // primitive order
+  uint32_t draw;    // global draw index
+  float z0, dzdx, dzdy, pad2;
+};
+
+// A primitive's record: the raster part (snapped window coordinates and
+// depth plane) followed by the resolve part (clip-space x, y, w and the
+// texture coordinates of the unclipped triangle).
+struct alignas(16) PrimRec {
+  RasterRec r;
+  float cx0, cy0, cw0, cx1;
+  float cy1, cw1, cx2, cy2;
+  float cw2, s0, t0, s1;
+  float t1, s2, t2, pad;
+};
+
+// A primitive that needs clipping (crosses the near/far plane or the guard band).
+struct alignas(8) PrimRef {
+  uint32_t draw;  // global draw index
+  uint32_t prim;  // scene-local primitive order
+};
+
+// device-side counters, reset by k_nodes at the start of every frame; each
+// on its own 128-byte line (index = counter * kCtrStride)
+enum : uint32_t {
+  kCtrClipped = 0,
+  kCtrSubTris = 1,
+  kCtrSetupQueue = 2,  // next setup work item (dynamic work queue)
+  kCtrCount = 3
+};
+constexpr uint32_t kCtrStride = 32;
+
 
 struct alignas(16) RasterItem {
   uint32_t scene;
@@ -108,6 +158,14 @@ struct FrameParams {
   // binning
   uint32_t* bin_count;
   uint32_t* bin_entries;
+  PrimRec* recs;                // per primitive, indexed by rec_base + prim
+  RasterRec* sub_recs;          // clipped fan triangles
+  PrimRef* clipped;             // primitives that need clipping
+  uint32_t* counters;           // kCtr*
+  unsigned long long* vis;      // per-pixel visibility keys, scene planes
+  uint32_t clipped_cap, sub_cap;
+  uint32_t queue_grain;         // setup work items fetched per queue atomic
+  uint32_t pad_;
   // outputs (env-major planes), any may be null
   uint32_t* out_color;
   float* out_depth;

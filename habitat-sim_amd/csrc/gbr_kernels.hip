@@ -215,8 +215,10 @@ __device__ void mat4_mul(const float* a, const float* b, float* out) {
 
 __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ parents,
                                                   const float* __restrict__ local,
-                                                  float* __restrict__ abs_out, uint32_t n) {
+                                                  float* __restrict__ abs_out, uint32_t n,
+                                                  uint32_t* __restrict__ counters) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < kCtrCount) counters[i * kCtrStride] = 0u;  // first kernel of every frame
   if (i >= n) return;
   int depth = 0;
   for (int32_t q = parents[i]; q >= 0 && depth < 1024; q = parents[q]) ++depth;
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
 }
 
 // --------------------------------------------------------------------------
-// k_setup_bin
+// Edge functions and depth plane (shared by setup and raster)
 // --------------------------------------------------------------------------
 
 __device__ __forceinline__ void load_mat(const float* src, float* m) {
@@ -244,157 +246,13 @@ __device__ __forceinline__ void load_mat(const float* src, float* m) {
   for (int k = 0; k < 16; ++k) m[k] = src[k];
 }
 
-__global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
-                                                      const WorkItem* __restrict__ items) {
-  __shared__ int32_t s_x[kVertexCache];
-  __shared__ int32_t s_y[kVertexCache];
-  __shared__ uint32_t s_f[kVertexCache];
-
-  const WorkItem wi = items[blockIdx.x];
-  const DrawDesc d = p.draws[wi.draw];
-  const SceneDesc sc = p.scenes[wi.scene];
-  float M[16], PV[16];
-  load_mat(p.abs_nodes + (size_t)d.node * 16, M);
-  load_mat(p.cameras + (size_t)wi.scene * 16, PV);
-  const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
-  const float gx = kGuardPx / hw, gy = kGuardPx / hh;
-  const bool cached = wi.vcount <= (uint32_t)kVertexCache;
-
-  if (cached) {
-    for (uint32_t v = threadIdx.x; v < wi.vcount; v += kBlock) {
-      uint32_t gv = wi.vmin + v;
-      float4 c = vertex_clip(M, PV, p.pos_x[gv], p.pos_y[gv], p.pos_z[gv]);
-      Snap s = snap_vertex(c, hw, hh, gx, gy);
-      s_x[v] = s.X;
-      s_y[v] = s.Y;
-      s_f[v] = s.flags;
-    }
-  }
-  __syncthreads();
-
-  const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
-  bool active = false;
-  PixRect pr = {0, 0, -1, -1};
-  uint32_t prim = 0;
-  if (threadIdx.x < wi.tri_count) {
-    const uint32_t t = wi.tri_start + threadIdx.x;
-    prim = d.prim_base + t;
-    const uint32_t* ip = p.indices + d.index_base + 3 * t;
-    uint32_t vi[3] = {ip[0], ip[1], ip[2]};
-    int32_t X[3], Y[3];
-    uint32_t f[3];
-    if (cached) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        uint32_t l = vi[k] - wi.vmin;
-        X[k] = s_x[l];
-        Y[k] = s_y[l];
-        f[k] = s_f[l];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float4 c = vertex_clip(M, PV, p.pos_x[vi[k]], p.pos_y[vi[k]], p.pos_z[vi[k]]);
-        Snap s = snap_vertex(c, hw, hh, gx, gy);
-        X[k] = s.X;
-        Y[k] = s.Y;
-        f[k] = s.flags;
-      }
-    }
-    if ((f[0] & f[1] & f[2] & 63u) == 0) {
-      if (((f[0] | f[1] | f[2]) & kNeedClip) == 0) {
-        if (tri_area(X, Y) > 0) {
-          pr = tri_pixels(X, Y, tile);
-          active = pr.x0 <= pr.x1 && pr.y0 <= pr.y1;
-        }
-      } else {
-        // rare path: clip against near/far/guard planes, bin the union of
-        // the front-facing fan triangles
-        float4 poly[kMaxPoly];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-          poly[k] = vertex_clip(M, PV, p.pos_x[vi[k]], p.pos_y[vi[k]], p.pos_z[vi[k]]);
-        int n = clip_poly(poly, 3, gx, gy);
-        bool ok = n >= 3;
-        int32_t PX[kMaxPoly], PY[kMaxPoly];
-        float PZ[kMaxPoly];
-        for (int k = 0; k < n && ok; ++k) {
-          if (!(poly[k].w > 0.0f)) ok = false;
-          else project(poly[k], hw, hh, PX[k], PY[k], PZ[k]);
-        }
-        if (ok) {
-          PixRect u = {1 << 30, 1 << 30, -(1 << 30), -(1 << 30)};
-          for (int k = 1; k + 1 < n; ++k) {
-            int32_t tx[3] = {PX[0], PX[k], PX[k + 1]};
-            int32_t ty[3] = {PY[0], PY[k], PY[k + 1]};
-            if (tri_area(tx, ty) <= 0) continue;
-            PixRect q = tri_pixels(tx, ty, tile);
-            if (q.x0 > q.x1 || q.y0 > q.y1) continue;
-            u.x0 = min(u.x0, q.x0); u.y0 = min(u.y0, q.y0);
-            u.x1 = max(u.x1, q.x1); u.y1 = max(u.y1, q.y1);
-          }
-          pr = u;
-          active = u.x0 <= u.x1 && u.y0 <= u.y1;
-        }
-      }
-    }
-  }
-
-  // bin append, aggregated per wave: lanes targeting the leader's bin take
-  // consecutive slots from one atomicAdd (slot = base + prefix popcount)
-  const int sh = p.bin_shift;
-  int bx0 = pr.x0 >> sh, bx1 = pr.x1 >> sh, by0 = pr.y0 >> sh, by1 = pr.y1 >> sh;
-  int cbx = bx0, cby = by0;
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t written = 0;
-  while (true) {
-    unsigned long long act = __ballot(active);
-    if (act == 0ull) break;
-    int leader = __ffsll((long long)act) - 1;
-    int mybin = cby * sc.bins_x + cbx;
-    int lbin = __shfl(mybin, leader);
-    bool match = active && (mybin == lbin);
-    unsigned long long mm = __ballot(match);
-    uint32_t cnt = (uint32_t)__popcll(mm);
-    uint32_t base = 0;
-    const uint32_t gbin = sc.bin_base + (uint32_t)lbin;
-    if (lane == leader) base = atomicAdd(&p.bin_count[gbin], cnt);
-    base = __shfl(base, leader);
-    if (match) {
-      uint32_t slot = base + (uint32_t)__popcll(mm & lt_mask);
-      if (slot < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)lbin * sc.bin_cap + slot] = prim;
-      ++written;
-      if (++cbx > bx1) {
-        cbx = bx0;
-        if (++cby > by1) active = false;
-      }
-    }
-  }
-  // count bin entries per wave (frame statistics)
-  for (int o = 32; o > 0; o >>= 1) written += __shfl_xor(written, o);
-  if (lane == 0 && written) atomicAdd(&p.stats[0], (unsigned long long)written);
-}
-
-// --------------------------------------------------------------------------
-// k_raster: per (scene, bin) raster into LDS + resolve
-// --------------------------------------------------------------------------
-
-struct BigTri {
-  int32_t X[3], Y[3];
-  float z0, dzdx, dzdy;
-  uint32_t prim;
-  PixRect r;
-};
-
 struct TriSetup {
   long long dx[3], dy[3];
   long long bias[3];
   float z0, dzdx, dzdy;
 };
 
-__device__ __forceinline__ void setup_edges(const int32_t* X, const int32_t* Y, const float* Z,
-                                            TriSetup& s) {
+__device__ __forceinline__ void setup_edges_only(const int32_t* X, const int32_t* Y, TriSetup& s) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     int j = (i + 1) % 3;
@@ -402,6 +260,11 @@ __device__ __forceinline__ void setup_edges(const int32_t* X, const int32_t* Y, 
     s.dy[i] = (long long)Y[j] - Y[i];
     s.bias[i] = (s.dy[i] < 0 || (s.dy[i] == 0 && s.dx[i] > 0)) ? 0 : -1;
   }
+}
+
+// depth plane relative to vertex 0 (same op order as the oracle)
+__device__ __forceinline__ void depth_plane(const int32_t* X, const int32_t* Y, const float* Z,
+                                            float& z0, float& dzdx_out, float& dzdy_out) {
   float fx1 = (float)(X[1] - X[0]), fy1 = (float)(Y[1] - Y[0]);
   float fx2 = (float)(X[2] - X[0]), fy2 = (float)(Y[2] - Y[0]);
   float fa = fx1 * fy2;
@@ -411,11 +274,17 @@ __device__ __forceinline__ void setup_edges(const int32_t* X, const int32_t* Y, 
   float pp, qq;
   pp = dz1 * fy2; qq = dz2 * fy1;
   float dzdx = pp - qq;
-  s.dzdx = dzdx / fa;
+  dzdx_out = dzdx / fa;
   pp = dz2 * fx1; qq = dz1 * fx2;
   float dzdy = pp - qq;
-  s.dzdy = dzdy / fa;
-  s.z0 = Z[0];
+  dzdy_out = dzdy / fa;
+  z0 = Z[0];
+}
+
+__device__ __forceinline__ void setup_edges(const int32_t* X, const int32_t* Y, const float* Z,
+                                            TriSetup& s) {
+  setup_edges_only(X, Y, s);
+  depth_plane(X, Y, Z, s.z0, s.dzdx, s.dzdy);
 }
 
 __device__ __forceinline__ bool covers(const TriSetup& s, const int32_t* X, const int32_t* Y,
@@ -442,6 +311,232 @@ __device__ __forceinline__ float depth_at(float z0, float dzdx, float dzdy, int3
   return z;
 }
 
+__device__ __forceinline__ void vertex_uv(const FrameParams& p, uint32_t flags, uint32_t v,
+                                          const float* tm, float& s, float& t) {
+  float u, vv;
+  if (flags & kDrawUvHalf) {
+    uint32_t h2 = p.uv16[v];
+    u = __half2float(__ushort_as_half((unsigned short)(h2 & 0xffff)));
+    vv = __half2float(__ushort_as_half((unsigned short)(h2 >> 16)));
+  } else {
+    float2 t2 = p.uv32[v];
+    u = t2.x;
+    vv = t2.y;
+  }
+  // texture matrix in the vertex stage: ((c0*u + c1*v) + c2)
+  float a = tm[0] * u; float a1 = tm[3] * vv; a = a + a1; a = a + tm[6];
+  float b = tm[1] * u; float b1 = tm[4] * vv; b = b + b1; b = b + tm[7];
+  s = a;
+  t = b;
+}
+
+// wave-aggregated append of one PrimRef per flagged lane (ballot + prefix
+// popcount, one atomic per wave)
+__device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counter, bool flag,
+                                           PrimRef ref, PrimRef* list, uint32_t cap, int lane,
+                                           unsigned long long lt_mask) {
+  const unsigned long long m = __ballot(flag);
+  if (m == 0ull) return;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&p.counters[counter], (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (flag) {
+    const uint32_t slot = base + (uint32_t)__popcll(m & lt_mask);
+    if (slot < cap) list[slot] = ref;
+  }
+}
+
+// --------------------------------------------------------------------------
+// k_setup_bin: persistent workgroups draining a queue of work items (chunks
+// of <= 1024 consecutive triangles of one draw)
+//
+//  1. vertex stage for the chunk's vertex range -> LDS: snapped window
+//     coordinates, window depth, outcodes, clip-space x/y/w and the
+//     texture coordinates after the draw's texture matrix
+//  2. triangle setup, 4 triangles per lane; a survivor writes its record
+//     (raster + resolve parts) straight from LDS and counts itself into
+//     LDS per-bin counters (LDS atomics give its local slot)
+//  3. one global atomicAdd per touched bin reserves the item's range in
+//     the bin; 4. entries are scattered to base + local slot
+// --------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
+                                                      const WorkItem* __restrict__ items,
+                                                      uint32_t n_items) {
+  __shared__ int32_t s_x[kVertexCache];
+  __shared__ int32_t s_y[kVertexCache];
+  __shared__ float s_z[kVertexCache];
+  __shared__ uint32_t s_f[kVertexCache];
+  __shared__ float s_cx[kVertexCache];
+  __shared__ float s_cy[kVertexCache];
+  __shared__ float s_cw[kVertexCache];
+  __shared__ float s_s[kVertexCache];
+  __shared__ float s_t[kVertexCache];
+  __shared__ uint32_t s_bcount[kMaxSceneBins];
+  __shared__ uint32_t s_bbase[kMaxSceneBins];
+  __shared__ uint32_t s_pair[kPairCap];  // bin << 21 | local slot << 10 | triangle in chunk
+  __shared__ uint32_t s_npairs;
+  __shared__ uint32_t s_item;
+
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t written = 0;
+  uint32_t grab = 0, grab_end = 0;  // items claimed from the queue (uniform)
+
+  while (true) {
+    if (grab == grab_end) {
+      if (threadIdx.x == 0) s_item = atomicAdd(&p.counters[kCtrSetupQueue * kCtrStride], p.queue_grain);
+      __syncthreads();
+      grab = s_item;
+      grab_end = min(grab + p.queue_grain, n_items);
+      __syncthreads();  // s_item may be rewritten only after everyone read it
+      if (grab >= n_items) break;
+    }
+    const uint32_t item = grab++;
+    const WorkItem wi = items[item];
+    const SceneDesc sc = p.scenes[wi.scene];
+    const DrawDesc d = p.draws[wi.draw];
+    const MaterialDesc& mat = p.materials[d.material];
+    float M[16], PV[16], tm[9];
+    load_mat(p.abs_nodes + (size_t)wi.node * 16, M);
+    load_mat(p.cameras + (size_t)wi.scene * 16, PV);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) tm[q] = mat.texmat[q];
+    const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
+    const float gx = kGuardPx / hw, gy = kGuardPx / hh;
+    const bool cached = wi.vcount <= (uint32_t)kVertexCache;
+    // scenes with more bins than the LDS counters hold append every pair
+    // straight to the global bins
+    const int nbins = sc.bins_x * sc.bins_y;
+    const int nlb = nbins <= kMaxSceneBins ? nbins : 0;
+
+    for (int b = threadIdx.x; b < nlb; b += kBlock) s_bcount[b] = 0u;
+    if (threadIdx.x == 0) s_npairs = nlb ? 0u : (uint32_t)kPairCap;
+    if (cached) {
+      for (uint32_t v = threadIdx.x; v < wi.vcount; v += kBlock) {
+        const uint32_t gv = wi.vmin + v;
+        const float4 c = vertex_clip(M, PV, p.pos_x[gv], p.pos_y[gv], p.pos_z[gv]);
+        const Snap s = snap_vertex(c, hw, hh, gx, gy);
+        s_x[v] = s.X;
+        s_y[v] = s.Y;
+        s_z[v] = s.z;
+        s_f[v] = s.flags;
+        s_cx[v] = c.x;
+        s_cy[v] = c.y;
+        s_cw[v] = c.w;
+        float ss, tt;
+        vertex_uv(p, d.flags, gv, tm, ss, tt);
+        s_s[v] = ss;
+        s_t[v] = tt;
+      }
+    }
+    __syncthreads();
+
+    const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
+    const int sh = p.bin_shift;
+#pragma unroll 1
+    for (int k = 0; k < kChunkTris / kBlock; ++k) {
+      const uint32_t j = (uint32_t)k * kBlock + threadIdx.x;
+      bool clip_me = false;
+      const uint32_t prim = wi.prim + j;
+      if (j < wi.tri_count && !cached) {
+        clip_me = true;  // vertex range beyond the LDS cache: k_setup_clip takes it whole
+      } else if (j < wi.tri_count) {
+        const uint32_t* ip = p.indices + wi.first_index + 3 * j;
+        const uint32_t vi[3] = {ip[0], ip[1], ip[2]};
+        int32_t X[3], Y[3];
+        float Z[3];
+        uint32_t f[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const uint32_t l = vi[q] - wi.vmin;
+          X[q] = s_x[l];
+          Y[q] = s_y[l];
+          Z[q] = s_z[l];
+          f[q] = s_f[l];
+        }
+        PixRect pr = {0, 0, -1, -1};
+        bool active = false;
+        if ((f[0] & f[1] & f[2] & 63u) == 0) {
+          if (((f[0] | f[1] | f[2]) & kNeedClip) == 0) {
+            if (tri_area(X, Y) > 0) {
+              pr = tri_pixels(X, Y, tile);
+              active = pr.x0 <= pr.x1 && pr.y0 <= pr.y1;
+            }
+          } else {
+            clip_me = true;  // rare: k_setup_clip handles the fan
+          }
+        }
+        if (active) {
+          PrimRec rec;
+          rec.r.X0 = X[0]; rec.r.Y0 = Y[0]; rec.r.X1 = X[1]; rec.r.Y1 = Y[1];
+          rec.r.X2 = X[2]; rec.r.Y2 = Y[2];
+          rec.r.prim = prim;
+          rec.r.draw = wi.draw;
+          depth_plane(X, Y, Z, rec.r.z0, rec.r.dzdx, rec.r.dzdy);
+          rec.r.pad2 = 0.0f;
+          const uint32_t l0 = vi[0] - wi.vmin, l1 = vi[1] - wi.vmin, l2 = vi[2] - wi.vmin;
+          rec.cx0 = s_cx[l0]; rec.cy0 = s_cy[l0]; rec.cw0 = s_cw[l0];
+          rec.cx1 = s_cx[l1]; rec.cy1 = s_cy[l1]; rec.cw1 = s_cw[l1];
+          rec.cx2 = s_cx[l2]; rec.cy2 = s_cy[l2]; rec.cw2 = s_cw[l2];
+          rec.s0 = s_s[l0]; rec.t0 = s_t[l0];
+          rec.s1 = s_s[l1]; rec.t1 = s_t[l1];
+          rec.s2 = s_s[l2]; rec.t2 = s_t[l2];
+          rec.pad = 0.0f;
+          p.recs[(size_t)sc.rec_base + prim] = rec;
+          // count into the LDS per-bin counters; pairs beyond the LDS list
+          // go straight to the global bin
+          for (int by = pr.y0 >> sh; by <= (pr.y1 >> sh); ++by)
+            for (int bx = pr.x0 >> sh; bx <= (pr.x1 >> sh); ++bx) {
+              const uint32_t b = (uint32_t)(by * sc.bins_x + bx);
+              const uint32_t pi = s_npairs < (uint32_t)kPairCap ? atomicAdd(&s_npairs, 1u) : kPairCap;
+              if (pi < (uint32_t)kPairCap) {
+                const uint32_t slot = atomicAdd(&s_bcount[b], 1u);
+                s_pair[pi] = (b << 21) | (slot << 10) | j;
+              } else {
+                const uint32_t slot = atomicAdd(&p.bin_count[sc.bin_base + b], 1u);
+                if (slot < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b * sc.bin_cap + slot] = prim;
+              }
+              ++written;
+            }
+        }
+      }
+      append_ref(p, kCtrClipped * kCtrStride, clip_me, PrimRef{wi.draw, prim}, p.clipped,
+                 p.clipped_cap, lane, lt_mask);
+    }
+    __syncthreads();
+    // one global reservation per touched bin
+    for (int b = threadIdx.x; b < nlb; b += kBlock) {
+      const uint32_t n = s_bcount[b];
+      if (n) s_bbase[b] = atomicAdd(&p.bin_count[sc.bin_base + b], n);
+    }
+    __syncthreads();
+    const uint32_t np = min(s_npairs, (uint32_t)kPairCap);
+    for (uint32_t i = threadIdx.x; i < np; i += kBlock) {
+      const uint32_t e = s_pair[i];
+      const uint32_t b = e >> 21;
+      const uint32_t slot = s_bbase[b] + ((e >> 10) & 0x7ffu);
+      if (slot < sc.bin_cap)
+        p.bin_entries[sc.entry_base + (size_t)b * sc.bin_cap + slot] = wi.prim + (e & 0x3ffu);
+    }
+    __syncthreads();  // LDS is reused by the next item
+  }
+  for (int o = 32; o > 0; o >>= 1) written += __shfl_xor(written, o);
+  if (lane == 0 && written) atomicAdd(&p.stats[0], (unsigned long long)written);
+}
+
+// --------------------------------------------------------------------------
+// Rasterisation helpers: LDS visibility tile with 64-bit atomicMin keys
+// --------------------------------------------------------------------------
+
+struct BigTri {
+  int32_t X[3], Y[3];
+  float z0, dzdx, dzdy;
+  uint32_t prim;
+  PixRect r;
+};
+
 struct RasterCtx {
   unsigned long long* vis;
   BigTri* big;
@@ -450,59 +545,62 @@ struct RasterCtx {
   int B;
 };
 
-__device__ void raster_tri(const RasterCtx& rc, const int32_t* X, const int32_t* Y, const float* Z,
-                           uint32_t prim) {
-  if (tri_area(X, Y) <= 0) return;
+// rasterise one front-facing triangle with a known depth plane
+__device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32_t* Y, float z0,
+                              float dzdx, float dzdy, uint32_t prim) {
   PixRect r = tri_pixels(X, Y, rc.bin);
   if (r.x0 > r.x1 || r.y0 > r.y1) return;
-  TriSetup s;
-  setup_edges(X, Y, Z, s);
   const int npx = (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
   if (npx > kSmallRaster) {
-    int slot = atomicAdd(rc.nbig, 1);
+    const int slot = atomicAdd(rc.nbig, 1);
     if (slot < kBigQueue) {
       BigTri& b = rc.big[slot];
 #pragma unroll
       for (int k = 0; k < 3; ++k) { b.X[k] = X[k]; b.Y[k] = Y[k]; }
-      b.z0 = s.z0; b.dzdx = s.dzdx; b.dzdy = s.dzdy;
+      b.z0 = z0; b.dzdx = dzdx; b.dzdy = dzdy;
       b.prim = prim;
       b.r = r;
       return;
     }
   }
+  TriSetup s;
+  setup_edges_only(X, Y, s);
   for (int py = r.y0; py <= r.y1; ++py) {
-    long long Py = (long long)py * 256 + 128;
+    const long long Py = (long long)py * 256 + 128;
     for (int px = r.x0; px <= r.x1; ++px) {
-      long long Px = (long long)px * 256 + 128;
+      const long long Px = (long long)px * 256 + 128;
       if (!covers(s, X, Y, Px, Py)) continue;
-      float z = depth_at(s.z0, s.dzdx, s.dzdy, X[0], Y[0], Px, Py);
+      const float z = depth_at(z0, dzdx, dzdy, X[0], Y[0], Px, Py);
       if (z < 1.0f) {
-        unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | prim;
+        const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | prim;
         atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
       }
     }
   }
 }
 
+__device__ __forceinline__ void raster_tri(const RasterCtx& rc, const int32_t* X, const int32_t* Y,
+                                           const float* Z, uint32_t prim) {
+  if (tri_area(X, Y) <= 0) return;
+  PixRect r = tri_pixels(X, Y, rc.bin);
+  if (r.x0 > r.x1 || r.y0 > r.y1) return;
+  float z0, dzdx, dzdy;
+  depth_plane(X, Y, Z, z0, dzdx, dzdy);
+  raster_planed(rc, X, Y, z0, dzdx, dzdy, prim);
+}
+
 __device__ void raster_big(const RasterCtx& rc, const BigTri& b) {
   TriSetup s;
-  // edges from the stored vertices; depth plane stored
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    int j = (i + 1) % 3;
-    s.dx[i] = (long long)b.X[j] - b.X[i];
-    s.dy[i] = (long long)b.Y[j] - b.Y[i];
-    s.bias[i] = (s.dy[i] < 0 || (s.dy[i] == 0 && s.dx[i] > 0)) ? 0 : -1;
-  }
+  setup_edges_only(b.X, b.Y, s);
   const int w = b.r.x1 - b.r.x0 + 1;
   const int n = w * (b.r.y1 - b.r.y0 + 1);
   for (int k = threadIdx.x; k < n; k += kBlock) {
-    int px = b.r.x0 + k % w, py = b.r.y0 + k / w;
-    long long Px = (long long)px * 256 + 128, Py = (long long)py * 256 + 128;
+    const int px = b.r.x0 + k % w, py = b.r.y0 + k / w;
+    const long long Px = (long long)px * 256 + 128, Py = (long long)py * 256 + 128;
     if (!covers(s, b.X, b.Y, Px, Py)) continue;
-    float z = depth_at(b.z0, b.dzdx, b.dzdy, b.X[0], b.Y[0], Px, Py);
+    const float z = depth_at(b.z0, b.dzdx, b.dzdy, b.X[0], b.Y[0], Px, Py);
     if (z < 1.0f) {
-      unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | b.prim;
+      const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | b.prim;
       atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
     }
   }
@@ -514,14 +612,16 @@ __device__ __forceinline__ uint32_t find_draw(const FrameParams& p, const SceneD
   uint32_t lo = 0, hi = sc.draw_count;
   const bool lds = sc.draw_count <= (uint32_t)kDrawCache;
   while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    uint32_t b = lds ? s_base[mid] : p.draws[sc.draw_begin + mid].prim_base;
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t b = lds ? s_base[mid] : p.draws[sc.draw_begin + mid].prim_base;
     if (b <= prim) lo = mid + 1;
     else hi = mid;
   }
   return sc.draw_begin + lo - 1;
 }
 
+// full re-derivation of a primitive from the vertex streams (clipped
+// primitives and the overflow fallback)
 __device__ void raster_prim(const FrameParams& p, const SceneDesc& sc, const float* PV,
                             const RasterCtx& rc, const uint32_t* s_base, uint32_t prim) {
   const uint32_t di = find_draw(p, sc, s_base, prim);
@@ -533,7 +633,7 @@ __device__ void raster_prim(const FrameParams& p, const SceneDesc& sc, const flo
   float4 c[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    uint32_t v = ip[k];
+    const uint32_t v = ip[k];
     c[k] = vertex_clip(M, PV, p.pos_x[v], p.pos_y[v], p.pos_z[v]);
   }
   if (outcode(c[0]) & outcode(c[1]) & outcode(c[2])) return;
@@ -558,7 +658,7 @@ __device__ void raster_prim(const FrameParams& p, const SceneDesc& sc, const flo
   }
   float4 poly[kMaxPoly];
   poly[0] = c[0]; poly[1] = c[1]; poly[2] = c[2];
-  int n = clip_poly(poly, 3, gx, gy);
+  const int n = clip_poly(poly, 3, gx, gy);
   if (n < 3) return;
   int32_t PX[kMaxPoly], PY[kMaxPoly];
   float PZ[kMaxPoly];
@@ -573,6 +673,7 @@ __device__ void raster_prim(const FrameParams& p, const SceneDesc& sc, const flo
     raster_tri(rc, tx, ty, tz, prim);
   }
 }
+
 
 // ---- texture sampling (GL 4.6 section 8.14) -------------------------------
 
@@ -785,32 +886,17 @@ __device__ __forceinline__ uint32_t to_unorm8(float c) {
   return (uint32_t)(int)v;
 }
 
-__device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const float* PV,
-                                const DrawDesc& d, uint32_t tri, int px, int py) {
-  float M[16];
-  load_mat(p.abs_nodes + (size_t)d.node * 16, M);
+// colour of the visible primitive at pixel (px, py), from its resolve record
+__device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const PrimRec& rs,
+                               const DrawDesc& d, uint32_t prim, int px, int py) {
   const MaterialDesc mat = p.materials[d.material];
-  const uint32_t* ip = p.indices + d.index_base + 3 * tri;
   TriAttr A;
-  uint32_t vi[3] = {ip[0], ip[1], ip[2]};
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    uint32_t v = vi[k];
-    float4 c = vertex_clip(M, PV, p.pos_x[v], p.pos_y[v], p.pos_z[v]);
-    A.cx[k] = c.x; A.cy[k] = c.y; A.cw[k] = c.w;
-    float u, vv;
-    if (d.flags & kDrawUvHalf) {
-      uint32_t h2 = p.uv16[v];
-      u = __half2float(__ushort_as_half((unsigned short)(h2 & 0xffff)));
-      vv = __half2float(__ushort_as_half((unsigned short)(h2 >> 16)));
-    } else {
-      float2 t2 = p.uv32[v];
-      u = t2.x; vv = t2.y;
-    }
-    float s = mat.texmat[0] * u; float s1 = mat.texmat[3] * vv; s = s + s1; s = s + mat.texmat[6];
-    float tt = mat.texmat[1] * u; float t1 = mat.texmat[4] * vv; tt = tt + t1; tt = tt + mat.texmat[7];
-    A.s[k] = s; A.t[k] = tt;
-  }
+  A.cx[0] = rs.cx0; A.cy[0] = rs.cy0; A.cw[0] = rs.cw0;
+  A.cx[1] = rs.cx1; A.cy[1] = rs.cy1; A.cw[1] = rs.cw1;
+  A.cx[2] = rs.cx2; A.cy[2] = rs.cy2; A.cw[2] = rs.cw2;
+  A.s[0] = rs.s0; A.t[0] = rs.t0;
+  A.s[1] = rs.s1; A.t[1] = rs.t1;
+  A.s[2] = rs.s2; A.t[2] = rs.t2;
   bary_rows(A);
   const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
   const float ihw = 1.0f / hw, ihh = 1.0f / hh;
@@ -819,27 +905,27 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const
   float c[4] = {mat.ambient[0], mat.ambient[1], mat.ambient[2], mat.ambient[3]};
   if (!p.no_textures && mat.texture > 0) {
     const TextureDesc& td = p.textures[mat.texture];
-    float s = interp3(b, A.s[0], A.s[1], A.s[2]);
-    float t = interp3(b, A.t[0], A.t[1], A.t[2]);
-    int qx = px & ~1, qy = py & ~1;
+    const float s = interp3(b, A.s[0], A.s[1], A.s[2]);
+    const float t = interp3(b, A.t[0], A.t[1], A.t[2]);
+    const int qx = px & ~1, qy = py & ~1;
     float b00[3], b10[3], b01[3];
     bary_at(A, hw, hh, ihw, ihh, (float)qx + 0.5f, (float)qy + 0.5f, b00);
     bary_at(A, hw, hh, ihw, ihh, (float)qx + 1.5f, (float)qy + 0.5f, b10);
     bary_at(A, hw, hh, ihw, ihh, (float)qx + 0.5f, (float)qy + 1.5f, b01);
-    float s00 = interp3(b00, A.s[0], A.s[1], A.s[2]);
-    float t00 = interp3(b00, A.t[0], A.t[1], A.t[2]);
-    float s10 = interp3(b10, A.s[0], A.s[1], A.s[2]);
-    float t10 = interp3(b10, A.t[0], A.t[1], A.t[2]);
-    float s01 = interp3(b01, A.s[0], A.s[1], A.s[2]);
-    float t01 = interp3(b01, A.t[0], A.t[1], A.t[2]);
-    float fw = (float)td.width, fh = (float)td.height;
+    const float s00 = interp3(b00, A.s[0], A.s[1], A.s[2]);
+    const float t00 = interp3(b00, A.t[0], A.t[1], A.t[2]);
+    const float s10 = interp3(b10, A.s[0], A.s[1], A.s[2]);
+    const float t10 = interp3(b10, A.t[0], A.t[1], A.t[2]);
+    const float s01 = interp3(b01, A.s[0], A.s[1], A.s[2]);
+    const float t01 = interp3(b01, A.t[0], A.t[1], A.t[2]);
+    const float fw = (float)td.width, fh = (float)td.height;
     float dux = s10 - s00; dux = dux * fw;
     float dvx = t10 - t00; dvx = dvx * fh;
     float duy = s01 - s00; duy = duy * fw;
     float dvy = t01 - t00; dvy = dvy * fh;
     float rx = dux * dux; float rx2 = dvx * dvx; rx = rx + rx2;
     float ry = duy * duy; float ry2 = dvy * dvy; ry = ry + ry2;
-    float r2 = rx > ry ? rx : ry;
+    const float r2 = rx > ry ? rx : ry;
     float lambda = gbr_log2(r2);
     lambda = lambda * 0.5f;
     float tc[4];
@@ -849,23 +935,25 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const
   }
   const uint32_t cf = (d.flags >> kDrawColorShift) & 3u;
   if (cf != kColorNone) {
+    const uint32_t* ip = p.indices + d.index_base + 3 * (prim - d.prim_base);
     float col[3][4];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+      const uint32_t v = ip[k];
       if (cf == kColorU8) {
-        uint32_t u8 = p.col8[vi[k]];
+        const uint32_t u8 = p.col8[v];
         col[k][0] = p.unorm_lut[u8 & 0xff];
         col[k][1] = p.unorm_lut[(u8 >> 8) & 0xff];
         col[k][2] = p.unorm_lut[(u8 >> 16) & 0xff];
         col[k][3] = p.unorm_lut[u8 >> 24];
       } else {
-        float4 f = p.col32[vi[k]];
+        const float4 f = p.col32[v];
         col[k][0] = f.x; col[k][1] = f.y; col[k][2] = f.z; col[k][3] = f.w;
       }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      float v = interp3(b, col[0][k], col[1][k], col[2][k]);
+      const float v = interp3(b, col[0][k], col[1][k], col[2][k]);
       c[k] = c[k] * v;
     }
   }
@@ -873,45 +961,169 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const
          (to_unorm8(c[3]) << 24);
 }
 
+
+// --------------------------------------------------------------------------
+// k_setup_clip: primitives crossing the near/far plane or the guard band.
+// Sutherland-Hodgman fan, one raster record per front-facing fan triangle,
+// each binned on its own (entry = kSubTri | record).
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ void bin_rect_atomic(const FrameParams& p, const SceneDesc& sc,
+                                                const PixRect& r, uint32_t entry) {
+  const int sh = p.bin_shift;
+  for (int by = r.y0 >> sh; by <= (r.y1 >> sh); ++by)
+    for (int bx = r.x0 >> sh; bx <= (r.x1 >> sh); ++bx) {
+      const uint32_t b = (uint32_t)(by * sc.bins_x + bx);
+      const uint32_t slot = atomicAdd(&p.bin_count[sc.bin_base + b], 1u);
+      if (slot < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b * sc.bin_cap + slot] = entry;
+    }
+}
+
+// forces the exact rescan for every bin the rectangle touches
+__device__ __forceinline__ void overflow_rect(const FrameParams& p, const SceneDesc& sc,
+                                              const PixRect& r) {
+  const int sh = p.bin_shift;
+  for (int by = r.y0 >> sh; by <= (r.y1 >> sh); ++by)
+    for (int bx = r.x0 >> sh; bx <= (r.x1 >> sh); ++bx)
+      atomicAdd(&p.bin_count[sc.bin_base + (uint32_t)(by * sc.bins_x + bx)], sc.bin_cap + 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p) {
+  const uint32_t n = min(p.counters[kCtrClipped * kCtrStride], p.clipped_cap);
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const PrimRef ref = p.clipped[i];
+    const DrawDesc d = p.draws[ref.draw];
+    const SceneDesc sc = p.scenes[d.scene];
+    float M[16], PV[16];
+    load_mat(p.abs_nodes + (size_t)d.node * 16, M);
+    load_mat(p.cameras + (size_t)d.scene * 16, PV);
+    const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
+    const float gx = kGuardPx / hw, gy = kGuardPx / hh;
+    const uint32_t* ip = p.indices + d.index_base + 3 * (ref.prim - d.prim_base);
+    float4 c[3], poly[kMaxPoly];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const uint32_t v = ip[q];
+      c[q] = vertex_clip(M, PV, p.pos_x[v], p.pos_y[v], p.pos_z[v]);
+      poly[q] = c[q];
+    }
+    const int nv = clip_poly(poly, 3, gx, gy);
+    if (nv < 3) continue;
+    int32_t PX[kMaxPoly], PY[kMaxPoly];
+    float PZ[kMaxPoly];
+    bool ok = true;
+    for (int q = 0; q < nv && ok; ++q) {
+      if (!(poly[q].w > 0.0f)) ok = false;
+      else project(poly[q], hw, hh, PX[q], PY[q], PZ[q]);
+    }
+    if (!ok) continue;
+    const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
+    bool any = false;
+    for (int q = 1; q + 1 < nv; ++q) {
+      const int32_t tx[3] = {PX[0], PX[q], PX[q + 1]};
+      const int32_t ty[3] = {PY[0], PY[q], PY[q + 1]};
+      const float tz[3] = {PZ[0], PZ[q], PZ[q + 1]};
+      if (tri_area(tx, ty) <= 0) continue;
+      const PixRect r = tri_pixels(tx, ty, tile);
+      if (r.x0 > r.x1 || r.y0 > r.y1) continue;
+      any = true;
+      const uint32_t slot = atomicAdd(&p.counters[kCtrSubTris * kCtrStride], 1u);
+      if (slot < p.sub_cap) {
+        RasterRec rr;
+        rr.X0 = tx[0]; rr.Y0 = ty[0]; rr.X1 = tx[1]; rr.Y1 = ty[1]; rr.X2 = tx[2]; rr.Y2 = ty[2];
+        rr.prim = ref.prim;
+        rr.draw = ref.draw;
+        depth_plane(tx, ty, tz, rr.z0, rr.dzdx, rr.dzdy);
+        rr.pad2 = 0.0f;
+        p.sub_recs[slot] = rr;
+        bin_rect_atomic(p, sc, r, kSubTri | slot);
+      } else {
+        overflow_rect(p, sc, r);
+      }
+    }
+    if (any) {
+      // resolve part of the primitive's record (the raster part stays unused:
+      // its fan triangles carry their own)
+      const MaterialDesc& mat = p.materials[d.material];
+      float tm[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) tm[q] = mat.texmat[q];
+      PrimRec rec = {};
+      rec.r.prim = ref.prim;
+      rec.r.draw = ref.draw;
+      rec.cx0 = c[0].x; rec.cy0 = c[0].y; rec.cw0 = c[0].w;
+      rec.cx1 = c[1].x; rec.cy1 = c[1].y; rec.cw1 = c[1].w;
+      rec.cx2 = c[2].x; rec.cy2 = c[2].y; rec.cw2 = c[2].w;
+      vertex_uv(p, d.flags, ip[0], tm, rec.s0, rec.t0);
+      vertex_uv(p, d.flags, ip[1], tm, rec.s1, rec.t1);
+      vertex_uv(p, d.flags, ip[2], tm, rec.s2, rec.t2);
+      p.recs[(size_t)sc.rec_base + ref.prim] = rec;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// k_raster: per (scene, bin), binned records -> LDS visibility -> vis keys
+// --------------------------------------------------------------------------
+
+struct BinInfo {
+  SceneDesc sc;
+  PixRect bin;
+  uint32_t gbin;
+};
+
+__device__ __forceinline__ BinInfo bin_info(const FrameParams& p, const RasterItem& it) {
+  BinInfo b;
+  b.sc = p.scenes[it.scene];
+  const int B = p.bin_size;
+  const int bx = (int)(it.bin % (uint32_t)b.sc.bins_x), by = (int)(it.bin / (uint32_t)b.sc.bins_x);
+  b.bin.x0 = bx * B;
+  b.bin.y0 = by * B;
+  b.bin.x1 = min(bx * B + B, b.sc.width) - 1;
+  b.bin.y1 = min(by * B + B, b.sc.height) - 1;
+  b.gbin = b.sc.bin_base + it.bin;
+  return b;
+}
+
+__device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b,
+                                          const unsigned long long* s_vis, int B) {
+  const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
+  for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
+    const int lx = l % bw, ly = l / bw;
+    p.vis[(size_t)b.sc.plane_offset + (size_t)(b.bin.y0 + ly) * b.sc.width + b.bin.x0 + lx] =
+        s_vis[ly * B + lx];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
                                                    const RasterItem* __restrict__ items) {
   __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
   __shared__ BigTri s_big[kBigQueue];
-  __shared__ uint32_t s_base[kDrawCache];
   __shared__ int s_nbig;
 
-  const RasterItem it = items[blockIdx.x];
-  const SceneDesc sc = p.scenes[it.scene];
+  const BinInfo b = bin_info(p, items[blockIdx.x]);
+  const uint32_t n = p.bin_count[b.gbin];
+  if (n > b.sc.bin_cap) return;  // overflowed: k_raster_full rescans this bin
   const int B = p.bin_size;
-  const int bx = (int)(it.bin % (uint32_t)sc.bins_x), by = (int)(it.bin / (uint32_t)sc.bins_x);
   RasterCtx rc;
   rc.vis = s_vis;
   rc.big = s_big;
   rc.nbig = &s_nbig;
   rc.B = B;
-  rc.bin.x0 = bx * B;
-  rc.bin.y0 = by * B;
-  rc.bin.x1 = min(bx * B + B, sc.width) - 1;
-  rc.bin.y1 = min(by * B + B, sc.height) - 1;
-
+  rc.bin = b.bin;
   for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
-  if (sc.draw_count <= (uint32_t)kDrawCache)
-    for (uint32_t i = threadIdx.x; i < sc.draw_count; i += kBlock)
-      s_base[i] = p.draws[sc.draw_begin + i].prim_base;
   if (threadIdx.x == 0) s_nbig = 0;
-  float PV[16];
-  load_mat(p.cameras + (size_t)it.scene * 16, PV);
   __syncthreads();
-
-  const uint32_t gbin = sc.bin_base + it.bin;
-  const uint32_t n = p.bin_count[gbin];
-  const bool overflow = n > sc.bin_cap;
-  const uint32_t total = overflow ? sc.prim_count : n;
-  for (uint32_t base = 0; base < total; base += kBlock) {
+  const uint32_t* entries = p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
+  for (uint32_t base = 0; base < n; base += kBlock) {
     const uint32_t k = base + threadIdx.x;
-    if (k < total) {
-      const uint32_t prim = overflow ? k : p.bin_entries[sc.entry_base + (size_t)it.bin * sc.bin_cap + k];
-      raster_prim(p, sc, PV, rc, s_base, prim);
+    if (k < n) {
+      const uint32_t e = entries[k];
+      const RasterRec rr = (e & kSubTri) ? p.sub_recs[e & ~kSubTri]
+                                         : p.recs[(size_t)b.sc.rec_base + e].r;
+      const int32_t X[3] = {rr.X0, rr.X1, rr.X2};
+      const int32_t Y[3] = {rr.Y0, rr.Y1, rr.Y2};
+      raster_planed(rc, X, Y, rr.z0, rr.dzdx, rr.dzdy, rr.prim);
     }
     __syncthreads();
     const int nb = min(s_nbig, kBigQueue);
@@ -920,15 +1132,70 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
     if (threadIdx.x == 0) s_nbig = 0;
     __syncthreads();
   }
+  store_vis(p, b, s_vis, B);
+  if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
+}
 
-  // resolve: every pixel of the bin, written once
+// exact fallback for bins whose list overflowed: every primitive of the scene
+__global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
+                                                        const RasterItem* __restrict__ items) {
+  __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
+  __shared__ BigTri s_big[kBigQueue];
+  __shared__ uint32_t s_base[kDrawCache];
+  __shared__ int s_nbig;
+
+  const RasterItem it = items[blockIdx.x];
+  const BinInfo b = bin_info(p, it);
+  if (p.bin_count[b.gbin] <= b.sc.bin_cap) return;
+  const int B = p.bin_size;
+  RasterCtx rc;
+  rc.vis = s_vis;
+  rc.big = s_big;
+  rc.nbig = &s_nbig;
+  rc.B = B;
+  rc.bin = b.bin;
+  for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
+  if (b.sc.draw_count <= (uint32_t)kDrawCache)
+    for (uint32_t i = threadIdx.x; i < b.sc.draw_count; i += kBlock)
+      s_base[i] = p.draws[b.sc.draw_begin + i].prim_base;
+  if (threadIdx.x == 0) s_nbig = 0;
+  float PV[16];
+  load_mat(p.cameras + (size_t)it.scene * 16, PV);
+  __syncthreads();
+  for (uint32_t base = 0; base < b.sc.prim_count; base += kBlock) {
+    const uint32_t k = base + threadIdx.x;
+    if (k < b.sc.prim_count) raster_prim(p, b.sc, PV, rc, s_base, k);
+    __syncthreads();
+    const int nb = min(s_nbig, kBigQueue);
+    for (int q = 0; q < nb; ++q) raster_big(rc, s_big[q]);
+    __syncthreads();
+    if (threadIdx.x == 0) s_nbig = 0;
+    __syncthreads();
+  }
+  store_vis(p, b, s_vis, B);
+  if (threadIdx.x == 0) {
+    atomicAdd(&p.stats[1], 1ull);
+    p.bin_count[b.gbin] = 0;
+  }
+}
+
+// --------------------------------------------------------------------------
+// k_resolve: per pixel, visibility key -> record -> colour, window depth,
+// linear depth, semantic id; every output written once
+// --------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
+                                                    const RasterItem* __restrict__ items) {
+  const RasterItem it = items[blockIdx.x];
+  const BinInfo b = bin_info(p, it);
+  const SceneDesc& sc = b.sc;
   const float un0 = p.unprojections[2 * it.scene], un1 = p.unprojections[2 * it.scene + 1];
   const float far_depth = un1 / (1.0f + un0);
-  const int bw = rc.bin.x1 - rc.bin.x0 + 1, bh = rc.bin.y1 - rc.bin.y0 + 1;
+  const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
   for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
-    const int lx = l % bw, ly = l / bw;
-    const int px = rc.bin.x0 + lx, py = rc.bin.y0 + ly;
-    const unsigned long long key = s_vis[ly * B + lx];
+    const int px = b.bin.x0 + l % bw, py = b.bin.y0 + l / bw;
+    const unsigned long long key =
+        p.vis[(size_t)sc.plane_offset + (size_t)py * sc.width + px];
     const int row = p.top_down ? (sc.height - 1 - py) : py;
     const size_t o = (size_t)sc.plane_offset + (size_t)row * sc.width + px;
     if (key == kClearKey) {
@@ -940,19 +1207,15 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
     }
     const uint32_t prim = (uint32_t)key;
     const float z = __uint_as_float((uint32_t)(key >> 32));
-    const uint32_t di = find_draw(p, sc, s_base, prim);
-    const DrawDesc d = p.draws[di];
-    if (p.out_color) p.out_color[o] = shade_pixel(p, sc, PV, d, prim - d.prim_base, px, py);
+    const PrimRec rs = p.recs[(size_t)sc.rec_base + prim];
+    const DrawDesc d = p.draws[rs.r.draw];
+    if (p.out_color) p.out_color[o] = shade_pixel(p, sc, rs, d, prim, px, py);
     if (p.out_depth) p.out_depth[o] = z;
     if (p.out_lindepth) {
-      float lin = un1 / (z + un0);
+      const float lin = un1 / (z + un0);
       p.out_lindepth[o] = lin == far_depth ? 0.0f : lin;
     }
     if (p.out_semantic) p.out_semantic[o] = d.semantic;
-  }
-  if (threadIdx.x == 0) {
-    if (overflow) atomicAdd(&p.stats[1], 1ull);
-    p.bin_count[gbin] = 0;  // ready for the next frame
   }
 }
 
@@ -991,16 +1254,29 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g) {
 // --------------------------------------------------------------------------
 
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
-                        hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_nodes, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, parents,
-                     local, abs_out, n);
+                        uint32_t* counters, hipStream_t s) {
+  const uint32_t blocks = n == 0 ? 1 : (n + kBlock - 1) / kBlock;  // always runs: resets counters
+  hipLaunchKernelGGL(k_nodes, dim3(blocks), dim3(kBlock), 0, s, parents, local, abs_out, n,
+                     counters);
   return hipGetLastError();
 }
 
-hipError_t launch_setup(const FrameParams& p, const WorkItem* items, uint32_t n, hipStream_t s) {
+int setup_blocks_per_cu() {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_setup_bin, kBlock, 0) != hipSuccess || b < 1)
+    b = 4;
+  return b;
+}
+
+hipError_t launch_setup(const FrameParams& p, const WorkItem* items, uint32_t n, uint32_t grid,
+                        hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup_bin, dim3(n), dim3(kBlock), 0, s, p, items);
+  hipLaunchKernelGGL(k_setup_bin, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_clip(const FrameParams& p, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_setup_clip, dim3(grid), dim3(kBlock), 0, s, p);
   return hipGetLastError();
 }
 
@@ -1008,6 +1284,12 @@ hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_raster, dim3(n), dim3(kBlock), 0, s, p, items);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_raster_full, dim3(n), dim3(kBlock), 0, s, p, items);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_resolve, dim3(n), dim3(kBlock), 0, s, p, items);
   return hipGetLastError();
 }
 

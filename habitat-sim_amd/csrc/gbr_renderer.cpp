@@ -28,8 +28,11 @@
 
 namespace gbr {
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
+                        uint32_t* counters, hipStream_t s);
+hipError_t launch_clip(const FrameParams& p, uint32_t grid, hipStream_t s);
+int setup_blocks_per_cu();
+hipError_t launch_setup(const FrameParams& p, const WorkItem* items, uint32_t n, uint32_t grid,
                         hipStream_t s);
-hipError_t launch_setup(const FrameParams& p, const WorkItem* items, uint32_t n, hipStream_t s);
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n,
                          hipStream_t s);
 hipError_t launch_gather(const GatherArgs& g, hipStream_t s);
@@ -139,7 +142,7 @@ struct MeshView {
   uint32_t material;
   Mat4 xf;
   uint32_t ranges_chunk = 0;              // chunk size the ranges were computed for
-  std::vector<uint32_t> ranges;           // (vmin, vcount) per chunk
+  std::vector<uint32_t> ranges;           // (first tri, tri count, vmin, vcount) per chunk
 };
 
 struct Draw {
@@ -176,6 +179,8 @@ struct gbr_renderer {
   gbr_config cfg{};
   gbr_tuning tun{};
   int device = 0;
+  int cu_count = 256;
+  int setup_bpc = 4;
   hipStream_t stream = nullptr;
 
   // assets (host mirrors)
@@ -222,6 +227,12 @@ struct gbr_renderer {
   DevBuf<float> d_local, d_abs;
   DevBuf<float> d_cameras, d_unproj;
   DevBuf<uint32_t> d_bin_count, d_bin_entries;
+  DevBuf<PrimRec> d_recs;
+  DevBuf<RasterRec> d_sub_recs;
+  DevBuf<PrimRef> d_clipped;
+  DevBuf<uint32_t> d_counters;
+  DevBuf<unsigned long long> d_vis;
+  uint32_t clipped_cap = 0, sub_cap = 0;
   DevBuf<unsigned long long> d_stats;
   DevBuf<uint32_t> d_color, d_sem;
   DevBuf<float> d_depth, d_lin;
@@ -291,22 +302,35 @@ uint32_t batch_id(Scene& sc, uint32_t vcolor, uint32_t mesh, uint32_t tex) {
   return (uint32_t)sc.batches.size() - 1;
 }
 
-// Per-chunk vertex ranges of a mesh view (for the LDS vertex cache)
+// Chunks of a mesh view for the setup work items: runs of consecutive
+// triangles, at most `chunk` long, cut where the referenced vertex range
+// would outgrow the LDS vertex cache.  Four entries per chunk: first
+// triangle, triangle count, lowest vertex, vertex range length.
 const std::vector<uint32_t>& view_ranges(gbr_renderer* r, MeshView& v, uint32_t chunk) {
   if (v.ranges_chunk == chunk) return v.ranges;
   v.ranges.clear();
   const MeshInfo& m = r->meshes[v.mesh];
   const uint32_t ntri = v.index_count / 3;
-  for (uint32_t t0 = 0; t0 < ntri; t0 += chunk) {
-    uint32_t t1 = std::min(ntri, t0 + chunk);
-    uint32_t lo = 0xffffffffu, hi = 0;
-    for (uint32_t i = m.ibase + v.index_offset + 3 * t0; i < m.ibase + v.index_offset + 3 * t1; ++i) {
-      lo = std::min(lo, r->indices[i]);
-      hi = std::max(hi, r->indices[i]);
+  const uint32_t* ix = r->indices.data() + m.ibase + v.index_offset;
+  uint32_t t0 = 0;
+  while (t0 < ntri) {
+    uint32_t lo = 0xffffffffu, hi = 0, t = t0;
+    while (t < ntri && t - t0 < chunk) {
+      uint32_t nlo = lo, nhi = hi;
+      for (int k = 0; k < 3; ++k) {
+        nlo = std::min(nlo, ix[3 * t + k]);
+        nhi = std::max(nhi, ix[3 * t + k]);
+      }
+      if (t > t0 && nhi - nlo + 1 > (uint32_t)kVertexCache) break;
+      lo = nlo;
+      hi = nhi;
+      ++t;
     }
-    if (lo > hi) lo = hi = 0;
+    v.ranges.push_back(t0);
+    v.ranges.push_back(t - t0);
     v.ranges.push_back(lo);
     v.ranges.push_back(hi - lo + 1);
+    t0 = t;
   }
   v.ranges_chunk = chunk;
   return v.ranges;
@@ -385,22 +409,26 @@ gbr_status rebuild_structure(gbr_renderer* r) {
       dd.semantic = dr.semantic;
       dd.flags = (m.color_fmt << kDrawColorShift) | (m.uv_half ? kDrawUvHalf : 0u);
       dd.prim_base = prim;
+      dd.scene = si;
       const uint32_t gdraw = (uint32_t)draws.size();
       draws.push_back(dd);
       const std::vector<uint32_t>& rg = view_ranges(r, v, chunk);
-      for (uint32_t t0 = 0, c = 0; t0 < dd.tri_count; t0 += chunk, ++c) {
+      for (size_t c = 0; c + 3 < rg.size(); c += 4) {
         WorkItem wi{};
         wi.scene = si;
         wi.draw = gdraw;
-        wi.tri_start = t0;
-        wi.tri_count = std::min(chunk, dd.tri_count - t0);
-        wi.vmin = rg[2 * c];  // global vertex indices
-        wi.vcount = rg[2 * c + 1];
+        wi.first_index = dd.index_base + 3 * rg[c];
+        wi.tri_count = rg[c + 1];
+        wi.prim = prim + rg[c];
+        wi.vmin = rg[c + 2];  // global vertex indices
+        wi.vcount = rg[c + 3];
+        wi.node = dd.node;
         items.push_back(wi);
       }
       prim += dd.tri_count;
     }
     sd.prim_count = prim;
+    sd.rec_base = tris;
     tris += prim;
     const uint32_t nb = (uint32_t)(bins_x * bins_y);
     sd.bin_base = (uint32_t)bins;
@@ -431,6 +459,13 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   r->n_bins = bins;
   TRY(r->d_bin_entries.reserve(std::max<uint64_t>(entries, 1)));
   r->entries_cap = entries;
+  TRY(r->d_recs.reserve(std::max<uint64_t>(tris, 1)));
+  // every primitive is listed at most once; the sub-triangle pool is bounded
+  // (bins of an unstored fan fall back to the exact rescan)
+  r->clipped_cap = (uint32_t)std::max<uint64_t>(tris, 1);
+  r->sub_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(tris, 1024), 1u << 21);
+  TRY(r->d_clipped.reserve(r->clipped_cap));
+  TRY(r->d_sub_recs.reserve(r->sub_cap));
   HIP_TRY(hipStreamSynchronize(s));
   r->structure_dirty = false;
   return GBR_OK;
@@ -493,6 +528,8 @@ gbr_status do_draw(gbr_renderer* r) {
   else TRY(upload_dirty_transforms(r));
   TRY(upload_cameras(r));
   TRY(r->d_stats.reserve(4));
+  TRY(r->d_counters.reserve(kCtrCount * kCtrStride));
+  TRY(r->d_vis.reserve((size_t)r->scene_count() * r->tile_pixels()));
   HIP_TRY(hipMemsetAsync(r->d_stats.ptr, 0, 4 * sizeof(unsigned long long), r->stream));
 
   FrameParams p{};
@@ -515,6 +552,14 @@ gbr_status do_draw(gbr_renderer* r) {
   p.abs_nodes = r->d_abs.ptr;
   p.bin_count = r->d_bin_count.ptr;
   p.bin_entries = r->d_bin_entries.ptr;
+  p.recs = r->d_recs.ptr;
+  p.sub_recs = r->d_sub_recs.ptr;
+  p.clipped = r->d_clipped.ptr;
+  p.counters = r->d_counters.ptr;
+  p.vis = r->d_vis.ptr;
+  p.clipped_cap = r->clipped_cap;
+  p.queue_grain = (uint32_t)r->tun.queue_grain;
+  p.sub_cap = r->sub_cap;
   const uint32_t pl = r->cfg.planes;
   p.out_color = (pl & GBR_PLANE_COLOR) ? r->d_color.ptr : nullptr;
   p.out_depth = (pl & GBR_PLANE_DEPTH) ? r->d_depth.ptr : nullptr;
@@ -528,8 +573,11 @@ gbr_status do_draw(gbr_renderer* r) {
   p.top_down = r->cfg.top_down ? 1 : 0;
   p.no_textures = (r->cfg.flags & GBR_FLAG_NO_TEXTURES) ? 1 : 0;
 
-  HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, r->stream));
-  HIP_TRY(launch_setup(p, r->d_items.ptr, r->n_items, r->stream));
+  HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, p.counters,
+                       r->stream));
+  HIP_TRY(launch_setup(p, r->d_items.ptr, r->n_items, (uint32_t)(r->cu_count * r->setup_bpc),
+                       r->stream));
+  HIP_TRY(launch_clip(p, (uint32_t)r->cu_count * 4u, r->stream));
   HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, r->stream));
   return GBR_OK;
 }
@@ -632,20 +680,23 @@ gbr_status gbr_tuning_init(gbr_tuning* t) {
   if (!t) return fail(GBR_ERR_ARGUMENT, "gbr_tuning_init: null");
   std::memset(t, 0, sizeof *t);
   t->bin_size = 32;
-  t->chunk_triangles = 256;
+  t->chunk_triangles = kChunkTris;
   t->bin_capacity_factor = 2.0f;
   t->min_bin_capacity = 256;
   t->cull_mode = GBR_CULL_NONE;
+  t->queue_grain = 2;
   return GBR_OK;
 }
 
 static gbr_status validate_tuning(const gbr_tuning* t) {
   if (t->bin_size != 8 && t->bin_size != 16 && t->bin_size != 32)
     return fail(GBR_ERR_ARGUMENT, "tuning: bin_size must be 8, 16 or 32");
-  if (t->chunk_triangles < 1 || t->chunk_triangles > kBlock)
-    return fail(GBR_ERR_ARGUMENT, "tuning: chunk_triangles must be in [1, 256]");
+  if (t->chunk_triangles < 1 || t->chunk_triangles > kChunkTris)
+    return fail(GBR_ERR_ARGUMENT, "tuning: chunk_triangles must be in [1, 1024]");
   if (!(t->bin_capacity_factor > 0.0f) || t->min_bin_capacity < 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: capacity must be positive");
+  if (t->queue_grain < 1 || t->queue_grain > 64)
+    return fail(GBR_ERR_ARGUMENT, "tuning: queue_grain must be in [1, 64]");
   if (t->cull_mode != GBR_CULL_NONE)
     return fail(GBR_ERR_UNSUPPORTED, "tuning: only GBR_CULL_NONE is available");
   return GBR_OK;
@@ -670,6 +721,12 @@ gbr_status gbr_create(const gbr_config* config, gbr_renderer** out) {
   if (!r) return fail(GBR_ERR_OOM, "gbr_create: out of host memory");
   r->cfg = *config;
   r->device = dev;
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      r->cu_count = cus;
+    r->setup_bpc = setup_blocks_per_cu();
+  }
   gbr_tuning_init(&r->tun);
   const uint64_t n = (uint64_t)config->tile_count[0] * config->tile_count[1];
   r->scenes.resize(n);
@@ -713,6 +770,11 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   r->d_textures.release();
   r->d_materials.release();
   r->d_draws.release();
+  r->d_recs.release();
+  r->d_sub_recs.release();
+  r->d_clipped.release();
+  r->d_counters.release();
+  r->d_vis.release();
   r->d_scenes.release();
   r->d_items.release();
   r->d_ritems.release();

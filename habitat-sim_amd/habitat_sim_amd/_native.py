@@ -46,7 +46,8 @@ class SceneStatsInfo(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("bin_size", ctypes.c_int32), ("chunk_triangles", ctypes.c_int32),
                 ("bin_capacity_factor", ctypes.c_float), ("min_bin_capacity", ctypes.c_int32),
-                ("cull_mode", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("cull_mode", ctypes.c_int32), ("queue_grain", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class FrameStatsInfo(ctypes.Structure):

@@ -104,11 +104,12 @@ typedef struct gbr_scene_stats_info {
 /* new: launch parameters, chosen by the autotuner */
 typedef struct gbr_tuning {
   int32_t bin_size;           /* screen bin edge in pixels: 8, 16 or 32 */
-  int32_t chunk_triangles;    /* triangles per setup work item: 64..256 */
+  int32_t chunk_triangles;    /* triangles per setup work item: 1..1024 */
   float bin_capacity_factor;  /* per-bin entry capacity relative to tris/bins */
   int32_t min_bin_capacity;   /* lower bound of the per-bin capacity */
   int32_t cull_mode;          /* GBR_CULL_* */
-  int32_t reserved[3];
+  int32_t queue_grain;        /* setup work items claimed per queue fetch: 1..64 */
+  int32_t reserved[2];
 } gbr_tuning;
 
 /* new: per-frame counters of the last gbr_draw() (read after a sync) */
