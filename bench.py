@@ -4,18 +4,23 @@ scene, 20 shared textured templates, trilinear sampling).
 
 A step = one draw() of all environments after a per-step camera update (a
 seeded random walk), with every input already resident in HBM; outputs stay
-in HBM (colour RGBA8, linear depth FP32, semantic uint32 planes).
+in HBM (colour RGBA8, linear depth FP32, semantic uint32 planes).  The walk is
+run once per culling mode (plain = every triangle set up, frustum, hiz =
+two-phase depth-pyramid occlusion with the previous frame as proposal); the
+headline value is the hiz mode, and the final frame of every mode is compared
+pixel by pixel with the plain one.
 
-Prints ONE JSON line (rank 0).  ``roofline`` uses the algorithmic bytes of a
-frame,
+Prints ONE JSON line (rank 0).  ``roofline`` is the headline mode's dominant
+stage: the frozen frame formula
 
-  B_frame = sum_env [ sum_inst (12 T + 16 V + 64) + sum_sensor W H 4 + 8 W H ]
+  B_frame = sum_env [ sum_drawn (12 T + 16 V + 64) + sum_sensor W H 4 + 8 W H ]
 
-(T triangles and V vertices of each drawn instance; 3 sensors), divided by
-the frame's GPU time measured with HIP events recorded on the renderer's own
-stream.  ``cpu_baseline`` times the CPU oracle (the project's C restatement of
-the GL pipeline, OpenMP over environments) on a bounded sample of the same
-workload on this host.
+split by stage (setup: geometry of the chunks it set up; resolve: outputs and
+texture footprint; cull: work item + bounds + transform per chunk), divided by
+that stage's mean duration from HIP events recorded on the renderer's stream
+around its launches over the timed steps.  ``cpu_baseline`` times the CPU
+oracle (the project's C restatement of the GL pipeline, OpenMP over
+environments) on a bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -31,35 +36,6 @@ for p in (ROOT, os.path.join(ROOT, "habitat-sim_amd")):
 
 METRIC = "frames/s @128x128 RGB-D, 512 envs; learned-cull vs plain speedup at 0 px diff"
 HBM_PEAK_GBS = 8000.0
-
-
-def algorithmic_bytes(spec, fd, sensors=3) -> float:
-    tv = {}
-    for name, (lo, hi) in _template_ranges(fd).items():
-        t = v = 0
-        for mi in range(lo, hi):
-            m = fd.meshes[fd.mesh_entries[mi].mesh]
-            t += len(m.indices) // 3
-            v += len(m.positions)
-        tv[name] = (t, v)
-    w, h = spec.tile
-    total = 0.0
-    for env in spec.envs:
-        for inst in env.instances:
-            t, v = tv[inst.template]
-            total += 12 * t + 16 * v + 64
-        total += sensors * w * h * 4 + 8 * w * h
-    return total
-
-
-def _template_ranges(fd):
-    """template name -> range of mesh entries (children of the named root)."""
-    out = {}
-    for i, n in enumerate(fd.nodes):
-        if n.parent == -1 and n.name:
-            kids = [k for k, e in enumerate(fd.mesh_entries) if fd.nodes[e.node].parent == i]
-            out[n.name] = (min(kids), max(kids) + 1) if kids else (0, 0)
-    return out
 
 
 def camera_walk(spec, steps, seed=7):
@@ -137,6 +113,68 @@ def copy_probe(nbytes: int) -> None:
     b.close()
 
 
+MODES = (("plain", 0), ("frustum", 1), ("hiz", 2))
+HEADLINE = "hiz"
+
+
+def stage_bytes(stage, spec, st, n_items, n_nodes):
+    """Algorithmic bytes a stage is responsible for in one frame (the frozen
+    frame formula split by stage; intermediate buffers count nothing)."""
+    w, h = spec.tile
+    envs = len(spec.envs)
+    t2, v2 = st["triangles_phase2"], st["vertices_phase2"]
+    t1, v1 = st["triangles_setup"] - t2, st["vertices_setup"] - v2
+    if stage == "setup1":
+        return 12 * t1 + 16 * v1 + 64 * st["items_phase1"]
+    if stage == "setup2":
+        return 12 * t2 + 16 * v2 + 64 * st["items_phase2"]
+    if stage == "cull":
+        return n_items * (32 + 24 + 64)  # work item, bounds, transform
+    if stage == "resolve":
+        return envs * (3 * w * h * 4 + 8 * w * h)
+    if stage == "nodes":
+        return n_nodes * 128
+    return 0
+
+
+def run_mode(r, mode, spec, views, proj, warmup, steps):
+    from habitat_sim_amd import hip
+    import numpy as np
+    r.set_tuning(cull_mode=mode, timing=1)
+    stream = r.stream()
+    for k in range(warmup):
+        r.update_cameras(0, proj, views[k])
+        r.draw()
+    r.synchronize()
+    r.frame_timings()  # drop the warmup frames
+    ev = [(hip.Event(), hip.Event()) for _ in range(steps)]
+    hip.hip().hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r.update_cameras(0, proj, views[warmup + k])
+        ev[k][0].record(stream)
+        r.draw()
+        ev[k][1].record(stream)
+    r.synchronize()
+    hip.hip().hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = sum(a.elapsed_ms(b) for a, b in ev) / steps
+    stages = r.frame_timings()
+    stats = r.frame_stats()
+    planes = {"color": r.color_image(), "semantic": r.semantic_image(),
+              "linear_depth": r.linear_depth_image()}
+    return {"wall": wall, "gpu_ms": gpu_ms, "stages": stages, "stats": stats, "planes": planes}
+
+
+def diff_pixels(a, b):
+    import numpy as np
+    d = np.zeros(a["semantic"].shape, bool)
+    for k in a:
+        x = a[k] != b[k]
+        d |= x.any(-1) if x.ndim == 3 else x
+    return int(d.sum())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,37 +212,45 @@ def main():
     proj = np.stack([env.projection for env in spec.envs]).astype(np.float32)
     n = len(spec.envs)
 
-    stream = r.stream()
-    for k in range(args.warmup):
-        r.update_cameras(0, proj, views[k])
-        r.draw()
-    r.synchronize()
-    ev = [(hip.Event(), hip.Event()) for _ in range(args.steps)]
-    hip.hip().hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        r.update_cameras(0, proj, views[args.warmup + k])
-        ev[k][0].record(stream)
-        r.draw()
-        ev[k][1].record(stream)
-    r.synchronize()
-    hip.hip().hipDeviceSynchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    gpu_ms = [a.elapsed_ms(b) for a, b in ev]
-    frame_ms = sum(gpu_ms) / len(gpu_ms)
-    stats = r.frame_stats()
+    runs = {}
+    for name, mode in MODES:
+        runs[name] = run_mode(r, mode, spec, views, proj, args.warmup, args.steps)
+    base = runs["plain"]
+    head = runs[HEADLINE]
+    n_items = head["stats"]["work_items"]
+    n_nodes = sum(2 + len(env.instances) for env in spec.envs)  # roots + children (upper bound)
 
-    bytes_frame = algorithmic_bytes(spec, spec.file)
-    achieved = bytes_frame / (frame_ms * 1e-3) / 1e9
+    cull = {}
+    for name, _ in MODES:
+        x = runs[name]
+        cull[name] = {
+            "env_frames_per_s": n * args.steps / x["wall"],
+            "gpu_ms_per_frame": x["gpu_ms"],
+            "speedup_vs_plain": base["wall"] / x["wall"],
+            "diff_pixels_vs_plain": diff_pixels(x["planes"], base["planes"]),
+            "items_phase1": x["stats"]["items_phase1"],
+            "items_phase2": x["stats"]["items_phase2"],
+            "triangles_setup": x["stats"]["triangles_setup"],
+            "bin_entries": x["stats"]["bin_entries"],
+            "stage_ms": {k: round(v, 4) for k, v in x["stages"].items() if v > 0},
+        }
+
+    # roofline of the headline mode's dominant stage
+    stages = head["stages"]
+    dom = max(stages, key=lambda k: stages[k])
+    dom_bytes = stage_bytes(dom, spec, head["stats"], n_items, n_nodes)
+    dom_ms = stages[dom]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    bytes_frame = sum(stage_bytes(k, spec, head["stats"], n_items, n_nodes) for k in stages)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_apartments512.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_frame")
+            tj = json.load(f)
+        traffic = tj.get("stages", {}).get(HEADLINE, {}).get(dom)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(spec, args.cpu_seconds)
-    value = n * args.steps / wall
+    value = n * args.steps / head["wall"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -212,7 +258,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -220,18 +266,21 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"apartments-{n}env-{args.res}x{args.res}-rgbd-semantic",
                    "envs": n, "resolution": [args.res, args.res],
-                   "triangles_per_frame": int(stats["triangles"]),
+                   "triangles_per_frame": int(head["stats"]["triangles"]),
                    "sensors": ["rgba8", "linear_depth_f32", "semantic_u32"],
-                   "cull": "none", "parallelism": "single-gpu"},
+                   "cull": HEADLINE, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "frame (k_nodes + k_setup_bin + k_raster)",
-                     "algorithmic_bytes_per_frame": bytes_frame,
-                     "gpu_ms_per_frame": frame_ms},
+                     "kernel": dom, "algorithmic_bytes_per_launch": dom_bytes,
+                     "launch_ms": dom_ms,
+                     "frame": {"algorithmic_bytes": bytes_frame, "gpu_ms": head["gpu_ms"],
+                               "achieved_gbs": bytes_frame / (head["gpu_ms"] * 1e-3) / 1e9}},
         "cpu_baseline": cpu,
         "batch_steps_per_s": value / n,
         "learned_cull_speedup": None,
-        "frame_stats": stats,
+        "cull_speedup": cull[HEADLINE]["speedup_vs_plain"],
+        "cull": cull,
+        "frame_stats": head["stats"],
     }
     if cpu:
         line["gpu_over_cpu"] = value / cpu["value"]

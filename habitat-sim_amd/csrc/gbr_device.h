@@ -82,7 +82,8 @@ struct alignas(16) RasterRec {
   int32_t X2, Y2;
   uint32_t prim;    // primitive order
   uint32_t draw;    // global draw index
-  float z0, dzdx, dzdy, pad2;
+  float z0, dzdx, dzdy;
+  float zlo;        // smallest vertex depth: fragments never land in front of it
 };
 
 // A primitive's record: the raster part (snapped window coordinates and
@@ -93,13 +94,16 @@ struct alignas(16) PrimRec {
   float cx0, cy0, cw0, cx1;
   float cy1, cw1, cx2, cy2;
   float cw2, s0, t0, s1;
-  float t1, s2, t2, pad;
+  float t1, s2, t2;
+  uint32_t item;    // work item (chunk) of the primitive
 };
 
 // A primitive that needs clipping (crosses the near/far plane or the guard band).
-struct alignas(8) PrimRef {
+struct alignas(16) PrimRef {
   uint32_t draw;  // global draw index
   uint32_t prim;  // scene-local primitive order
+  uint32_t item;  // work item (chunk) the primitive belongs to
+  uint32_t pad;
 };
 
 // device-side counters, reset by k_nodes at the start of every frame; each
@@ -107,10 +111,42 @@ struct alignas(8) PrimRef {
 enum : uint32_t {
   kCtrClipped = 0,
   kCtrSubTris = 1,
-  kCtrSetupQueue = 2,  // next setup work item (dynamic work queue)
-  kCtrCount = 3
+  kCtrQueue1 = 2,    // next setup work item, phase 1 (dynamic work queue)
+  kCtrQueue2 = 3,    // same, phase 2
+  kCtrList1 = 4,     // work items set up in phase 1
+  kCtrCand = 5,      // work items to test against the phase-1 depth pyramid
+  kCtrList2 = 6,     // work items set up in phase 2
+  kCtrClipMark = 7,  // clipped primitives handled by the phase-1 clip pass
+  kCtrCount = 8
 };
 constexpr uint32_t kCtrStride = 32;
+
+// frame statistics (64-bit): bin entries, overflowed bins, then triangles and
+// vertex-range entries set up in phase 1, the same for phase 2
+constexpr uint32_t kStatCount = 6;
+
+// Mesh-local bounding box of the vertices a work item references.
+struct alignas(16) ItemBounds {
+  float lo[3];
+  uint32_t pad0;
+  float hi[3];
+  uint32_t pad1;
+};
+
+// Per-frame screen footprint of a work item that passed the frustum test:
+// conservative pixel rectangle (inclusive) and smallest window depth any of
+// its fragments can have.
+struct alignas(16) ItemCull {
+  int16_t x0, y0, x1, y1;
+  float zmin;
+  uint32_t pad[2];
+};
+
+// culling modes (same values as GBR_CULL_*)
+enum : int32_t { kCullNone = 0, kCullFrustum = 1, kCullHiZ = 2, kCullLearned = 3 };
+
+// depth pyramid: level 0 = max depth per 8x8 pixels, level 1 per 32x32
+constexpr int kHiZ0 = 3, kHiZ1 = 5;  // log2 block edges
 
 
 struct alignas(16) RasterItem {
@@ -165,14 +201,29 @@ struct FrameParams {
   unsigned long long* vis;      // per-pixel visibility keys, scene planes
   uint32_t clipped_cap, sub_cap;
   uint32_t queue_grain;         // setup work items fetched per queue atomic
-  uint32_t pad_;
+  uint32_t n_items;             // work items of the frame
+  // culling
+  const WorkItem* items;
+  const ItemBounds* bounds;     // per work item
+  ItemCull* cull;               // per work item, this frame
+  uint32_t* list1;              // work items of phase 1
+  uint32_t* cand;               // work items awaiting the depth-pyramid test
+  uint32_t* list2;              // work items of phase 2
+  uint32_t* item_vis;           // frame number in which the item was last visible
+  const float* proposal;        // per work item probability of being visible (learned mode)
+  float* hiz;                   // depth pyramid, hiz_stride floats per scene
+  uint32_t frame;               // frame number (>= 2)
+  int32_t cull_mode;
+  int32_t hiz_w0, hiz_h0, hiz_w1, hiz_h1;  // pyramid level sizes in blocks
+  uint32_t hiz_stride;
+  float proposal_threshold;
   // outputs (env-major planes), any may be null
   uint32_t* out_color;
   float* out_depth;
   float* out_lindepth;
   uint32_t* out_semantic;
   // counters
-  unsigned long long* stats;  // [0] bin entries, [1] overflowed bins
+  unsigned long long* stats;  // kStatCount frame statistics
   int32_t bin_size;
   int32_t bin_shift;
   int32_t top_down;

@@ -216,9 +216,12 @@ __device__ void mat4_mul(const float* a, const float* b, float* out) {
 __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ parents,
                                                   const float* __restrict__ local,
                                                   float* __restrict__ abs_out, uint32_t n,
-                                                  uint32_t* __restrict__ counters) {
+                                                  uint32_t* __restrict__ counters,
+                                                  unsigned long long* __restrict__ stats) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < kCtrCount) counters[i * kCtrStride] = 0u;  // first kernel of every frame
+  // first kernel of every frame: reset the frame's counters
+  if (i < kCtrCount) counters[i * kCtrStride] = 0u;
+  if (i < kStatCount) stats[i] = 0ull;
   if (i >= n) return;
   int depth = 0;
   for (int32_t q = parents[i]; q >= 0 && depth < 1024; q = parents[q]) ++depth;
@@ -298,14 +301,21 @@ __device__ __forceinline__ bool covers(const TriSetup& s, const int32_t* X, cons
   return in;
 }
 
-__device__ __forceinline__ float depth_at(float z0, float dzdx, float dzdy, int32_t X0, int32_t Y0,
-                                          long long Px, long long Py) {
+// smallest of three vertex depths (same selection order as the oracle)
+__device__ __forceinline__ float min_depth(const float* Z) {
+  float m = Z[0] < Z[1] ? Z[0] : Z[1];
+  return Z[2] < m ? Z[2] : m;
+}
+
+__device__ __forceinline__ float depth_at(float z0, float dzdx, float dzdy, float zlo, int32_t X0,
+                                          int32_t Y0, long long Px, long long Py) {
   float ox = (float)(int32_t)(Px - X0);
   float oy = (float)(int32_t)(Py - Y0);
   float z = dzdx * ox;
   z = z0 + z;
   float zy = dzdy * oy;
   z = z + zy;
+  if (!(z >= zlo)) z = zlo;
   if (!(z > 0.0f)) z = 0.0f;
   if (z > 1.0f) z = 1.0f;
   return z;
@@ -362,8 +372,8 @@ __device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counte
 // --------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
-                                                      const WorkItem* __restrict__ items,
-                                                      uint32_t n_items) {
+                                                      const uint32_t* __restrict__ list,
+                                                      uint32_t list_ctr, uint32_t queue_ctr) {
   __shared__ int32_t s_x[kVertexCache];
   __shared__ int32_t s_y[kVertexCache];
   __shared__ float s_z[kVertexCache];
@@ -382,19 +392,25 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t written = 0;
+  unsigned long long tri_acc = 0, vtx_acc = 0;  // triangles / vertices set up (uniform)
   uint32_t grab = 0, grab_end = 0;  // items claimed from the queue (uniform)
+  // the whole frame, or the work items listed for this phase
+  const uint32_t n_items = list ? min(p.counters[list_ctr * kCtrStride], p.n_items) : p.n_items;
 
   while (true) {
     if (grab == grab_end) {
-      if (threadIdx.x == 0) s_item = atomicAdd(&p.counters[kCtrSetupQueue * kCtrStride], p.queue_grain);
+      if (threadIdx.x == 0) s_item = atomicAdd(&p.counters[queue_ctr * kCtrStride], p.queue_grain);
       __syncthreads();
       grab = s_item;
       grab_end = min(grab + p.queue_grain, n_items);
       __syncthreads();  // s_item may be rewritten only after everyone read it
       if (grab >= n_items) break;
     }
-    const uint32_t item = grab++;
-    const WorkItem wi = items[item];
+    const uint32_t item = list ? list[grab] : grab;
+    ++grab;
+    const WorkItem wi = p.items[item];
+    tri_acc += wi.tri_count;
+    vtx_acc += wi.vcount;
     const SceneDesc sc = p.scenes[wi.scene];
     const DrawDesc d = p.draws[wi.draw];
     const MaterialDesc& mat = p.materials[d.material];
@@ -475,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
           rec.r.prim = prim;
           rec.r.draw = wi.draw;
           depth_plane(X, Y, Z, rec.r.z0, rec.r.dzdx, rec.r.dzdy);
-          rec.r.pad2 = 0.0f;
+          rec.r.zlo = min_depth(Z);
           const uint32_t l0 = vi[0] - wi.vmin, l1 = vi[1] - wi.vmin, l2 = vi[2] - wi.vmin;
           rec.cx0 = s_cx[l0]; rec.cy0 = s_cy[l0]; rec.cw0 = s_cw[l0];
           rec.cx1 = s_cx[l1]; rec.cy1 = s_cy[l1]; rec.cw1 = s_cw[l1];
@@ -483,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
           rec.s0 = s_s[l0]; rec.t0 = s_t[l0];
           rec.s1 = s_s[l1]; rec.t1 = s_t[l1];
           rec.s2 = s_s[l2]; rec.t2 = s_t[l2];
-          rec.pad = 0.0f;
+          rec.item = item;
           p.recs[(size_t)sc.rec_base + prim] = rec;
           // count into the LDS per-bin counters; pairs beyond the LDS list
           // go straight to the global bin
@@ -502,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
             }
         }
       }
-      append_ref(p, kCtrClipped * kCtrStride, clip_me, PrimRef{wi.draw, prim}, p.clipped,
+      append_ref(p, kCtrClipped * kCtrStride, clip_me, PrimRef{wi.draw, prim, item, 0u}, p.clipped,
                  p.clipped_cap, lane, lt_mask);
     }
     __syncthreads();
@@ -524,6 +540,11 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
   }
   for (int o = 32; o > 0; o >>= 1) written += __shfl_xor(written, o);
   if (lane == 0 && written) atomicAdd(&p.stats[0], (unsigned long long)written);
+  if (threadIdx.x == 0 && tri_acc) {
+    const uint32_t st = queue_ctr == kCtrQueue2 ? 4u : 2u;
+    atomicAdd(&p.stats[st], tri_acc);
+    atomicAdd(&p.stats[st + 1], vtx_acc);
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -532,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
 
 struct BigTri {
   int32_t X[3], Y[3];
-  float z0, dzdx, dzdy;
+  float z0, dzdx, dzdy, zlo;
   uint32_t prim;
   PixRect r;
 };
@@ -547,7 +568,7 @@ struct RasterCtx {
 
 // rasterise one front-facing triangle with a known depth plane
 __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32_t* Y, float z0,
-                              float dzdx, float dzdy, uint32_t prim) {
+                              float dzdx, float dzdy, float zlo, uint32_t prim) {
   PixRect r = tri_pixels(X, Y, rc.bin);
   if (r.x0 > r.x1 || r.y0 > r.y1) return;
   const int npx = (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
@@ -557,7 +578,7 @@ __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32
       BigTri& b = rc.big[slot];
 #pragma unroll
       for (int k = 0; k < 3; ++k) { b.X[k] = X[k]; b.Y[k] = Y[k]; }
-      b.z0 = z0; b.dzdx = dzdx; b.dzdy = dzdy;
+      b.z0 = z0; b.dzdx = dzdx; b.dzdy = dzdy; b.zlo = zlo;
       b.prim = prim;
       b.r = r;
       return;
@@ -570,7 +591,7 @@ __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32
     for (int px = r.x0; px <= r.x1; ++px) {
       const long long Px = (long long)px * 256 + 128;
       if (!covers(s, X, Y, Px, Py)) continue;
-      const float z = depth_at(z0, dzdx, dzdy, X[0], Y[0], Px, Py);
+      const float z = depth_at(z0, dzdx, dzdy, zlo, X[0], Y[0], Px, Py);
       if (z < 1.0f) {
         const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | prim;
         atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
@@ -586,7 +607,7 @@ __device__ __forceinline__ void raster_tri(const RasterCtx& rc, const int32_t* X
   if (r.x0 > r.x1 || r.y0 > r.y1) return;
   float z0, dzdx, dzdy;
   depth_plane(X, Y, Z, z0, dzdx, dzdy);
-  raster_planed(rc, X, Y, z0, dzdx, dzdy, prim);
+  raster_planed(rc, X, Y, z0, dzdx, dzdy, min_depth(Z), prim);
 }
 
 __device__ void raster_big(const RasterCtx& rc, const BigTri& b) {
@@ -598,7 +619,7 @@ __device__ void raster_big(const RasterCtx& rc, const BigTri& b) {
     const int px = b.r.x0 + k % w, py = b.r.y0 + k / w;
     const long long Px = (long long)px * 256 + 128, Py = (long long)py * 256 + 128;
     if (!covers(s, b.X, b.Y, Px, Py)) continue;
-    const float z = depth_at(b.z0, b.dzdx, b.dzdy, b.X[0], b.Y[0], Px, Py);
+    const float z = depth_at(b.z0, b.dzdx, b.dzdy, b.zlo, b.X[0], b.Y[0], Px, Py);
     if (z < 1.0f) {
       const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | b.prim;
       atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
@@ -988,9 +1009,13 @@ __device__ __forceinline__ void overflow_rect(const FrameParams& p, const SceneD
       atomicAdd(&p.bin_count[sc.bin_base + (uint32_t)(by * sc.bins_x + bx)], sc.bin_cap + 1u);
 }
 
-__global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p) {
+// phase 1 takes the clipped list from its start and leaves its end in
+// kCtrClipMark; phase 2 continues from there
+__global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p, int phase) {
   const uint32_t n = min(p.counters[kCtrClipped * kCtrStride], p.clipped_cap);
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  const uint32_t start = phase == 1 ? 0u : p.counters[kCtrClipMark * kCtrStride];
+  if (phase == 1 && blockIdx.x == 0 && threadIdx.x == 0) p.counters[kCtrClipMark * kCtrStride] = n;
+  for (uint32_t i = start + blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const PrimRef ref = p.clipped[i];
     const DrawDesc d = p.draws[ref.draw];
     const SceneDesc sc = p.scenes[d.scene];
@@ -1034,7 +1059,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p) {
         rr.prim = ref.prim;
         rr.draw = ref.draw;
         depth_plane(tx, ty, tz, rr.z0, rr.dzdx, rr.dzdy);
-        rr.pad2 = 0.0f;
+        rr.zlo = min_depth(tz);
         p.sub_recs[slot] = rr;
         bin_rect_atomic(p, sc, r, kSubTri | slot);
       } else {
@@ -1051,6 +1076,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p) {
       PrimRec rec = {};
       rec.r.prim = ref.prim;
       rec.r.draw = ref.draw;
+      rec.item = ref.item;
       rec.cx0 = c[0].x; rec.cy0 = c[0].y; rec.cw0 = c[0].w;
       rec.cx1 = c[1].x; rec.cy1 = c[1].y; rec.cw1 = c[1].w;
       rec.cx2 = c[2].x; rec.cy2 = c[2].y; rec.cw2 = c[2].w;
@@ -1095,8 +1121,11 @@ __device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b
   }
 }
 
+// phase 1 starts every bin from the clear key; phase 2 merges into the
+// phase-1 keys and leaves bins without new entries untouched
 __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
-                                                   const RasterItem* __restrict__ items) {
+                                                   const RasterItem* __restrict__ items,
+                                                   int phase) {
   __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
   __shared__ BigTri s_big[kBigQueue];
   __shared__ int s_nbig;
@@ -1104,6 +1133,7 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
   const BinInfo b = bin_info(p, items[blockIdx.x]);
   const uint32_t n = p.bin_count[b.gbin];
   if (n > b.sc.bin_cap) return;  // overflowed: k_raster_full rescans this bin
+  if (phase == 2 && n == 0) return;
   const int B = p.bin_size;
   RasterCtx rc;
   rc.vis = s_vis;
@@ -1111,7 +1141,16 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
   rc.nbig = &s_nbig;
   rc.B = B;
   rc.bin = b.bin;
-  for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
+  if (phase == 2) {
+    const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
+    for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
+      const int lx = l % bw, ly = l / bw;
+      s_vis[ly * B + lx] =
+          p.vis[(size_t)b.sc.plane_offset + (size_t)(b.bin.y0 + ly) * b.sc.width + b.bin.x0 + lx];
+    }
+  } else {
+    for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
+  }
   if (threadIdx.x == 0) s_nbig = 0;
   __syncthreads();
   const uint32_t* entries = p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
@@ -1123,7 +1162,7 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
                                          : p.recs[(size_t)b.sc.rec_base + e].r;
       const int32_t X[3] = {rr.X0, rr.X1, rr.X2};
       const int32_t Y[3] = {rr.Y0, rr.Y1, rr.Y2};
-      raster_planed(rc, X, Y, rr.z0, rr.dzdx, rr.dzdy, rr.prim);
+      raster_planed(rc, X, Y, rr.z0, rr.dzdx, rr.dzdy, rr.zlo, rr.prim);
     }
     __syncthreads();
     const int nb = min(s_nbig, kBigQueue);
@@ -1209,6 +1248,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     const float z = __uint_as_float((uint32_t)(key >> 32));
     const PrimRec rs = p.recs[(size_t)sc.rec_base + prim];
     const DrawDesc d = p.draws[rs.r.draw];
+    if (p.item_vis && rs.item < p.n_items) p.item_vis[rs.item] = p.frame;  // feedback for the next frame
     if (p.out_color) p.out_color[o] = shade_pixel(p, sc, rs, d, prim, px, py);
     if (p.out_depth) p.out_depth[o] = z;
     if (p.out_lindepth) {
@@ -1217,6 +1257,198 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     if (p.out_semantic) p.out_semantic[o] = d.semantic;
   }
+}
+
+// --------------------------------------------------------------------------
+// Chunk culling
+//
+// k_cull tests each work item's mesh-local bounding box against the view
+// volume.  The box corners go through the same transform as the vertices;
+// eps bounds the float error of one transformed coordinate (8 ulp of the
+// |PV|.|M|.|p| magnitude, with an 8x margin), so an item rejected here has
+// every vertex outside one clip plane as setup computes it: its triangles
+// would all be trivially rejected, and skipping them changes nothing.
+//
+// Items that pass and lie strictly inside the near, far and guard-band
+// planes (so none of their triangles is clipped) get a conservative pixel
+// rectangle and the smallest window depth any fragment can have (fragments
+// are never in front of their triangle's nearest vertex).  If that depth is
+// greater than the largest phase-1 depth over the rectangle, every fragment
+// of the item fails the LESS test against what phase 1 already stored, so
+// the item is skipped in phase 2 without changing a pixel.
+// --------------------------------------------------------------------------
+
+constexpr float kEps = 4e-6f;  // relative error bound per clip coordinate (with margin)
+
+__device__ __forceinline__ void abs_bound(const float* M, const float* PV, float ax, float ay,
+                                          float az, float* e) {
+  float m[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    m[r] = fabsf(M[r]) * ax + fabsf(M[4 + r]) * ay + fabsf(M[8 + r]) * az + fabsf(M[12 + r]);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float a = fabsf(PV[r]) * m[0] + fabsf(PV[4 + r]) * m[1] + fabsf(PV[8 + r]) * m[2] +
+                    fabsf(PV[12 + r]) * m[3];
+    e[r] = kEps * a * 1.001f + 1e-30f;
+  }
+}
+
+// wave-aggregated append of one item index per flagged lane
+__device__ __forceinline__ void append_item(uint32_t* counter, uint32_t* list, bool flag,
+                                            uint32_t item, int lane, unsigned long long lt_mask) {
+  const unsigned long long m = __ballot(flag);
+  if (m == 0ull) return;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (flag) list[base + (uint32_t)__popcll(m & lt_mask)] = item;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
+  const uint32_t item = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  bool to1 = false, toc = false;
+  if (item < p.n_items) {
+    const WorkItem wi = p.items[item];
+    const SceneDesc sc = p.scenes[wi.scene];
+    const ItemBounds bb = p.bounds[item];
+    float M[16], PV[16];
+    load_mat(p.abs_nodes + (size_t)wi.node * 16, M);
+    load_mat(p.cameras + (size_t)wi.scene * 16, PV);
+    float e[4];
+    abs_bound(M, PV, fmaxf(fabsf(bb.lo[0]), fabsf(bb.hi[0])), fmaxf(fabsf(bb.lo[1]), fabsf(bb.hi[1])),
+              fmaxf(fabsf(bb.lo[2]), fabsf(bb.hi[2])), e);
+    const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
+    const float gx = kGuardPx / hw, gy = kGuardPx / hh;
+    const float mx = 2.0f * (e[0] + e[3]), my = 2.0f * (e[1] + e[3]), mz = 2.0f * (e[2] + e[3]);
+    uint32_t out_all = 63u;
+    bool inner = true;  // strictly inside near, far and the guard band
+    float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    float zmin = INFINITY, wmin = INFINITY;
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+      const float4 c = vertex_clip(M, PV, (k & 1) ? bb.hi[0] : bb.lo[0], (k & 2) ? bb.hi[1] : bb.lo[1],
+                                   (k & 4) ? bb.hi[2] : bb.lo[2]);
+      uint32_t o = 0;
+      if (c.x - c.w > mx) o |= 1;
+      if (-c.x - c.w > mx) o |= 2;
+      if (c.y - c.w > my) o |= 4;
+      if (-c.y - c.w > my) o |= 8;
+      if (-c.z - c.w > mz) o |= 16;
+      if (c.z - c.w > mz) o |= 32;
+      out_all &= o;
+      inner = inner && c.w > 2.0f * e[3] && c.z + c.w > mz && c.w - c.z > mz &&
+              gx * c.w - fabsf(c.x) > 2.0f * (gx * e[3] + e[0]) &&
+              gy * c.w - fabsf(c.y) > 2.0f * (gy * e[3] + e[1]);
+      if (inner) {
+        const float iw = 1.0f / c.w;
+        const float nx = c.x * iw, ny = c.y * iw, nz = c.z * iw;
+        xmin = fminf(xmin, nx); xmax = fmaxf(xmax, nx);
+        ymin = fminf(ymin, ny); ymax = fmaxf(ymax, ny);
+        zmin = fminf(zmin, nz); wmin = fminf(wmin, c.w);
+      }
+    }
+    if (out_all == 0u) {
+      if (!inner) {
+        to1 = true;  // crosses a clip plane: always drawn in phase 1
+      } else {
+        // error of a projected coordinate: (dx + |x/w| dw) / (w - dw), plus rounding
+        const float den = wmin - 2.0f * e[3];
+        const float ax = fmaxf(fabsf(xmin), fabsf(xmax)), ay = fmaxf(fabsf(ymin), fabsf(ymax));
+        const float ex = (2.0f * e[0] + ax * 2.0f * e[3]) / den + 1e-6f * (1.0f + ax);
+        const float ey = (2.0f * e[1] + ay * 2.0f * e[3]) / den + 1e-6f * (1.0f + ay);
+        const float ez = (2.0f * e[2] + 2.0f * e[3]) / den + 4e-6f;
+        const int x0 = max((int)floorf((xmin - ex) * hw + hw - 0.5f) - 1, 0);
+        const int x1 = min((int)ceilf((xmax + ex) * hw + hw - 0.5f) + 1, sc.width - 1);
+        const int y0 = max((int)floorf((ymin - ey) * hh + hh - 0.5f) - 1, 0);
+        const int y1 = min((int)ceilf((ymax + ey) * hh + hh - 0.5f) + 1, sc.height - 1);
+        if (x0 <= x1 && y0 <= y1) {  // else: no pixel centre of the tile can be covered
+          ItemCull ic;
+          ic.x0 = (int16_t)x0; ic.y0 = (int16_t)y0; ic.x1 = (int16_t)x1; ic.y1 = (int16_t)y1;
+          ic.zmin = (zmin - ez) * 0.5f + 0.5f;
+          ic.pad[0] = wi.scene;
+          ic.pad[1] = 0u;
+          bool proposed = false;
+          if (p.cull_mode == kCullFrustum) proposed = true;
+          else if (p.cull_mode == kCullHiZ) proposed = p.item_vis[item] + 1u == p.frame;
+          else proposed = p.proposal[item] > p.proposal_threshold;
+          if (proposed) {
+            to1 = true;
+          } else {
+            p.cull[item] = ic;
+            toc = true;
+          }
+        }
+      }
+    }
+  }
+  append_item(&p.counters[kCtrList1 * kCtrStride], p.list1, to1, item, lane, lt_mask);
+  append_item(&p.counters[kCtrCand * kCtrStride], p.cand, toc, item, lane, lt_mask);
+}
+
+// depth pyramid from the phase-1 visibility keys: one workgroup per 32x32
+// region of a scene
+__global__ __launch_bounds__(kBlock) void k_hiz(FrameParams p, uint32_t regions_per_scene) {
+  __shared__ uint32_t s_max[16];
+  const uint32_t scene = blockIdx.x / regions_per_scene;
+  const uint32_t reg = blockIdx.x % regions_per_scene;
+  const SceneDesc sc = p.scenes[scene];
+  const int rx = (int)(reg % (uint32_t)p.hiz_w1), ry = (int)(reg / (uint32_t)p.hiz_w1);
+  if (threadIdx.x < 16) s_max[threadIdx.x] = 0u;
+  __syncthreads();
+  const int lx = threadIdx.x & 31;
+  const int px = rx * 32 + lx;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ly = (threadIdx.x >> 5) + 8 * q;
+    const int py = ry * 32 + ly;
+    if (px < sc.width && py < sc.height) {
+      const unsigned long long key = p.vis[(size_t)sc.plane_offset + (size_t)py * sc.width + px];
+      atomicMax(&s_max[(ly >> 3) * 4 + (lx >> 3)], (uint32_t)(key >> 32));  // depth >= 0
+    }
+  }
+  __syncthreads();
+  float* h = p.hiz + (size_t)scene * p.hiz_stride;
+  if (threadIdx.x < 16) {
+    const int bx = rx * 4 + (int)(threadIdx.x & 3), by = ry * 4 + (int)(threadIdx.x >> 2);
+    if (bx < p.hiz_w0 && by < p.hiz_h0) h[by * p.hiz_w0 + bx] = __uint_as_float(s_max[threadIdx.x]);
+  }
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (int k = 0; k < 16; ++k) m = max(m, s_max[k]);
+    h[p.hiz_w0 * p.hiz_h0 + ry * p.hiz_w1 + rx] = __uint_as_float(m);
+  }
+}
+
+// candidates against the depth pyramid -> phase-2 list
+__global__ __launch_bounds__(kBlock) void k_cull_hiz(FrameParams p) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t n = min(p.counters[kCtrCand * kCtrStride], p.n_items);
+  bool keep = false;
+  uint32_t item = 0;
+  if (i < n) {
+    item = p.cand[i];
+    const ItemCull ic = p.cull[item];
+    const float* h = p.hiz + (size_t)ic.pad[0] * p.hiz_stride;
+    int lvl = kHiZ0, w = p.hiz_w0;
+    int bx0 = ic.x0 >> kHiZ0, bx1 = ic.x1 >> kHiZ0, by0 = ic.y0 >> kHiZ0, by1 = ic.y1 >> kHiZ0;
+    if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > 16) {
+      lvl = kHiZ1;
+      w = p.hiz_w1;
+      h += p.hiz_w0 * p.hiz_h0;
+      bx0 = ic.x0 >> kHiZ1; bx1 = ic.x1 >> kHiZ1; by0 = ic.y0 >> kHiZ1; by1 = ic.y1 >> kHiZ1;
+    }
+    (void)lvl;
+    for (int by = by0; by <= by1 && !keep; ++by)
+      for (int bx = bx0; bx <= bx1; ++bx)
+        if (!(ic.zmin > h[by * w + bx])) { keep = true; break; }
+  }
+  append_item(&p.counters[kCtrList2 * kCtrStride], p.list2, keep, item, lane, lt_mask);
 }
 
 // --------------------------------------------------------------------------
@@ -1254,10 +1486,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g) {
 // --------------------------------------------------------------------------
 
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
-                        uint32_t* counters, hipStream_t s) {
+                        uint32_t* counters, unsigned long long* stats, hipStream_t s) {
   const uint32_t blocks = n == 0 ? 1 : (n + kBlock - 1) / kBlock;  // always runs: resets counters
   hipLaunchKernelGGL(k_nodes, dim3(blocks), dim3(kBlock), 0, s, parents, local, abs_out, n,
-                     counters);
+                     counters, stats);
   return hipGetLastError();
 }
 
@@ -1268,28 +1500,54 @@ int setup_blocks_per_cu() {
   return b;
 }
 
-hipError_t launch_setup(const FrameParams& p, const WorkItem* items, uint32_t n, uint32_t grid,
-                        hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup_bin, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items, n);
+// list == nullptr: every work item of the frame; else the items listed in
+// list[0 .. counters[list_ctr])
+hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
+                        uint32_t queue_ctr, uint32_t grid, hipStream_t s) {
+  if (p.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_setup_bin, dim3(grid < p.n_items ? grid : p.n_items), dim3(kBlock), 0, s, p,
+                     list, list_ctr, queue_ctr);
   return hipGetLastError();
 }
 
-hipError_t launch_clip(const FrameParams& p, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_setup_clip, dim3(grid), dim3(kBlock), 0, s, p);
+hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s) {
+  hipLaunchKernelGGL(k_setup_clip, dim3(grid), dim3(kBlock), 0, s, p, phase);
   return hipGetLastError();
 }
 
-hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n,
+hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_raster, dim3(n), dim3(kBlock), 0, s, p, items);
+  hipLaunchKernelGGL(k_raster, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_raster_full, dim3(n), dim3(kBlock), 0, s, p, items);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve(const FrameParams& p, const RasterItem* items, uint32_t n,
+                          hipStream_t s) {
+  if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_resolve, dim3(n), dim3(kBlock), 0, s, p, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_cull(const FrameParams& p, hipStream_t s) {
+  if (p.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cull, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_hiz(const FrameParams& p, uint32_t scenes, hipStream_t s) {
+  const uint32_t regions = (uint32_t)(p.hiz_w1 * p.hiz_h1);
+  if (scenes == 0 || regions == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hiz, dim3(scenes * regions), dim3(kBlock), 0, s, p, regions);
+  return hipGetLastError();
+}
+
+hipError_t launch_cull_hiz(const FrameParams& p, hipStream_t s) {
+  if (p.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cull_hiz, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p);
   return hipGetLastError();
 }
 

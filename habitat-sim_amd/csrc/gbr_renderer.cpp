@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,13 +29,18 @@
 
 namespace gbr {
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
-                        uint32_t* counters, hipStream_t s);
-hipError_t launch_clip(const FrameParams& p, uint32_t grid, hipStream_t s);
+                        uint32_t* counters, unsigned long long* stats, hipStream_t s);
+hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s);
 int setup_blocks_per_cu();
-hipError_t launch_setup(const FrameParams& p, const WorkItem* items, uint32_t n, uint32_t grid,
-                        hipStream_t s);
-hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n,
+hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
+                        uint32_t queue_ctr, uint32_t grid, hipStream_t s);
+hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
                          hipStream_t s);
+hipError_t launch_resolve(const FrameParams& p, const RasterItem* items, uint32_t n,
+                          hipStream_t s);
+hipError_t launch_cull(const FrameParams& p, hipStream_t s);
+hipError_t launch_hiz(const FrameParams& p, uint32_t scenes, hipStream_t s);
+hipError_t launch_cull_hiz(const FrameParams& p, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& g, hipStream_t s);
 }  // namespace gbr
 
@@ -135,15 +141,24 @@ struct MeshInfo {
   uint32_t uv_half;
 };
 
+// A run of consecutive triangles of a mesh view: the unit of setup work and
+// of culling.
+struct ChunkInfo {
+  uint32_t t0, tn;        // first triangle (view-relative), triangle count
+  uint32_t vmin, vcount;  // referenced global vertex range
+  float lo[3], hi[3];     // mesh-local bounding box of the referenced vertices
+};
+
 struct MeshView {
   uint32_t mesh;
   uint32_t index_offset;  // in indices, relative to the mesh
   uint32_t index_count;
   uint32_t material;
   Mat4 xf;
-  uint32_t ranges_chunk = 0;              // chunk size the ranges were computed for
-  std::vector<uint32_t> ranges;           // (first tri, tri count, vmin, vcount) per chunk
+  uint32_t ranges_chunk = 0;              // chunk size the chunks were computed for
+  std::vector<ChunkInfo> chunks;
 };
+
 
 struct Draw {
   uint32_t batch, node, material, view, semantic;
@@ -222,6 +237,11 @@ struct gbr_renderer {
   DevBuf<DrawDesc> d_draws;
   DevBuf<SceneDesc> d_scenes;
   DevBuf<WorkItem> d_items;
+  DevBuf<ItemBounds> d_bounds;
+  DevBuf<ItemCull> d_cull;
+  DevBuf<uint32_t> d_list1, d_cand, d_list2, d_item_vis;
+  DevBuf<float> d_hiz;
+  uint32_t frame_no = 1;
   DevBuf<RasterItem> d_ritems;
   DevBuf<int32_t> d_parents;
   DevBuf<float> d_local, d_abs;
@@ -241,6 +261,12 @@ struct gbr_renderer {
   float* h_cam = nullptr;
   size_t h_cam_cap = 0;
   hipEvent_t cam_event = nullptr;
+  // optional per-stage timing of gbr_draw (tuning.timing): one event set per
+  // frame since the last gbr_frame_timings(), up to kTimingFrames frames
+  std::vector<std::array<hipEvent_t, 2 * GBR_TIMING_SLOTS>> t_pool;
+  std::vector<std::array<bool, GBR_TIMING_SLOTS>> t_used;
+  size_t t_frames = 0;
+  int t_cur = -1;
   bool cam_event_pending = false;
 
   uint64_t tile_pixels() const { return (uint64_t)cfg.tile_size[0] * cfg.tile_size[1]; }
@@ -304,11 +330,10 @@ uint32_t batch_id(Scene& sc, uint32_t vcolor, uint32_t mesh, uint32_t tex) {
 
 // Chunks of a mesh view for the setup work items: runs of consecutive
 // triangles, at most `chunk` long, cut where the referenced vertex range
-// would outgrow the LDS vertex cache.  Four entries per chunk: first
-// triangle, triangle count, lowest vertex, vertex range length.
-const std::vector<uint32_t>& view_ranges(gbr_renderer* r, MeshView& v, uint32_t chunk) {
-  if (v.ranges_chunk == chunk) return v.ranges;
-  v.ranges.clear();
+// would outgrow the LDS vertex cache.
+const std::vector<ChunkInfo>& view_chunks(gbr_renderer* r, MeshView& v, uint32_t chunk) {
+  if (v.ranges_chunk == chunk) return v.chunks;
+  v.chunks.clear();
   const MeshInfo& m = r->meshes[v.mesh];
   const uint32_t ntri = v.index_count / 3;
   const uint32_t* ix = r->indices.data() + m.ibase + v.index_offset;
@@ -326,14 +351,28 @@ const std::vector<uint32_t>& view_ranges(gbr_renderer* r, MeshView& v, uint32_t 
       hi = nhi;
       ++t;
     }
-    v.ranges.push_back(t0);
-    v.ranges.push_back(t - t0);
-    v.ranges.push_back(lo);
-    v.ranges.push_back(hi - lo + 1);
+    ChunkInfo ci{};
+    ci.t0 = t0;
+    ci.tn = t - t0;
+    ci.vmin = lo;
+    ci.vcount = hi - lo + 1;
+    for (int k = 0; k < 3; ++k) {
+      ci.lo[k] = INFINITY;
+      ci.hi[k] = -INFINITY;
+    }
+    for (uint32_t i = 3 * t0; i < 3 * t; ++i) {
+      const uint32_t g = ix[i];
+      const float q[3] = {r->pos_x[g], r->pos_y[g], r->pos_z[g]};
+      for (int k = 0; k < 3; ++k) {
+        ci.lo[k] = std::min(ci.lo[k], q[k]);
+        ci.hi[k] = std::max(ci.hi[k], q[k]);
+      }
+    }
+    v.chunks.push_back(ci);
     t0 = t;
   }
   v.ranges_chunk = chunk;
-  return v.ranges;
+  return v.chunks;
 }
 
 gbr_status upload_assets(gbr_renderer* r) {
@@ -369,6 +408,7 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   std::vector<float> local;
   std::vector<DrawDesc> draws;
   std::vector<WorkItem> items;
+  std::vector<ItemBounds> bounds;
   std::vector<RasterItem> ritems;
   r->scene_node_base.assign(nsc, 0);
   r->scene_desc.assign(nsc, SceneDesc{});
@@ -412,18 +452,23 @@ gbr_status rebuild_structure(gbr_renderer* r) {
       dd.scene = si;
       const uint32_t gdraw = (uint32_t)draws.size();
       draws.push_back(dd);
-      const std::vector<uint32_t>& rg = view_ranges(r, v, chunk);
-      for (size_t c = 0; c + 3 < rg.size(); c += 4) {
+      for (const ChunkInfo& ci : view_chunks(r, v, chunk)) {
         WorkItem wi{};
         wi.scene = si;
         wi.draw = gdraw;
-        wi.first_index = dd.index_base + 3 * rg[c];
-        wi.tri_count = rg[c + 1];
-        wi.prim = prim + rg[c];
-        wi.vmin = rg[c + 2];  // global vertex indices
-        wi.vcount = rg[c + 3];
+        wi.first_index = dd.index_base + 3 * ci.t0;
+        wi.tri_count = ci.tn;
+        wi.prim = prim + ci.t0;
+        wi.vmin = ci.vmin;  // global vertex indices
+        wi.vcount = ci.vcount;
         wi.node = dd.node;
         items.push_back(wi);
+        ItemBounds ib{};
+        for (int k = 0; k < 3; ++k) {
+          ib.lo[k] = ci.lo[k];
+          ib.hi[k] = ci.hi[k];
+        }
+        bounds.push_back(ib);
       }
       prim += dd.tri_count;
     }
@@ -451,6 +496,18 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   TRY(upload(r->d_draws, draws, s));
   TRY(upload(r->d_scenes, r->scene_desc, s));
   TRY(upload(r->d_items, items, s));
+  TRY(upload(r->d_bounds, bounds, s));
+  {
+    const size_t n = std::max<size_t>(items.size(), 1);
+    TRY(r->d_cull.reserve(n));
+    TRY(r->d_list1.reserve(n));
+    TRY(r->d_cand.reserve(n));
+    TRY(r->d_list2.reserve(n));
+    TRY(r->d_item_vis.reserve(n));
+    // visibility history refers to the old item numbering
+    HIP_TRY(hipMemsetAsync(r->d_item_vis.ptr, 0, n * sizeof(uint32_t), s));
+    r->frame_no = 1;
+  }
   TRY(upload(r->d_ritems, ritems, s));
   if (bins > r->n_bins || r->d_bin_count.ptr == nullptr) {
     TRY(r->d_bin_count.reserve(bins + 1));
@@ -520,6 +577,37 @@ gbr_status ensure_planes(gbr_renderer* r) {
   return GBR_OK;
 }
 
+constexpr size_t kTimingFrames = 4096;
+
+gbr_status ensure_timing_events(gbr_renderer* r) {
+  r->t_cur = -1;
+  if (!r->tun.timing || r->t_frames >= kTimingFrames) return GBR_OK;
+  if (r->t_frames == r->t_pool.size()) {
+    std::array<hipEvent_t, 2 * GBR_TIMING_SLOTS> set{};
+    for (hipEvent_t& e : set) HIP_TRY(hipEventCreate(&e));
+    r->t_pool.push_back(set);
+    r->t_used.emplace_back();
+  }
+  r->t_cur = (int)r->t_frames++;
+  r->t_used[r->t_cur].fill(false);
+  return GBR_OK;
+}
+
+// Brackets one stage of the frame with events on the renderer's stream.
+struct StageTimer {
+  gbr_renderer* r;
+  int slot;
+  StageTimer(gbr_renderer* rr, int s) : r(rr), slot(s) {
+    if (r->t_cur >= 0) (void)hipEventRecord(r->t_pool[r->t_cur][2 * slot], r->stream);
+  }
+  ~StageTimer() {
+    if (r->t_cur >= 0) {
+      (void)hipEventRecord(r->t_pool[r->t_cur][2 * slot + 1], r->stream);
+      r->t_used[r->t_cur][slot] = true;
+    }
+  }
+};
+
 gbr_status do_draw(gbr_renderer* r) {
   TRY(set_device(r));
   TRY(ensure_planes(r));
@@ -527,10 +615,10 @@ gbr_status do_draw(gbr_renderer* r) {
   if (r->structure_dirty) TRY(rebuild_structure(r));
   else TRY(upload_dirty_transforms(r));
   TRY(upload_cameras(r));
-  TRY(r->d_stats.reserve(4));
+  TRY(r->d_stats.reserve(kStatCount));
   TRY(r->d_counters.reserve(kCtrCount * kCtrStride));
   TRY(r->d_vis.reserve((size_t)r->scene_count() * r->tile_pixels()));
-  HIP_TRY(hipMemsetAsync(r->d_stats.ptr, 0, 4 * sizeof(unsigned long long), r->stream));
+  TRY(ensure_timing_events(r));
 
   FrameParams p{};
   p.pos_x = r->d_pos_x.ptr;
@@ -559,6 +647,32 @@ gbr_status do_draw(gbr_renderer* r) {
   p.vis = r->d_vis.ptr;
   p.clipped_cap = r->clipped_cap;
   p.queue_grain = (uint32_t)r->tun.queue_grain;
+  p.n_items = r->n_items;
+  p.items = r->d_items.ptr;
+  p.bounds = r->d_bounds.ptr;
+  p.cull = r->d_cull.ptr;
+  p.list1 = r->d_list1.ptr;
+  p.cand = r->d_cand.ptr;
+  p.list2 = r->d_list2.ptr;
+  p.cull_mode = r->tun.cull_mode;
+  // visibility feedback is kept in every mode, so switching to the
+  // previous-frame proposal starts from the last frame drawn
+  p.item_vis = r->d_item_vis.ptr;
+  p.frame = ++r->frame_no;
+  p.proposal = nullptr;
+  p.proposal_threshold = 0.5f;
+  {
+    const int W = r->cfg.tile_size[0], H = r->cfg.tile_size[1];
+    p.hiz_w0 = (W + 7) >> 3;
+    p.hiz_h0 = (H + 7) >> 3;
+    p.hiz_w1 = (W + 31) >> 5;
+    p.hiz_h1 = (H + 31) >> 5;
+    p.hiz_stride = (uint32_t)(p.hiz_w0 * p.hiz_h0 + p.hiz_w1 * p.hiz_h1);
+    if (p.cull_mode >= kCullHiZ) {
+      TRY(r->d_hiz.reserve((size_t)p.hiz_stride * std::max<uint32_t>(r->scene_count(), 1)));
+      p.hiz = r->d_hiz.ptr;
+    }
+  }
   p.sub_cap = r->sub_cap;
   const uint32_t pl = r->cfg.planes;
   p.out_color = (pl & GBR_PLANE_COLOR) ? r->d_color.ptr : nullptr;
@@ -573,12 +687,55 @@ gbr_status do_draw(gbr_renderer* r) {
   p.top_down = r->cfg.top_down ? 1 : 0;
   p.no_textures = (r->cfg.flags & GBR_FLAG_NO_TEXTURES) ? 1 : 0;
 
-  HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, p.counters,
-                       r->stream));
-  HIP_TRY(launch_setup(p, r->d_items.ptr, r->n_items, (uint32_t)(r->cu_count * r->setup_bpc),
-                       r->stream));
-  HIP_TRY(launch_clip(p, (uint32_t)r->cu_count * 4u, r->stream));
-  HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, r->stream));
+  {
+    StageTimer t(r, GBR_TIMING_NODES);
+    HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, p.counters,
+                         p.stats, r->stream));
+  }
+  const uint32_t setup_grid = (uint32_t)(r->cu_count * r->setup_bpc);
+  const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
+  const bool culled = p.cull_mode != kCullNone;
+  if (culled) {
+    // phase 1: items that pass the frustum test and are proposed visible
+    // (all of them in frustum mode); the rest become phase-2 candidates
+    StageTimer t(r, GBR_TIMING_CULL);
+    HIP_TRY(launch_cull(p, r->stream));
+  }
+  {
+    StageTimer t(r, GBR_TIMING_SETUP1);
+    HIP_TRY(launch_setup(p, culled ? p.list1 : nullptr, kCtrList1, kCtrQueue1, setup_grid,
+                         r->stream));
+  }
+  {
+    StageTimer t(r, GBR_TIMING_CLIP1);
+    HIP_TRY(launch_clip(p, clip_grid, 1, r->stream));
+  }
+  {
+    StageTimer t(r, GBR_TIMING_RASTER1);
+    HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 1, r->stream));
+  }
+  if (p.cull_mode >= kCullHiZ) {
+    // phase 2: candidates not hidden behind the phase-1 depth pyramid
+    {
+      StageTimer t(r, GBR_TIMING_HIZ);
+      HIP_TRY(launch_hiz(p, r->scene_count(), r->stream));
+      HIP_TRY(launch_cull_hiz(p, r->stream));
+    }
+    {
+      StageTimer t(r, GBR_TIMING_SETUP2);
+      HIP_TRY(launch_setup(p, p.list2, kCtrList2, kCtrQueue2, setup_grid, r->stream));
+    }
+    {
+      StageTimer t(r, GBR_TIMING_CLIP2);
+      HIP_TRY(launch_clip(p, clip_grid, 2, r->stream));
+    }
+    {
+      StageTimer t(r, GBR_TIMING_RASTER2);
+      HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 2, r->stream));
+    }
+  }
+  StageTimer t(r, GBR_TIMING_RESOLVE);
+  HIP_TRY(launch_resolve(p, r->d_ritems.ptr, r->n_ritems, r->stream));
   return GBR_OK;
 }
 
@@ -695,10 +852,15 @@ static gbr_status validate_tuning(const gbr_tuning* t) {
     return fail(GBR_ERR_ARGUMENT, "tuning: chunk_triangles must be in [1, 1024]");
   if (!(t->bin_capacity_factor > 0.0f) || t->min_bin_capacity < 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: capacity must be positive");
+  if (t->timing != 0 && t->timing != 1)
+    return fail(GBR_ERR_ARGUMENT, "tuning: timing must be 0 or 1");
   if (t->queue_grain < 1 || t->queue_grain > 64)
     return fail(GBR_ERR_ARGUMENT, "tuning: queue_grain must be in [1, 64]");
-  if (t->cull_mode != GBR_CULL_NONE)
-    return fail(GBR_ERR_UNSUPPORTED, "tuning: only GBR_CULL_NONE is available");
+  if (t->cull_mode == GBR_CULL_LEARNED)
+    return fail(GBR_ERR_UNSUPPORTED, "tuning: GBR_CULL_LEARNED needs a visibility model");
+  if (t->cull_mode != GBR_CULL_NONE && t->cull_mode != GBR_CULL_FRUSTUM &&
+      t->cull_mode != GBR_CULL_HIZ)
+    return fail(GBR_ERR_ARGUMENT, "tuning: unknown cull_mode");
   return GBR_OK;
 }
 
@@ -708,6 +870,8 @@ gbr_status gbr_create(const gbr_config* config, gbr_renderer** out) {
   if (config->tile_size[0] < 1 || config->tile_size[1] < 1 || config->tile_count[0] < 1 ||
       config->tile_count[1] < 1)
     return fail(GBR_ERR_ARGUMENT, "gbr_create: tile size and count must be positive");
+  if (config->tile_size[0] > 8192 || config->tile_size[1] > 8192)
+    return fail(GBR_ERR_ARGUMENT, "gbr_create: tile size above 8192 pixels (guard band)");
   if (config->max_light_count != 0)
     return fail(GBR_ERR_UNSUPPORTED, "gbr_create: lights (max_light_count > 0) are not supported");
   if (config->planes & ~GBR_PLANE_ALL) return fail(GBR_ERR_ARGUMENT, "gbr_create: unknown plane bits");
@@ -785,6 +949,9 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   r->d_stage.release();
   if (r->h_cam) (void)hipHostFree(r->h_cam);
   if (r->cam_event) (void)hipEventDestroy(r->cam_event);
+  for (auto& set : r->t_pool)
+    for (hipEvent_t e : set)
+      if (e) (void)hipEventDestroy(e);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
   return GBR_OK;
@@ -1286,13 +1453,48 @@ gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
   out->triangles = r->n_tris;
   out->raster_items = r->n_ritems;
   if (r->d_stats.ptr) {
-    unsigned long long s[4] = {0, 0, 0, 0};
+    unsigned long long s[kStatCount] = {};
     TRY(set_device(r));
     HIP_TRY(hipMemcpyAsync(s, r->d_stats.ptr, sizeof s, hipMemcpyDeviceToHost, r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
     out->bin_entries = s[0];
     out->overflowed_bins = s[1];
+    out->triangles_setup = s[2] + s[4];
+    out->vertices_setup = s[3] + s[5];
+    out->triangles_phase2 = s[4];
+    out->vertices_phase2 = s[5];
   }
+  if (r->d_counters.ptr && r->tun.cull_mode != GBR_CULL_NONE) {
+    std::vector<uint32_t> c(kCtrCount * kCtrStride);
+    HIP_TRY(hipMemcpyAsync(c.data(), r->d_counters.ptr, c.size() * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    out->items_phase1 = c[kCtrList1 * kCtrStride];
+    out->items_candidates = c[kCtrCand * kCtrStride];
+    out->items_phase2 = r->tun.cull_mode >= GBR_CULL_HIZ ? c[kCtrList2 * kCtrStride] : 0;
+  } else {
+    out->items_phase1 = r->n_items;
+  }
+  return GBR_OK;
+}
+
+gbr_status gbr_frame_timings(gbr_renderer* r, float* ms) {
+  if (!r || !ms) return fail(GBR_ERR_ARGUMENT, "frame_timings: null argument");
+  for (int i = 0; i < GBR_TIMING_SLOTS; ++i) ms[i] = 0.0f;
+  if (!r->tun.timing) return fail(GBR_ERR_STATE, "frame_timings: tuning.timing is off");
+  if (r->t_frames == 0) return fail(GBR_ERR_STATE, "frame_timings: no frame drawn since the last call");
+  TRY(set_device(r));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  double sum[GBR_TIMING_SLOTS] = {};
+  for (size_t f = 0; f < r->t_frames; ++f)
+    for (int i = 0; i < GBR_TIMING_SLOTS; ++i) {
+      if (!r->t_used[f][i]) continue;
+      float v = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&v, r->t_pool[f][2 * i], r->t_pool[f][2 * i + 1]));
+      sum[i] += v;
+    }
+  for (int i = 0; i < GBR_TIMING_SLOTS; ++i) ms[i] = (float)(sum[i] / (double)r->t_frames);
+  r->t_frames = 0;
   return GBR_OK;
 }
 

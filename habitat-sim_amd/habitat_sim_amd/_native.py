@@ -23,7 +23,9 @@ FILE_GENERATE_MIPMAP = 0x2
 PLANE_COLOR, PLANE_DEPTH, PLANE_LINEAR_DEPTH, PLANE_SEMANTIC, PLANE_ALL = 0x1, 0x2, 0x4, 0x8, 0xF
 FORMAT_RGBA8, FORMAT_RGB8 = 0, 1
 MEMORY_HOST, MEMORY_DEVICE = 0, 1
-CULL_NONE, CULL_FRUSTUM = 0, 1
+CULL_NONE, CULL_FRUSTUM, CULL_HIZ, CULL_LEARNED = 0, 1, 2, 3
+TIMING_STAGES = ("nodes", "cull", "setup1", "clip1", "raster1", "hiz", "setup2", "clip2",
+                 "raster2", "resolve")
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_u8_p = ctypes.POINTER(ctypes.c_uint8)
@@ -47,13 +49,16 @@ class Tuning(ctypes.Structure):
     _fields_ = [("bin_size", ctypes.c_int32), ("chunk_triangles", ctypes.c_int32),
                 ("bin_capacity_factor", ctypes.c_float), ("min_bin_capacity", ctypes.c_int32),
                 ("cull_mode", ctypes.c_int32), ("queue_grain", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("timing", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
 
 class FrameStatsInfo(ctypes.Structure):
     _fields_ = [("work_items", ctypes.c_uint64), ("triangles", ctypes.c_uint64),
                 ("bin_entries", ctypes.c_uint64), ("overflowed_bins", ctypes.c_uint64),
-                ("raster_items", ctypes.c_uint64)]
+                ("raster_items", ctypes.c_uint64), ("items_phase1", ctypes.c_uint64),
+                ("items_candidates", ctypes.c_uint64), ("items_phase2", ctypes.c_uint64),
+                ("triangles_setup", ctypes.c_uint64), ("vertices_setup", ctypes.c_uint64),
+                ("triangles_phase2", ctypes.c_uint64), ("vertices_phase2", ctypes.c_uint64)]
 
 
 class MeshDesc(ctypes.Structure):
@@ -155,6 +160,7 @@ SIGNATURES = {
     "gbr_get_tuning": (S, [R, ctypes.POINTER(Tuning)]),
     "gbr_set_tuning": (S, [R, ctypes.POINTER(Tuning)]),
     "gbr_frame_stats": (S, [R, ctypes.POINTER(FrameStatsInfo)]),
+    "gbr_frame_timings": (S, [R, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None

@@ -429,6 +429,12 @@ class RendererStandalone:
             setattr(t, k, v)
         N.call("gbr_set_tuning", self._h, ctypes.byref(t))
 
+    def frame_timings(self) -> dict:
+        """Milliseconds per stage of the last draw() (needs tuning timing=1)."""
+        ms = (ctypes.c_float * len(N.TIMING_STAGES))()
+        N.call("gbr_frame_timings", self._h, ms)
+        return dict(zip(N.TIMING_STAGES, list(ms)))
+
     def frame_stats(self) -> dict:
         s = N.FrameStatsInfo()
         N.call("gbr_frame_stats", self._h, ctypes.byref(s))

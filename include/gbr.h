@@ -76,6 +76,9 @@ enum {
 /* culling modes (new) */
 #define GBR_CULL_NONE 0    /* the reference's method: every triangle set up */
 #define GBR_CULL_FRUSTUM 1 /* per-chunk frustum test before setup (exact) */
+#define GBR_CULL_HIZ 2     /* + two-phase depth-pyramid occlusion test; phase 1 =
+                              chunks visible in the previous frame (exact) */
+#define GBR_CULL_LEARNED 3 /* + phase 1 = chunks a visibility model predicts (exact) */
 
 typedef struct gbr_renderer gbr_renderer;
 
@@ -109,8 +112,22 @@ typedef struct gbr_tuning {
   int32_t min_bin_capacity;   /* lower bound of the per-bin capacity */
   int32_t cull_mode;          /* GBR_CULL_* */
   int32_t queue_grain;        /* setup work items claimed per queue fetch: 1..64 */
-  int32_t reserved[2];
+  int32_t timing;             /* 1: time each stage of gbr_draw() with HIP events */
+  int32_t reserved[1];
 } gbr_tuning;
+
+/* stages timed when gbr_tuning.timing is set (gbr_frame_timings) */
+#define GBR_TIMING_NODES 0   /* absolute transforms, counter reset */
+#define GBR_TIMING_CULL 1    /* chunk frustum test + proposal */
+#define GBR_TIMING_SETUP1 2  /* triangle setup + binning, phase 1 */
+#define GBR_TIMING_CLIP1 3   /* near/guard-band clipping, phase 1 */
+#define GBR_TIMING_RASTER1 4 /* bin rasterisation, phase 1 */
+#define GBR_TIMING_HIZ 5     /* depth pyramid + occlusion test */
+#define GBR_TIMING_SETUP2 6
+#define GBR_TIMING_CLIP2 7
+#define GBR_TIMING_RASTER2 8
+#define GBR_TIMING_RESOLVE 9 /* shading, depth, semantic planes */
+#define GBR_TIMING_SLOTS 10
 
 /* new: per-frame counters of the last gbr_draw() (read after a sync) */
 typedef struct gbr_frame_stats_info {
@@ -119,6 +136,13 @@ typedef struct gbr_frame_stats_info {
   uint64_t bin_entries;       /* triangle-bin pairs written */
   uint64_t overflowed_bins;   /* bins that exceeded capacity (exact fallback) */
   uint64_t raster_items;      /* bins rasterised */
+  uint64_t items_phase1;      /* work items set up in phase 1 (all, without culling) */
+  uint64_t items_candidates;  /* work items tested against the depth pyramid */
+  uint64_t items_phase2;      /* candidates set up in phase 2 */
+  uint64_t triangles_setup;   /* triangles of the work items set up */
+  uint64_t vertices_setup;    /* vertex-range entries of the work items set up */
+  uint64_t triangles_phase2;  /* of which in phase 2 */
+  uint64_t vertices_phase2;
 } gbr_frame_stats_info;
 
 /* ---- parsed-file description consumed by gbr_add_file_data() (new) ---- */
@@ -292,6 +316,10 @@ gbr_status gbr_unproject_depth(const float* unprojection2, float* depth, int64_t
 gbr_status gbr_get_tuning(const gbr_renderer* r, gbr_tuning* out);
 gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* tuning);
 gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out);
+/* new: milliseconds per GBR_TIMING_* stage, averaged over the gbr_draw()
+   calls since the previous call (at most 4096; a stage that did not run
+   counts 0); needs gbr_tuning.timing = 1; synchronises the stream */
+gbr_status gbr_frame_timings(gbr_renderer* r, float* ms);
 
 #ifdef __cplusplus
 }

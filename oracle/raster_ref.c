@@ -36,7 +36,9 @@
  *     edge belongs to the triangle iff the edge is a left edge or a
  *     horizontal edge with the interior at larger y
  *   - window depth interpolated linearly in screen space from the snapped
- *     vertices, clamped to [0,1]
+ *     vertices, raised to the triangle's smallest vertex depth (a NaN or a
+ *     rounding overshoot in a sliver never lands in front of its triangle)
+ *     and clamped to [0,1]
  *   - attributes perspective-correct, evaluated from the unclipped
  *     triangle's homogeneous barycentrics at the pixel centre
  *   - texture LOD from coarse 2x2-quad derivatives of the same triangle's
@@ -561,6 +563,8 @@ static void raster_tri(env_ctx* C, const tri_attr* A, int draw_global,
   p = dz2 * fx1; q = dz1 * fx2;
   float dzdy = p - q;
   dzdy = dzdy / fa;
+  float zlo = Z[0] < Z[1] ? Z[0] : Z[1];
+  zlo = Z[2] < zlo ? Z[2] : zlo;
 
   for (int py = y0; py <= y1; ++py) {
     int64_t Py = (int64_t)py * 256 + 128;
@@ -580,6 +584,7 @@ static void raster_tri(env_ctx* C, const tri_attr* A, int draw_global,
       z = Z[0] + z;
       float zy = dzdy * oy;
       z = z + zy;
+      if (!(z >= zlo)) z = zlo;
       if (!(z > 0.0f)) z = 0.0f;
       if (z > 1.0f) z = 1.0f;
       if (z < C->zbuf[pix]) {
