@@ -122,8 +122,13 @@ enum : uint32_t {
 constexpr uint32_t kCtrStride = 32;
 
 // frame statistics (64-bit): bin entries, overflowed bins, then triangles and
-// vertex-range entries set up in phase 1, the same for phase 2
-constexpr uint32_t kStatCount = 6;
+// vertex-range entries set up in phase 1, the same for phase 2, then pixels
+// tested by the rasteriser and fragments inside their triangles
+constexpr uint32_t kStatCount = 8;
+// each statistic is striped over kStatStripes words (stripe = workgroup index
+// mod kStatStripes) so that workgroups rarely add to the same word; the host
+// sums the stripes
+constexpr uint32_t kStatStripes = 32;
 
 // Mesh-local bounding box of the vertices a work item references.
 struct alignas(16) ItemBounds {
@@ -217,13 +222,14 @@ struct FrameParams {
   int32_t hiz_w0, hiz_h0, hiz_w1, hiz_h1;  // pyramid level sizes in blocks
   uint32_t hiz_stride;
   float proposal_threshold;
+  int32_t raster_mode;  // 0: a lane per triangle, 1: a lane per (triangle, pixel)
   // outputs (env-major planes), any may be null
   uint32_t* out_color;
   float* out_depth;
   float* out_lindepth;
   uint32_t* out_semantic;
   // counters
-  unsigned long long* stats;  // kStatCount frame statistics
+  unsigned long long* stats;  // kStatCount x kStatStripes frame statistics
   int32_t bin_size;
   int32_t bin_shift;
   int32_t top_down;

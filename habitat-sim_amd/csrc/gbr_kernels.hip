@@ -191,6 +191,25 @@ __device__ __forceinline__ long long tri_area(const int32_t* X, const int32_t* Y
          (long long)(Y[1] - Y[0]) * (long long)(X[2] - X[0]);
 }
 
+// adds v to frame statistic `stat` (striped by workgroup)
+__device__ __forceinline__ void add_stat(unsigned long long* stats, uint32_t stat,
+                                         unsigned long long v) {
+  atomicAdd(&stats[stat * kStatStripes + (blockIdx.x & (kStatStripes - 1))], v);
+}
+
+// block-wide sum of one value per thread, returned to thread 0
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v,
+                                                        unsigned long long* s_red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBlock / 64; ++w) t += s_red[w];
+  return t;
+}
+
 // --------------------------------------------------------------------------
 // k_nodes: absolute node transforms
 // --------------------------------------------------------------------------
@@ -221,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   // first kernel of every frame: reset the frame's counters
   if (i < kCtrCount) counters[i * kCtrStride] = 0u;
-  if (i < kStatCount) stats[i] = 0ull;
+  for (uint32_t j = i; j < kStatCount * kStatStripes; j += gridDim.x * kBlock) stats[j] = 0ull;
   if (i >= n) return;
   int depth = 0;
   for (int32_t q = parents[i]; q >= 0 && depth < 1024; q = parents[q]) ++depth;
@@ -538,12 +557,15 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
     }
     __syncthreads();  // LDS is reused by the next item
   }
-  for (int o = 32; o > 0; o >>= 1) written += __shfl_xor(written, o);
-  if (lane == 0 && written) atomicAdd(&p.stats[0], (unsigned long long)written);
-  if (threadIdx.x == 0 && tri_acc) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  const unsigned long long wsum = block_sum(written, s_red);
+  if (threadIdx.x == 0) {
     const uint32_t st = queue_ctr == kCtrQueue2 ? 4u : 2u;
-    atomicAdd(&p.stats[st], tri_acc);
-    atomicAdd(&p.stats[st + 1], vtx_acc);
+    if (wsum) add_stat(p.stats, 0, wsum);
+    if (tri_acc) {
+      add_stat(p.stats, st, tri_acc);
+      add_stat(p.stats, st + 1, vtx_acc);
+    }
   }
 }
 
@@ -564,9 +586,32 @@ struct RasterCtx {
   int* nbig;
   PixRect bin;
   int B;
+  uint32_t* pairs;      // per-lane counters: pixels tested, fragments covered
+  uint32_t* frags;
 };
 
-// rasterise one front-facing triangle with a known depth plane
+__device__ __forceinline__ bool small_box(const int32_t* X, const int32_t* Y) {
+  const int32_t ex = max(X[0], max(X[1], X[2])) - min(X[0], min(X[1], X[2]));
+  const int32_t ey = max(Y[0], max(Y[1], Y[2])) - min(Y[0], min(Y[1], Y[2]));
+  // edge functions over the box, and one step past it, stay below 2^30
+  return ex < 16384 && ey < 16384;
+}
+
+__device__ __forceinline__ void depth_key_min(const RasterCtx& rc, float z0, float dzdx, float dzdy,
+                                              float zlo, const int32_t* X, const int32_t* Y,
+                                              uint32_t prim, int px, int py) {
+  const float z = depth_at(z0, dzdx, dzdy, zlo, X[0], Y[0], (long long)px * 256 + 128,
+                           (long long)py * 256 + 128);
+  if (z < 1.0f) {
+    const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | prim;
+    atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
+  }
+}
+
+// rasterise one front-facing triangle with a known depth plane: this lane
+// walks the clipped rectangle row by row with edge functions stepped by
+// additions; rectangles above kSmallRaster pixels go to the block's
+// cooperative queue
 __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32_t* Y, float z0,
                               float dzdx, float dzdy, float zlo, uint32_t prim) {
   PixRect r = tri_pixels(X, Y, rc.bin);
@@ -584,6 +629,33 @@ __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32
       return;
     }
   }
+  *rc.pairs += (uint32_t)npx;
+  if (small_box(X, Y)) {
+    int32_t dx[3], dy[3], bias[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int j = (i + 1) % 3;
+      dx[i] = X[j] - X[i];
+      dy[i] = Y[j] - Y[i];
+      bias[i] = (dy[i] < 0 || (dy[i] == 0 && dx[i] > 0)) ? 0 : -1;
+    }
+    const int32_t Px0 = r.x0 * 256 + 128;
+    for (int py = r.y0; py <= r.y1; ++py) {
+      const int32_t Py = py * 256 + 128;
+      int32_t e[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) e[i] = dx[i] * (Py - Y[i]) - dy[i] * (Px0 - X[i]) + bias[i];
+      for (int px = r.x0; px <= r.x1; ++px) {
+        if ((e[0] | e[1] | e[2]) >= 0) {
+          ++*rc.frags;
+          depth_key_min(rc, z0, dzdx, dzdy, zlo, X, Y, prim, px, py);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) e[i] -= dy[i] * 256;
+      }
+    }
+    return;
+  }
   TriSetup s;
   setup_edges_only(X, Y, s);
   for (int py = r.y0; py <= r.y1; ++py) {
@@ -591,11 +663,8 @@ __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32
     for (int px = r.x0; px <= r.x1; ++px) {
       const long long Px = (long long)px * 256 + 128;
       if (!covers(s, X, Y, Px, Py)) continue;
-      const float z = depth_at(z0, dzdx, dzdy, zlo, X[0], Y[0], Px, Py);
-      if (z < 1.0f) {
-        const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | prim;
-        atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
-      }
+      ++*rc.frags;
+      depth_key_min(rc, z0, dzdx, dzdy, zlo, X, Y, prim, px, py);
     }
   }
 }
@@ -611,19 +680,38 @@ __device__ __forceinline__ void raster_tri(const RasterCtx& rc, const int32_t* X
 }
 
 __device__ void raster_big(const RasterCtx& rc, const BigTri& b) {
-  TriSetup s;
-  setup_edges_only(b.X, b.Y, s);
   const int w = b.r.x1 - b.r.x0 + 1;
   const int n = w * (b.r.y1 - b.r.y0 + 1);
+  *rc.pairs += (uint32_t)((n - (int)threadIdx.x + kBlock - 1) / kBlock);
+  if (small_box(b.X, b.Y)) {
+    int32_t dx[3], dy[3], bias[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int j = (i + 1) % 3;
+      dx[i] = b.X[j] - b.X[i];
+      dy[i] = b.Y[j] - b.Y[i];
+      bias[i] = (dy[i] < 0 || (dy[i] == 0 && dx[i] > 0)) ? 0 : -1;
+    }
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+      const int px = b.r.x0 + k % w, py = b.r.y0 + k / w;
+      const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
+      int32_t e[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) e[i] = dx[i] * (Py - b.Y[i]) - dy[i] * (Px - b.X[i]) + bias[i];
+      if ((e[0] | e[1] | e[2]) < 0) continue;
+      ++*rc.frags;
+      depth_key_min(rc, b.z0, b.dzdx, b.dzdy, b.zlo, b.X, b.Y, b.prim, px, py);
+    }
+    return;
+  }
+  TriSetup s;
+  setup_edges_only(b.X, b.Y, s);
   for (int k = threadIdx.x; k < n; k += kBlock) {
     const int px = b.r.x0 + k % w, py = b.r.y0 + k / w;
     const long long Px = (long long)px * 256 + 128, Py = (long long)py * 256 + 128;
     if (!covers(s, b.X, b.Y, Px, Py)) continue;
-    const float z = depth_at(b.z0, b.dzdx, b.dzdy, b.zlo, b.X[0], b.Y[0], Px, Py);
-    if (z < 1.0f) {
-      const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | b.prim;
-      atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
-    }
+    ++*rc.frags;
+    depth_key_min(rc, b.z0, b.dzdx, b.dzdy, b.zlo, b.X, b.Y, b.prim, px, py);
   }
 }
 
@@ -727,6 +815,7 @@ __device__ __forceinline__ int wrap_coord(int i, int size, int mode) {
     if (m < 0) m += pp;
     return m < size ? m : pp - 1 - m;
   }
+  if ((size & (size - 1)) == 0) return i & (size - 1);  // non-negative i mod size
   int m = i % size;
   if (m < 0) m += size;
   return m;
@@ -738,18 +827,19 @@ __device__ __forceinline__ int floor_to_int(float u) {
   return (int)floorf(u);
 }
 
-__device__ __forceinline__ void fetch(const FrameParams& p, const TextureDesc& t, int level,
-                                      int layer, int i, int j, float* out) {
+__device__ __forceinline__ void fetch(const FrameParams& p, const float* lut, const TextureDesc& t,
+                                      int level, int layer, int i, int j, float* out) {
   int w = max(t.width >> level, 1), h = max(t.height >> level, 1);
   const uint32_t texel = *reinterpret_cast<const uint32_t*>(
       p.texels + t.level_offset[level] + (((long long)layer * h + j) * w + i) * 4);
-  out[0] = p.unorm_lut[texel & 0xff];
-  out[1] = p.unorm_lut[(texel >> 8) & 0xff];
-  out[2] = p.unorm_lut[(texel >> 16) & 0xff];
-  out[3] = p.unorm_lut[texel >> 24];
+  out[0] = lut[texel & 0xff];
+  out[1] = lut[(texel >> 8) & 0xff];
+  out[2] = lut[(texel >> 16) & 0xff];
+  out[3] = lut[texel >> 24];
 }
 
-__device__ void sample_level(const FrameParams& p, const TextureDesc& t, int level, int layer,
+__device__ void sample_level(const FrameParams& p, const float* lut, const TextureDesc& t, int level,
+                             int layer,
                              float s, float tc, bool linear, float* out) {
   int w = max(t.width >> level, 1), h = max(t.height >> level, 1);
   float u = s * (float)w;
@@ -757,7 +847,7 @@ __device__ void sample_level(const FrameParams& p, const TextureDesc& t, int lev
   if (!linear) {
     int i = wrap_coord(floor_to_int(u), w, t.wrap_s);
     int j = wrap_coord(floor_to_int(v), h, t.wrap_t);
-    fetch(p, t, level, layer, i, j, out);
+    fetch(p, lut, t, level, layer, i, j, out);
     return;
   }
   float uu = u - 0.5f;
@@ -771,10 +861,10 @@ __device__ void sample_level(const FrameParams& p, const TextureDesc& t, int lev
   i0 = wrap_coord(i0, w, t.wrap_s);
   j0 = wrap_coord(j0, h, t.wrap_t);
   float t00[4], t10[4], t01[4], t11[4];
-  fetch(p, t, level, layer, i0, j0, t00);
-  fetch(p, t, level, layer, i1, j0, t10);
-  fetch(p, t, level, layer, i0, j1, t01);
-  fetch(p, t, level, layer, i1, j1, t11);
+  fetch(p, lut, t, level, layer, i0, j0, t00);
+  fetch(p, lut, t, level, layer, i1, j0, t10);
+  fetch(p, lut, t, level, layer, i0, j1, t01);
+  fetch(p, lut, t, level, layer, i1, j1, t11);
   float ia = 1.0f - a;
   float ib = 1.0f - b;
   float w00 = ia * ib, w10 = a * ib, w01 = ia * b, w11 = a * b;
@@ -791,7 +881,8 @@ __device__ void sample_level(const FrameParams& p, const TextureDesc& t, int lev
   }
 }
 
-__device__ void sample_tex(const FrameParams& p, const TextureDesc& t, int layer, float s,
+__device__ void sample_tex(const FrameParams& p, const float* lut, const TextureDesc& t, int layer,
+                           float s,
                            float tc, float lambda, float* out) {
   if (layer < 0) layer = 0;
   if (layer >= t.layers) layer = t.layers - 1;
@@ -800,14 +891,14 @@ __device__ void sample_tex(const FrameParams& p, const TextureDesc& t, int layer
       (t.min_filter == GL_NEAREST_MIPMAP_NEAREST || t.min_filter == GL_NEAREST_MIPMAP_LINEAR))
     c = 0.5f;
   if (lambda <= c) {
-    sample_level(p, t, 0, layer, s, tc, t.mag_filter == GL_LINEAR, out);
+    sample_level(p, lut, t, 0, layer, s, tc, t.mag_filter == GL_LINEAR, out);
     return;
   }
   const int q = t.levels - 1;
   switch (t.min_filter) {
     case GL_NEAREST:
     case GL_LINEAR:
-      sample_level(p, t, 0, layer, s, tc, t.min_filter == GL_LINEAR, out);
+      sample_level(p, lut, t, 0, layer, s, tc, t.min_filter == GL_LINEAR, out);
       return;
     case GL_NEAREST_MIPMAP_NEAREST:
     case GL_LINEAR_MIPMAP_NEAREST: {
@@ -818,21 +909,21 @@ __device__ void sample_tex(const FrameParams& p, const TextureDesc& t, int layer
         float f = ceilf(lambda + 0.5f) - 1.0f;
         dlev = f > (float)q ? q : (int)f;
       }
-      sample_level(p, t, dlev, layer, s, tc, t.min_filter == GL_LINEAR_MIPMAP_NEAREST, out);
+      sample_level(p, lut, t, dlev, layer, s, tc, t.min_filter == GL_LINEAR_MIPMAP_NEAREST, out);
       return;
     }
     default: {
       bool lin = t.min_filter == GL_LINEAR_MIPMAP_LINEAR;
       if (lambda >= (float)q) {
-        sample_level(p, t, q, layer, s, tc, lin, out);
+        sample_level(p, lut, t, q, layer, s, tc, lin, out);
         return;
       }
       float fl = floorf(lambda);
       int d1 = (int)fl;
       float f = lambda - fl;
       float t1[4], t2[4];
-      sample_level(p, t, d1, layer, s, tc, lin, t1);
-      sample_level(p, t, d1 + 1, layer, s, tc, lin, t2);
+      sample_level(p, lut, t, d1, layer, s, tc, lin, t1);
+      sample_level(p, lut, t, d1 + 1, layer, s, tc, lin, t2);
       float g = 1.0f - f;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -907,38 +998,43 @@ __device__ __forceinline__ uint32_t to_unorm8(float c) {
   return (uint32_t)(int)v;
 }
 
-// colour of the visible primitive at pixel (px, py), from its resolve record
-__device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const PrimRec& rs,
+// colour of the visible primitive at a pixel.  A holds the triangle's rows,
+// b the pixel's barycentrics, (s, t) its texture coordinates; q00/q10/q01
+// the texture coordinates at the centres of its 2x2 quad (derivatives for
+// the level of detail) where a quad neighbour already computed them for the
+// same primitive (has_*), else they are evaluated here.
+struct QuadST {
+  float s00, t00, s10, t10, s01, t01;
+  bool h00, h10, h01;
+};
+
+__device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, const SceneDesc& sc,
+                               const TriAttr& A, const float* b, float s, float t,
+                               const MaterialDesc& mat, bool textured, const QuadST& q,
                                const DrawDesc& d, uint32_t prim, int px, int py) {
-  const MaterialDesc mat = p.materials[d.material];
-  TriAttr A;
-  A.cx[0] = rs.cx0; A.cy[0] = rs.cy0; A.cw[0] = rs.cw0;
-  A.cx[1] = rs.cx1; A.cy[1] = rs.cy1; A.cw[1] = rs.cw1;
-  A.cx[2] = rs.cx2; A.cy[2] = rs.cy2; A.cw[2] = rs.cw2;
-  A.s[0] = rs.s0; A.t[0] = rs.t0;
-  A.s[1] = rs.s1; A.t[1] = rs.t1;
-  A.s[2] = rs.s2; A.t[2] = rs.t2;
-  bary_rows(A);
   const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
   const float ihw = 1.0f / hw, ihh = 1.0f / hh;
-  float b[3];
-  bary_at(A, hw, hh, ihw, ihh, (float)px + 0.5f, (float)py + 0.5f, b);
   float c[4] = {mat.ambient[0], mat.ambient[1], mat.ambient[2], mat.ambient[3]};
-  if (!p.no_textures && mat.texture > 0) {
+  if (textured) {
     const TextureDesc& td = p.textures[mat.texture];
-    const float s = interp3(b, A.s[0], A.s[1], A.s[2]);
-    const float t = interp3(b, A.t[0], A.t[1], A.t[2]);
     const int qx = px & ~1, qy = py & ~1;
-    float b00[3], b10[3], b01[3];
-    bary_at(A, hw, hh, ihw, ihh, (float)qx + 0.5f, (float)qy + 0.5f, b00);
-    bary_at(A, hw, hh, ihw, ihh, (float)qx + 1.5f, (float)qy + 0.5f, b10);
-    bary_at(A, hw, hh, ihw, ihh, (float)qx + 0.5f, (float)qy + 1.5f, b01);
-    const float s00 = interp3(b00, A.s[0], A.s[1], A.s[2]);
-    const float t00 = interp3(b00, A.t[0], A.t[1], A.t[2]);
-    const float s10 = interp3(b10, A.s[0], A.s[1], A.s[2]);
-    const float t10 = interp3(b10, A.t[0], A.t[1], A.t[2]);
-    const float s01 = interp3(b01, A.s[0], A.s[1], A.s[2]);
-    const float t01 = interp3(b01, A.t[0], A.t[1], A.t[2]);
+    float s00 = q.s00, t00 = q.t00, s10 = q.s10, t10 = q.t10, s01 = q.s01, t01 = q.t01;
+    float bq[3];
+    if (!q.h00) {
+      bary_at(A, hw, hh, ihw, ihh, (float)qx + 0.5f, (float)qy + 0.5f, bq);
+      s00 = interp3(bq, A.s[0], A.s[1], A.s[2]);
+      t00 = interp3(bq, A.t[0], A.t[1], A.t[2]);
+    }
+    if (!q.h10) {
+      bary_at(A, hw, hh, ihw, ihh, (float)qx + 1.5f, (float)qy + 0.5f, bq);
+      s10 = interp3(bq, A.s[0], A.s[1], A.s[2]);
+      t10 = interp3(bq, A.t[0], A.t[1], A.t[2]);
+    }
+    if (!q.h01) {
+      bary_at(A, hw, hh, ihw, ihh, (float)qx + 0.5f, (float)qy + 1.5f, bq);
+      s01 = interp3(bq, A.s[0], A.s[1], A.s[2]);
+      t01 = interp3(bq, A.t[0], A.t[1], A.t[2]);
+    }
     const float fw = (float)td.width, fh = (float)td.height;
     float dux = s10 - s00; dux = dux * fw;
     float dvx = t10 - t00; dvx = dvx * fh;
@@ -950,7 +1046,7 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const
     float lambda = gbr_log2(r2);
     lambda = lambda * 0.5f;
     float tc[4];
-    sample_tex(p, td, (int)mat.layer, s, t, lambda, tc);
+    sample_tex(p, lut, td, (int)mat.layer, s, t, lambda, tc);
 #pragma unroll
     for (int k = 0; k < 4; ++k) c[k] = c[k] * tc[k];
   }
@@ -963,10 +1059,10 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const SceneDesc& sc, const
       const uint32_t v = ip[k];
       if (cf == kColorU8) {
         const uint32_t u8 = p.col8[v];
-        col[k][0] = p.unorm_lut[u8 & 0xff];
-        col[k][1] = p.unorm_lut[(u8 >> 8) & 0xff];
-        col[k][2] = p.unorm_lut[(u8 >> 16) & 0xff];
-        col[k][3] = p.unorm_lut[u8 >> 24];
+        col[k][0] = lut[u8 & 0xff];
+        col[k][1] = lut[(u8 >> 8) & 0xff];
+        col[k][2] = lut[(u8 >> 16) & 0xff];
+        col[k][3] = lut[u8 >> 24];
       } else {
         const float4 f = p.col32[v];
         col[k][0] = f.x; col[k][1] = f.y; col[k][2] = f.z; col[k][3] = f.w;
@@ -1089,7 +1185,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p, int phase)
 }
 
 // --------------------------------------------------------------------------
-// k_raster: per (scene, bin), binned records -> LDS visibility -> vis keys
+// Bin helpers shared by the raster and resolve kernels
 // --------------------------------------------------------------------------
 
 struct BinInfo {
@@ -1121,11 +1217,82 @@ __device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b
   }
 }
 
-// phase 1 starts every bin from the clear key; phase 2 merges into the
-// phase-1 keys and leaves bins without new entries untouched
-__global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
-                                                   const RasterItem* __restrict__ items,
-                                                   int phase) {
+// One binned triangle staged in LDS for pixel-parallel rasterisation.
+struct PairTri {
+  int32_t X[3], Y[3];
+  float z0, dzdx, dzdy, zlo;
+  uint32_t prim;
+  int16_t x0, y0;   // clipped pixel rectangle: origin
+  uint16_t w;       //   and width
+  uint16_t small;   // 1: bounding box under 128 px, edge functions fit in 32 bits
+  uint32_t inv_w;   // ceil(65536 / w): row of a rectangle index without a divide
+};
+
+// coverage (top-left rule) and depth of one (triangle, pixel) pair
+__device__ __forceinline__ bool raster_pair(const PairTri& t, unsigned long long* vis,
+                                            const PixRect& bin, int B, int px, int py) {
+  bool in = true;
+  if (t.small) {
+    const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int j = (i + 1) % 3;
+      const int32_t dx = t.X[j] - t.X[i], dy = t.Y[j] - t.Y[i];
+      const int32_t bias = (dy < 0 || (dy == 0 && dx > 0)) ? 0 : -1;
+      in = in && (dx * (Py - t.Y[i]) - dy * (Px - t.X[i]) + bias >= 0);
+    }
+  } else {
+    const long long Px = (long long)px * 256 + 128, Py = (long long)py * 256 + 128;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int j = (i + 1) % 3;
+      const long long dx = (long long)t.X[j] - t.X[i], dy = (long long)t.Y[j] - t.Y[i];
+      const long long bias = (dy < 0 || (dy == 0 && dx > 0)) ? 0 : -1;
+      in = in && (dx * (Py - t.Y[i]) - dy * (Px - t.X[i]) + bias >= 0);
+    }
+  }
+  if (!in) return false;
+  const float z = depth_at(t.z0, t.dzdx, t.dzdy, t.zlo, t.X[0], t.Y[0], (long long)px * 256 + 128,
+                           (long long)py * 256 + 128);
+  if (z < 1.0f) {
+    const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | t.prim;
+    atomicMin(&vis[(py - bin.y0) * B + (px - bin.x0)], key);
+  }
+  return true;
+}
+
+// LDS written by one lane and read by another lane of the same wave: LDS
+// instructions of a wave execute in order, so only the compiler must not
+// move accesses across this point
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// block-wide sums of the per-lane raster counters -> frame statistics
+// (every thread of the block calls this)
+__device__ __forceinline__ void add_raster_stats(const FrameParams& p, uint32_t pairs,
+                                                 uint32_t frags) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  const unsigned long long a = block_sum(pairs, s_red);
+  const unsigned long long b = block_sum(frags, s_red);
+  if (threadIdx.x == 0 && a) {
+    add_stat(p.stats, 6, a);
+    add_stat(p.stats, 7, b);
+  }
+}
+
+// --------------------------------------------------------------------------
+// k_raster_lane: per (scene, bin), one binned triangle per lane walking its
+// clipped rectangle; rectangles above kSmallRaster pixels are rasterised by
+// the whole block from an LDS queue.  Phase 1 starts every bin from the
+// clear key; phase 2 merges into the phase-1 keys and leaves bins without
+// new entries untouched.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
+                                                        const RasterItem* __restrict__ items,
+                                                        int phase) {
   __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
   __shared__ BigTri s_big[kBigQueue];
   __shared__ int s_nbig;
@@ -1135,12 +1302,15 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
   if (n > b.sc.bin_cap) return;  // overflowed: k_raster_full rescans this bin
   if (phase == 2 && n == 0) return;
   const int B = p.bin_size;
+  uint32_t pairs = 0, frags = 0;
   RasterCtx rc;
   rc.vis = s_vis;
   rc.big = s_big;
   rc.nbig = &s_nbig;
   rc.B = B;
   rc.bin = b.bin;
+  rc.pairs = &pairs;
+  rc.frags = &frags;
   if (phase == 2) {
     const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
     for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
@@ -1153,7 +1323,8 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
   }
   if (threadIdx.x == 0) s_nbig = 0;
   __syncthreads();
-  const uint32_t* entries = p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
+  const uint32_t* entries =
+      p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
   for (uint32_t base = 0; base < n; base += kBlock) {
     const uint32_t k = base + threadIdx.x;
     if (k < n) {
@@ -1173,6 +1344,117 @@ __global__ __launch_bounds__(kBlock) void k_raster(FrameParams p,
   }
   store_vis(p, b, s_vis, B);
   if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
+  add_raster_stats(p, pairs, frags);
+}
+
+// --------------------------------------------------------------------------
+// k_raster_pairs: per (scene, bin), binned triangles -> LDS visibility tile ->
+// visibility keys.  Waves claim 64 entries at a time, prefix-sum the pixel
+// counts of their clipped rectangles and spread the (triangle, pixel) pairs
+// over the lanes, so small and large triangles keep every lane busy.
+// Phase 1 starts every bin from the clear key; phase 2 merges into the
+// phase-1 keys and leaves bins without new entries untouched.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
+                                                         const RasterItem* __restrict__ items,
+                                                         int phase) {
+  __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
+  __shared__ PairTri s_tri[kBlock];
+  __shared__ uint32_t s_pref[kBlock];
+  __shared__ uint32_t s_next;
+
+  const BinInfo b = bin_info(p, items[blockIdx.x]);
+  const uint32_t n = p.bin_count[b.gbin];
+  if (n > b.sc.bin_cap) return;  // overflowed: k_raster_full rescans this bin
+  if (phase == 2 && n == 0) return;
+  const int B = p.bin_size;
+  if (phase == 2) {
+    const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
+    for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
+      const int lx = l % bw, ly = l / bw;
+      s_vis[ly * B + lx] =
+          p.vis[(size_t)b.sc.plane_offset + (size_t)(b.bin.y0 + ly) * b.sc.width + b.bin.x0 + lx];
+    }
+  } else {
+    for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
+  }
+  if (threadIdx.x == 0) s_next = 0u;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int wbase = threadIdx.x & ~63;
+  uint32_t pairs = 0, frags = 0;
+  const uint32_t* entries =
+      p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
+  while (true) {
+    uint32_t chunk = 0;
+    if (lane == 0) chunk = atomicAdd(&s_next, 64u);
+    chunk = __shfl(chunk, 0);
+    if (chunk >= n) break;
+    const uint32_t k = chunk + (uint32_t)lane;
+    uint32_t npx = 0;
+    if (k < n) {
+      const uint32_t e = entries[k];
+      const RasterRec rr = (e & kSubTri) ? p.sub_recs[e & ~kSubTri]
+                                         : p.recs[(size_t)b.sc.rec_base + e].r;
+      const int32_t X[3] = {rr.X0, rr.X1, rr.X2};
+      const int32_t Y[3] = {rr.Y0, rr.Y1, rr.Y2};
+      const PixRect r = tri_pixels(X, Y, b.bin);
+      if (r.x0 <= r.x1 && r.y0 <= r.y1) {
+        const int w = r.x1 - r.x0 + 1;
+        npx = (uint32_t)(w * (r.y1 - r.y0 + 1));
+        PairTri& t = s_tri[threadIdx.x];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          t.X[q] = X[q];
+          t.Y[q] = Y[q];
+        }
+        t.z0 = rr.z0;
+        t.dzdx = rr.dzdx;
+        t.dzdy = rr.dzdy;
+        t.zlo = rr.zlo;
+        t.prim = rr.prim;
+        t.x0 = (int16_t)r.x0;
+        t.y0 = (int16_t)r.y0;
+        t.w = (uint16_t)w;
+        const int32_t ex = max(X[0], max(X[1], X[2])) - min(X[0], min(X[1], X[2]));
+        const int32_t ey = max(Y[0], max(Y[1], Y[2])) - min(Y[0], min(Y[1], Y[2]));
+        t.small = (ex < 32768 && ey < 32768) ? 1 : 0;
+        t.inv_w = (65536u + (uint32_t)w - 1u) / (uint32_t)w;
+      }
+    }
+    uint32_t incl = npx;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    s_pref[threadIdx.x] = incl;
+    const uint32_t total = __shfl(incl, 63);
+    wave_lds_sync();
+    for (uint32_t base = 0; base < total; base += 64) {
+      const uint32_t g = base + (uint32_t)lane;
+      if (g < total) {
+        int lo = 0, hi = 63;  // first triangle whose inclusive prefix exceeds g
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_pref[wbase + mid] > g) hi = mid;
+          else lo = mid + 1;
+        }
+        const PairTri& t = s_tri[wbase + lo];
+        const uint32_t local = g - (lo ? s_pref[wbase + lo - 1] : 0u);
+        const uint32_t ly = (local * t.inv_w) >> 16;
+        const uint32_t lx = local - ly * t.w;
+        frags += raster_pair(t, s_vis, b.bin, B, t.x0 + (int)lx, t.y0 + (int)ly) ? 1u : 0u;
+      }
+    }
+    pairs += lane == 0 ? total : 0u;
+    wave_lds_sync();  // this wave's staging is rewritten by its next chunk
+  }
+  __syncthreads();
+  store_vis(p, b, s_vis, B);
+  if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
+  add_raster_stats(p, pairs, frags);
 }
 
 // exact fallback for bins whose list overflowed: every primitive of the scene
@@ -1187,12 +1469,15 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
   const BinInfo b = bin_info(p, it);
   if (p.bin_count[b.gbin] <= b.sc.bin_cap) return;
   const int B = p.bin_size;
+  uint32_t pairs = 0, frags = 0;
   RasterCtx rc;
   rc.vis = s_vis;
   rc.big = s_big;
   rc.nbig = &s_nbig;
   rc.B = B;
   rc.bin = b.bin;
+  rc.pairs = &pairs;
+  rc.frags = &frags;
   for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
   if (b.sc.draw_count <= (uint32_t)kDrawCache)
     for (uint32_t i = threadIdx.x; i < b.sc.draw_count; i += kBlock)
@@ -1213,9 +1498,10 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
   }
   store_vis(p, b, s_vis, B);
   if (threadIdx.x == 0) {
-    atomicAdd(&p.stats[1], 1ull);
+    add_stat(p.stats, 1, 1ull);
     p.bin_count[b.gbin] = 0;
   }
+  add_raster_stats(p, pairs, frags);
 }
 
 // --------------------------------------------------------------------------
@@ -1225,31 +1511,82 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
 
 __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
                                                     const RasterItem* __restrict__ items) {
+  __shared__ float s_lut[256];
+  static_assert(kBlock >= 256, "one LUT entry per thread");
+  if (threadIdx.x < 256) s_lut[threadIdx.x] = p.unorm_lut[threadIdx.x];
+  __syncthreads();
   const RasterItem it = items[blockIdx.x];
   const BinInfo b = bin_info(p, it);
   const SceneDesc& sc = b.sc;
   const float un0 = p.unprojections[2 * it.scene], un1 = p.unprojections[2 * it.scene + 1];
   const float far_depth = un1 / (1.0f + un0);
   const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
-  for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
+  const int npx = bw * bh;
+  // in a full bin the pixels of a 2x2 quad sit in lanes {l, l^1, l^bw, l^1^bw}
+  // of one wave (rows of bw pixels, bw in {8, 16, 32}, even bin origin)
+  const bool quads = bw == p.bin_size && bh == p.bin_size;
+  const int lane = threadIdx.x & 63;
+  const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
+  const float ihw = 1.0f / hw, ihh = 1.0f / hh;
+  for (int base = 0; base < npx; base += kBlock) {
+    const int l = base + (int)threadIdx.x;
+    const bool valid = l < npx;
     const int px = b.bin.x0 + l % bw, py = b.bin.y0 + l / bw;
     const unsigned long long key =
-        p.vis[(size_t)sc.plane_offset + (size_t)py * sc.width + px];
+        valid ? p.vis[(size_t)sc.plane_offset + (size_t)py * sc.width + px] : kClearKey;
+    const bool drawn = key != kClearKey;
+    const uint32_t prim = (uint32_t)key;
+    PrimRec rs;
+    DrawDesc d;
+    MaterialDesc mat;
+    TriAttr A;
+    float bc[3] = {1.0f, 0.0f, 0.0f};
+    float s = 0.0f, t = 0.0f;
+    bool textured = false;
+    if (drawn) {
+      rs = p.recs[(size_t)sc.rec_base + prim];
+      d = p.draws[rs.r.draw];
+      mat = p.materials[d.material];
+      if (p.item_vis && rs.item < p.n_items) p.item_vis[rs.item] = p.frame;  // feedback for the next frame
+      A.cx[0] = rs.cx0; A.cy[0] = rs.cy0; A.cw[0] = rs.cw0;
+      A.cx[1] = rs.cx1; A.cy[1] = rs.cy1; A.cw[1] = rs.cw1;
+      A.cx[2] = rs.cx2; A.cy[2] = rs.cy2; A.cw[2] = rs.cw2;
+      A.s[0] = rs.s0; A.t[0] = rs.t0;
+      A.s[1] = rs.s1; A.t[1] = rs.t1;
+      A.s[2] = rs.s2; A.t[2] = rs.t2;
+      bary_rows(A);
+      bary_at(A, hw, hh, ihw, ihh, (float)px + 0.5f, (float)py + 0.5f, bc);
+      textured = !p.no_textures && mat.texture > 0;
+      if (textured) {
+        s = interp3(bc, A.s[0], A.s[1], A.s[2]);
+        t = interp3(bc, A.t[0], A.t[1], A.t[2]);
+      }
+    }
+    QuadST q = {};
+    if (quads) {
+      const uint32_t tag = textured ? prim : 0xffffffffu;
+      const int l00 = lane & ~1 & ~bw, l10 = (lane | 1) & ~bw, l01 = (lane & ~1) | bw;
+      const uint32_t g00 = __shfl(tag, l00), g10 = __shfl(tag, l10), g01 = __shfl(tag, l01);
+      q.s00 = __shfl(s, l00); q.t00 = __shfl(t, l00);
+      q.s10 = __shfl(s, l10); q.t10 = __shfl(t, l10);
+      q.s01 = __shfl(s, l01); q.t01 = __shfl(t, l01);
+      q.h00 = textured && g00 == prim;
+      q.h10 = textured && g10 == prim;
+      q.h01 = textured && g01 == prim;
+    }
+    if (!valid) continue;
     const int row = p.top_down ? (sc.height - 1 - py) : py;
     const size_t o = (size_t)sc.plane_offset + (size_t)row * sc.width + px;
-    if (key == kClearKey) {
+    if (!drawn) {
       if (p.out_color) p.out_color[o] = kClearColor;
       if (p.out_depth) p.out_depth[o] = 1.0f;
       if (p.out_lindepth) p.out_lindepth[o] = 0.0f;
       if (p.out_semantic) p.out_semantic[o] = 0u;
       continue;
     }
-    const uint32_t prim = (uint32_t)key;
     const float z = __uint_as_float((uint32_t)(key >> 32));
-    const PrimRec rs = p.recs[(size_t)sc.rec_base + prim];
-    const DrawDesc d = p.draws[rs.r.draw];
-    if (p.item_vis && rs.item < p.n_items) p.item_vis[rs.item] = p.frame;  // feedback for the next frame
-    if (p.out_color) p.out_color[o] = shade_pixel(p, sc, rs, d, prim, px, py);
+    if (p.out_color)
+      p.out_color[o] = shade_pixel(p, s_lut, sc, A, bc, s, t, mat, textured, q, d, prim, px, py);
     if (p.out_depth) p.out_depth[o] = z;
     if (p.out_lindepth) {
       const float lin = un1 / (z + un0);
@@ -1518,7 +1855,10 @@ hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_raster, dim3(n), dim3(kBlock), 0, s, p, items, phase);
+  if (p.raster_mode == 1)
+    hipLaunchKernelGGL(k_raster_pairs, dim3(n), dim3(kBlock), 0, s, p, items, phase);
+  else
+    hipLaunchKernelGGL(k_raster_lane, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_raster_full, dim3(n), dim3(kBlock), 0, s, p, items);

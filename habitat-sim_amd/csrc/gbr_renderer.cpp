@@ -615,7 +615,7 @@ gbr_status do_draw(gbr_renderer* r) {
   if (r->structure_dirty) TRY(rebuild_structure(r));
   else TRY(upload_dirty_transforms(r));
   TRY(upload_cameras(r));
-  TRY(r->d_stats.reserve(kStatCount));
+  TRY(r->d_stats.reserve(kStatCount * kStatStripes));
   TRY(r->d_counters.reserve(kCtrCount * kCtrStride));
   TRY(r->d_vis.reserve((size_t)r->scene_count() * r->tile_pixels()));
   TRY(ensure_timing_events(r));
@@ -661,6 +661,7 @@ gbr_status do_draw(gbr_renderer* r) {
   p.frame = ++r->frame_no;
   p.proposal = nullptr;
   p.proposal_threshold = 0.5f;
+  p.raster_mode = r->tun.raster_mode;
   {
     const int W = r->cfg.tile_size[0], H = r->cfg.tile_size[1];
     p.hiz_w0 = (W + 7) >> 3;
@@ -841,7 +842,7 @@ gbr_status gbr_tuning_init(gbr_tuning* t) {
   t->bin_capacity_factor = 2.0f;
   t->min_bin_capacity = 256;
   t->cull_mode = GBR_CULL_NONE;
-  t->queue_grain = 2;
+  t->queue_grain = 4;
   return GBR_OK;
 }
 
@@ -852,6 +853,8 @@ static gbr_status validate_tuning(const gbr_tuning* t) {
     return fail(GBR_ERR_ARGUMENT, "tuning: chunk_triangles must be in [1, 1024]");
   if (!(t->bin_capacity_factor > 0.0f) || t->min_bin_capacity < 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: capacity must be positive");
+  if (t->raster_mode != 0 && t->raster_mode != 1)
+    return fail(GBR_ERR_ARGUMENT, "tuning: raster_mode must be 0 or 1");
   if (t->timing != 0 && t->timing != 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: timing must be 0 or 1");
   if (t->queue_grain < 1 || t->queue_grain > 64)
@@ -1453,16 +1456,22 @@ gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
   out->triangles = r->n_tris;
   out->raster_items = r->n_ritems;
   if (r->d_stats.ptr) {
+    unsigned long long striped[kStatCount * kStatStripes] = {};
     unsigned long long s[kStatCount] = {};
     TRY(set_device(r));
-    HIP_TRY(hipMemcpyAsync(s, r->d_stats.ptr, sizeof s, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipMemcpyAsync(striped, r->d_stats.ptr, sizeof striped, hipMemcpyDeviceToHost,
+                           r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
+    for (uint32_t k = 0; k < kStatCount; ++k)
+      for (uint32_t j = 0; j < kStatStripes; ++j) s[k] += striped[k * kStatStripes + j];
     out->bin_entries = s[0];
     out->overflowed_bins = s[1];
     out->triangles_setup = s[2] + s[4];
     out->vertices_setup = s[3] + s[5];
     out->triangles_phase2 = s[4];
     out->vertices_phase2 = s[5];
+    out->raster_pixels = s[6];
+    out->fragments = s[7];
   }
   if (r->d_counters.ptr && r->tun.cull_mode != GBR_CULL_NONE) {
     std::vector<uint32_t> c(kCtrCount * kCtrStride);

@@ -49,7 +49,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("bin_size", ctypes.c_int32), ("chunk_triangles", ctypes.c_int32),
                 ("bin_capacity_factor", ctypes.c_float), ("min_bin_capacity", ctypes.c_int32),
                 ("cull_mode", ctypes.c_int32), ("queue_grain", ctypes.c_int32),
-                ("timing", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
+                ("timing", ctypes.c_int32), ("raster_mode", ctypes.c_int32)]
 
 
 class FrameStatsInfo(ctypes.Structure):
@@ -58,7 +58,8 @@ class FrameStatsInfo(ctypes.Structure):
                 ("raster_items", ctypes.c_uint64), ("items_phase1", ctypes.c_uint64),
                 ("items_candidates", ctypes.c_uint64), ("items_phase2", ctypes.c_uint64),
                 ("triangles_setup", ctypes.c_uint64), ("vertices_setup", ctypes.c_uint64),
-                ("triangles_phase2", ctypes.c_uint64), ("vertices_phase2", ctypes.c_uint64)]
+                ("triangles_phase2", ctypes.c_uint64), ("vertices_phase2", ctypes.c_uint64),
+                ("raster_pixels", ctypes.c_uint64), ("fragments", ctypes.c_uint64)]
 
 
 class MeshDesc(ctypes.Structure):

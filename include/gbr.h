@@ -113,7 +113,8 @@ typedef struct gbr_tuning {
   int32_t cull_mode;          /* GBR_CULL_* */
   int32_t queue_grain;        /* setup work items claimed per queue fetch: 1..64 */
   int32_t timing;             /* 1: time each stage of gbr_draw() with HIP events */
-  int32_t reserved[1];
+  int32_t raster_mode;        /* 0: a lane per binned triangle, 1: a lane per
+                                 (triangle, pixel) pair; same output */
 } gbr_tuning;
 
 /* stages timed when gbr_tuning.timing is set (gbr_frame_timings) */
@@ -143,6 +144,8 @@ typedef struct gbr_frame_stats_info {
   uint64_t vertices_setup;    /* vertex-range entries of the work items set up */
   uint64_t triangles_phase2;  /* of which in phase 2 */
   uint64_t vertices_phase2;
+  uint64_t raster_pixels;     /* pixel centres tested against binned triangles */
+  uint64_t fragments;         /* of which inside their triangle */
 } gbr_frame_stats_info;
 
 /* ---- parsed-file description consumed by gbr_add_file_data() (new) ---- */

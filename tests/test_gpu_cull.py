@@ -61,11 +61,12 @@ def _walk(spec, steps, seed):
     return out
 
 
-@pytest.mark.parametrize("mode", MODES)
-def test_apartment_walk_matches_unculled(mode):
+@pytest.mark.parametrize("mode,raster_mode", [(N.CULL_FRUSTUM, 0), (N.CULL_HIZ, 0), (N.CULL_HIZ, 1)])
+def test_apartment_walk_matches_unculled(mode, raster_mode):
     spec = scenes.apartments(n_envs=48)
     plain = _renderer(spec, N.CULL_NONE)
     culled = _renderer(spec, mode)
+    culled.set_tuning(raster_mode=raster_mode)
     walk = _walk(spec, 6, seed=11)
     total = skipped = 0
     for k, views in enumerate(walk):
