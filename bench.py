@@ -6,21 +6,24 @@ A step = one draw() of all environments after a per-step camera update (a
 seeded random walk), with every input already resident in HBM; outputs stay
 in HBM (colour RGBA8, linear depth FP32, semantic uint32 planes).  The walk is
 run once per culling mode (plain = every triangle set up, frustum, hiz =
-two-phase depth-pyramid occlusion with the previous frame as proposal); the
-headline value is the hiz mode, and the final frame of every mode is compared
-pixel by pixel with the plain one.
+two-phase depth-pyramid occlusion with the previous frame as proposal,
+learned = the same with the visibility MLP as proposal, trained first on
+other walks); the headline value is the learned mode, and the final frame of
+every mode is compared pixel by pixel with the plain one.
 
-Prints ONE JSON line (rank 0).  ``roofline`` is the headline mode's dominant
-stage: the frozen frame formula
+Prints ONE JSON line (rank 0).  ``roofline`` is the headline mode's frame
+(its kernels back to back on the renderer's stream): the frozen formula
 
   B_frame = sum_env [ sum_drawn (12 T + 16 V + 64) + sum_sensor W H 4 + 8 W H ]
 
-split by stage (setup: geometry of the chunks it set up; resolve: outputs and
-texture footprint; cull: work item + bounds + transform per chunk), divided by
-that stage's mean duration from HIP events recorded on the renderer's stream
-around its launches over the timed steps.  ``cpu_baseline`` times the CPU
-oracle (the project's C restatement of the GL pipeline, OpenMP over
-environments) on a bounded sample of the same workload on this host.
+(drawn = chunks actually set up; culling adds 120 B of work item, bounds and
+transform per chunk, the learned proposal 64 B of features per scored chunk
+and its bf16 weights, the depth pyramid 8 B read per pixel) over the frame's
+mean GPU time from HIP events on that stream across the timed steps; the
+per-stage split of that time comes from events around each stage.
+``cpu_baseline`` times the CPU oracle (the project's C restatement of the GL
+pipeline, OpenMP over environments) on a bounded sample of the same workload
+on this host.
 """
 import argparse
 import json
@@ -113,11 +116,11 @@ def copy_probe(nbytes: int) -> None:
     b.close()
 
 
-MODES = (("plain", 0), ("frustum", 1), ("hiz", 2))
-HEADLINE = "hiz"
+MODES = (("plain", 0), ("frustum", 1), ("hiz", 2), ("learned", 3))
+HEADLINE = "learned"
 
 
-def stage_bytes(stage, spec, st, n_items, n_nodes):
+def stage_bytes(stage, spec, st, n_items, n_nodes, learned=False):
     """Algorithmic bytes a stage is responsible for in one frame (the frozen
     frame formula split by stage; intermediate buffers count nothing)."""
     w, h = spec.tile
@@ -128,8 +131,11 @@ def stage_bytes(stage, spec, st, n_items, n_nodes):
         return 12 * t1 + 16 * v1 + 64 * st["items_phase1"]
     if stage == "setup2":
         return 12 * t2 + 16 * v2 + 64 * st["items_phase2"]
-    if stage == "cull":
-        return n_items * (32 + 24 + 64)  # work item, bounds, transform
+    if stage == "cull":  # work item, bounds, transform; learned: feature rows + bf16 weights
+        rows = st.get("items_phase1", 0) + st.get("items_candidates", 0)
+        return n_items * (32 + 24 + 64) + (64 * rows + 2 * 74497 if learned else 0)
+    if stage == "hiz":  # depth pyramid: read the phase-1 depth, write 4/3 of a float per 16 px
+        return envs * (w * h * 8 + w * h * 4 // 12)
     if stage == "resolve":
         return envs * (3 * w * h * 4 + 8 * w * h)
     if stage == "nodes":
@@ -184,6 +190,9 @@ def main():
     ap.add_argument("--res", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--train-frames", type=int, default=64,
+                    help="frames of training rows for the visibility model")
+    ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (the reference kernel "
                          "for scaling hardware byte counters) and exit")
@@ -212,6 +221,15 @@ def main():
     proj = np.stack([env.projection for env in spec.envs]).astype(np.float32)
     n = len(spec.envs)
 
+    # the visibility model trains on its own walks (random starts, other
+    # seeds) before the learned mode is timed
+    from habitat_sim_amd import visibility
+    t0 = time.perf_counter()
+    training = visibility.train(r, spec, frames=args.train_frames, steps=args.train_steps, seed=3)
+    training["seconds"] = time.perf_counter() - t0
+    training["frames"] = args.train_frames
+    training["poses"] = args.train_frames * len(spec.envs)
+    training["steps"] = args.train_steps
     runs = {}
     for name, mode in MODES:
         runs[name] = run_mode(r, mode, spec, views, proj, args.warmup, args.steps)
@@ -235,19 +253,21 @@ def main():
             "stage_ms": {k: round(v, 4) for k, v in x["stages"].items() if v > 0},
         }
 
-    # roofline of the headline mode's dominant stage
+    # roofline of the headline mode's frame (its kernels back to back on one
+    # stream): the frame formula's bytes over the frame's GPU time
     stages = head["stages"]
     dom = max(stages, key=lambda k: stages[k])
-    dom_bytes = stage_bytes(dom, spec, head["stats"], n_items, n_nodes)
-    dom_ms = stages[dom]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    bytes_frame = sum(stage_bytes(k, spec, head["stats"], n_items, n_nodes) for k in stages)
+    learned = HEADLINE == "learned"
+    bytes_frame = sum(stage_bytes(k, spec, head["stats"], n_items, n_nodes, learned)
+                      for k in stages)
+    frame_ms = head["gpu_ms"]
+    achieved = bytes_frame / (frame_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_apartments512.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        traffic = tj.get("stages", {}).get(HEADLINE, {}).get(dom)
+        traffic = tj.get("frame_bytes", {}).get(HEADLINE)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(spec, args.cpu_seconds)
     value = n * args.steps / head["wall"]
@@ -271,14 +291,18 @@ def main():
                    "cull": HEADLINE, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": dom, "algorithmic_bytes_per_launch": dom_bytes,
-                     "launch_ms": dom_ms,
-                     "frame": {"algorithmic_bytes": bytes_frame, "gpu_ms": head["gpu_ms"],
-                               "achieved_gbs": bytes_frame / (head["gpu_ms"] * 1e-3) / 1e9}},
+                     "kernel": "frame", "algorithmic_bytes_per_launch": bytes_frame,
+                     "launch_ms": frame_ms,
+                     "dominant_stage": {"name": dom, "ms": stages[dom],
+                                        "share": stages[dom] / max(sum(stages.values()), 1e-9)},
+                     "alu": {"pixels_tested_per_s": head["stats"]["raster_pixels"] / (frame_ms * 1e-3),
+                             "fragments_per_s": head["stats"]["fragments"] / (frame_ms * 1e-3),
+                             "triangles_setup_per_s":
+                                 head["stats"]["triangles_setup"] / (frame_ms * 1e-3)}},
         "cpu_baseline": cpu,
         "batch_steps_per_s": value / n,
-        "learned_cull_speedup": None,
-        "cull_speedup": cull[HEADLINE]["speedup_vs_plain"],
+        "learned_cull_speedup": cull["learned"]["speedup_vs_plain"],
+        "vis_training": training,
         "cull": cull,
         "frame_stats": head["stats"],
     }

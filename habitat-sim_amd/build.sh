@@ -10,8 +10,11 @@ FLAGS=(--offload-arch=gfx950 -O3 -fPIC -std=c++17 -ffp-contract=off
        -Xclang -target-feature -Xclang -packed-fp32-ops
        -I"$ROOT/include" -I"$HERE/csrc" -Wall -Wno-unused-function)
 "$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_kernels.hip" -o "$OUT/gbr_kernels.o"
+"$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_mlp.hip" -o "$OUT/gbr_mlp.o"
 "$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_renderer.cpp" -o "$OUT/gbr_renderer.o"
-"$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$OUT/libgbr.so.tmp" "$OUT/gbr_kernels.o" "$OUT/gbr_renderer.o"
+"$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_vis.cpp" -o "$OUT/gbr_vis.o"
+"$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$OUT/libgbr.so.tmp" "$OUT/gbr_kernels.o" \
+  "$OUT/gbr_mlp.o" "$OUT/gbr_renderer.o" "$OUT/gbr_vis.o"
 mv "$OUT/libgbr.so.tmp" "$OUT/libgbr.so"
-rm -f "$OUT/gbr_kernels.o" "$OUT/gbr_renderer.o"
+rm -f "$OUT/gbr_kernels.o" "$OUT/gbr_mlp.o" "$OUT/gbr_renderer.o" "$OUT/gbr_vis.o"
 echo "built $OUT/libgbr.so"

@@ -117,7 +117,8 @@ enum : uint32_t {
   kCtrCand = 5,      // work items to test against the phase-1 depth pyramid
   kCtrList2 = 6,     // work items set up in phase 2
   kCtrClipMark = 7,  // clipped primitives handled by the phase-1 clip pass
-  kCtrCount = 8
+  kCtrProp = 8,      // work items with a visibility-feature row this frame
+  kCtrCount = 9
 };
 constexpr uint32_t kCtrStride = 32;
 
@@ -215,7 +216,6 @@ struct FrameParams {
   uint32_t* cand;               // work items awaiting the depth-pyramid test
   uint32_t* list2;              // work items of phase 2
   uint32_t* item_vis;           // frame number in which the item was last visible
-  const float* proposal;        // per work item probability of being visible (learned mode)
   float* hiz;                   // depth pyramid, hiz_stride floats per scene
   uint32_t frame;               // frame number (>= 2)
   int32_t cull_mode;
@@ -223,6 +223,11 @@ struct FrameParams {
   uint32_t hiz_stride;
   float proposal_threshold;
   int32_t raster_mode;  // 0: a lane per triangle, 1: a lane per (triangle, pixel)
+  // learned proposal: feature rows [kCtrProp count][32] bf16 and their items
+  const float* cam_info;   // [scene][8]: camera position xyz, forward xyz
+  uint16_t* features;      // null: no feature rows this frame
+  uint32_t* prop;
+  const float* logits;     // model output per feature row (learned mode)
   // outputs (env-major planes), any may be null
   uint32_t* out_color;
   float* out_depth;

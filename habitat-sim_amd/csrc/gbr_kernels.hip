@@ -1643,11 +1643,103 @@ __device__ __forceinline__ void append_item(uint32_t* counter, uint32_t* list, b
   if (flag) list[base + (uint32_t)__popcll(m & lt_mask)] = item;
 }
 
+// wave-aggregated append returning the flagged lane's slot
+__device__ __forceinline__ uint32_t append_slot(uint32_t* counter, bool flag, int lane,
+                                                unsigned long long lt_mask) {
+  const unsigned long long m = __ballot(flag);
+  if (m == 0ull) return 0u;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(m & lt_mask);
+}
+
+__device__ __forceinline__ uint32_t f2bf_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0x7fffffu) ? 0x40u : 0u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+__device__ __forceinline__ float log2c(float x) { return log2f(fmaxf(x, 1e-3f)); }
+
+// The 32 features of a (camera, work item) pair for the visibility model:
+//  0-2   visibility history: visible last frame, within 8 frames, age/64
+//  3-7   conservative pixel rectangle (normalised) and its area fraction
+//  8     log2 of the nearest linear depth / 8
+//  9-13  box centre in NDC x, y; log2 of centre, nearest, farthest clip w / 8
+//  14-15 log2 triangles / 10, log2 world-space box diagonal / 4
+//  16-18 world box centre / 8, 19-21 camera position / 8, 22-24 camera forward
+//  25-28 centre minus camera / 8 and its length / 8
+//  29    cosine between forward and the direction to the centre
+//  30    rectangle touches the tile border, 31 constant 1
+__device__ void item_features(const FrameParams& p, const WorkItem& wi, const SceneDesc& sc,
+                              const ItemBounds& bb, const float* M, const float* PV,
+                              const ItemCull& ic, float wmin, float wmax, uint32_t item,
+                              uint16_t* row) {
+  float f[32];
+  const uint32_t seen = p.item_vis[item];
+  f[0] = (seen + 1u == p.frame) ? 1.0f : 0.0f;
+  f[1] = (seen != 0u && p.frame - seen <= 8u) ? 1.0f : 0.0f;
+  f[2] = seen == 0u ? 1.0f : fminf((float)(p.frame - seen), 64.0f) / 64.0f;
+  const float iW = 1.0f / (float)sc.width, iH = 1.0f / (float)sc.height;
+  f[3] = ic.x0 * iW;
+  f[4] = ic.y0 * iH;
+  f[5] = (ic.x1 + 1) * iW;
+  f[6] = (ic.y1 + 1) * iH;
+  f[7] = (f[5] - f[3]) * (f[6] - f[4]);
+  const float un0 = p.unprojections[2 * wi.scene], un1 = p.unprojections[2 * wi.scene + 1];
+  f[8] = log2c(un1 / (fmaxf(ic.zmin, 0.0f) + un0)) / 8.0f;
+  const float cx = 0.5f * (bb.lo[0] + bb.hi[0]), cy = 0.5f * (bb.lo[1] + bb.hi[1]),
+              cz = 0.5f * (bb.lo[2] + bb.hi[2]);
+  const float4 cc = vertex_clip(M, PV, cx, cy, cz);
+  f[9] = fmaxf(fminf(cc.x / cc.w, 4.0f), -4.0f);
+  f[10] = fmaxf(fminf(cc.y / cc.w, 4.0f), -4.0f);
+  f[11] = log2c(cc.w) / 8.0f;
+  f[12] = log2c(wmin) / 8.0f;
+  f[13] = log2c(wmax) / 8.0f;
+  f[14] = log2c((float)wi.tri_count) / 10.0f;
+  float wc[4], wd[4];
+  mat4_vec(M, cx, cy, cz, 1.0f, wc);
+  mat4_vec(M, bb.hi[0] - bb.lo[0], bb.hi[1] - bb.lo[1], bb.hi[2] - bb.lo[2], 0.0f, wd);
+  f[15] = log2c(sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2])) / 4.0f;
+  const float* cam = p.cam_info + (size_t)wi.scene * 8;
+  float d[3], dd = 0.0f, cosf_ = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f[16 + k] = wc[k] / 8.0f;
+    f[19 + k] = cam[k] / 8.0f;
+    f[22 + k] = cam[3 + k];
+    d[k] = wc[k] - cam[k];
+    f[25 + k] = d[k] / 8.0f;
+    dd += d[k] * d[k];
+    cosf_ += d[k] * cam[3 + k];
+  }
+  dd = sqrtf(dd);
+  f[28] = dd / 8.0f;
+  f[29] = dd > 0.0f ? cosf_ / dd : 1.0f;
+  f[30] = (ic.x0 == 0 || ic.y0 == 0 || ic.x1 == sc.width - 1 || ic.y1 == sc.height - 1) ? 1.0f : 0.0f;
+  f[31] = 1.0f;
+  uint4* dst = reinterpret_cast<uint4*>(row);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint4 v;
+    v.x = f2bf_bits(f[8 * q + 0]) | (f2bf_bits(f[8 * q + 1]) << 16);
+    v.y = f2bf_bits(f[8 * q + 2]) | (f2bf_bits(f[8 * q + 3]) << 16);
+    v.z = f2bf_bits(f[8 * q + 4]) | (f2bf_bits(f[8 * q + 5]) << 16);
+    v.w = f2bf_bits(f[8 * q + 6]) | (f2bf_bits(f[8 * q + 7]) << 16);
+    dst[q] = v;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
   const uint32_t item = blockIdx.x * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  bool to1 = false, toc = false;
+  bool to1 = false, toc = false, feat = false;
+  ItemCull icf = {};
+  float wmin_f = 0.0f, wmax_f = 0.0f;
   if (item < p.n_items) {
     const WorkItem wi = p.items[item];
     const SceneDesc sc = p.scenes[wi.scene];
@@ -1664,7 +1756,7 @@ __global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
     uint32_t out_all = 63u;
     bool inner = true;  // strictly inside near, far and the guard band
     float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-    float zmin = INFINITY, wmin = INFINITY;
+    float zmin = INFINITY, wmin = INFINITY, wmax = 0.0f;
 #pragma unroll 1
     for (int k = 0; k < 8; ++k) {
       const float4 c = vertex_clip(M, PV, (k & 1) ? bb.hi[0] : bb.lo[0], (k & 2) ? bb.hi[1] : bb.lo[1],
@@ -1685,7 +1777,7 @@ __global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
         const float nx = c.x * iw, ny = c.y * iw, nz = c.z * iw;
         xmin = fminf(xmin, nx); xmax = fmaxf(xmax, nx);
         ymin = fminf(ymin, ny); ymax = fmaxf(ymax, ny);
-        zmin = fminf(zmin, nz); wmin = fminf(wmin, c.w);
+        zmin = fminf(zmin, nz); wmin = fminf(wmin, c.w); wmax = fmaxf(wmax, c.w);
       }
     }
     if (out_all == 0u) {
@@ -1708,15 +1800,23 @@ __global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
           ic.zmin = (zmin - ez) * 0.5f + 0.5f;
           ic.pad[0] = wi.scene;
           ic.pad[1] = 0u;
-          bool proposed = false;
-          if (p.cull_mode == kCullFrustum) proposed = true;
-          else if (p.cull_mode == kCullHiZ) proposed = p.item_vis[item] + 1u == p.frame;
-          else proposed = p.proposal[item] > p.proposal_threshold;
-          if (proposed) {
-            to1 = true;
+          if (p.features) {
+            feat = true;
+            icf = ic;
+            wmin_f = wmin;
+            wmax_f = wmax;
+          }
+          if (p.cull_mode == kCullLearned) {
+            p.cull[item] = ic;  // k_propose decides from the model's score
           } else {
-            p.cull[item] = ic;
-            toc = true;
+            const bool proposed =
+                p.cull_mode == kCullFrustum || p.item_vis[item] + 1u == p.frame;
+            if (proposed) {
+              to1 = true;
+            } else {
+              p.cull[item] = ic;
+              toc = true;
+            }
           }
         }
       }
@@ -1724,6 +1824,57 @@ __global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
   }
   append_item(&p.counters[kCtrList1 * kCtrStride], p.list1, to1, item, lane, lt_mask);
   append_item(&p.counters[kCtrCand * kCtrStride], p.cand, toc, item, lane, lt_mask);
+  if (p.features) {
+    const uint32_t row = append_slot(&p.counters[kCtrProp * kCtrStride], feat, lane, lt_mask);
+    if (feat) {
+      p.prop[row] = item;
+      const WorkItem wi = p.items[item];
+      float M[16], PV[16];
+      load_mat(p.abs_nodes + (size_t)wi.node * 16, M);
+      load_mat(p.cameras + (size_t)wi.scene * 16, PV);
+      item_features(p, wi, p.scenes[wi.scene], p.bounds[item], M, PV, icf, wmin_f, wmax_f, item,
+                    p.features + (size_t)row * 32);
+    }
+  }
+}
+
+// learned mode: the model's score splits the feature rows into phase 1 and
+// the depth-pyramid candidates
+__global__ __launch_bounds__(kBlock) void k_propose(FrameParams p) {
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t n = min(p.counters[kCtrProp * kCtrStride], p.n_items);
+  bool to1 = false, toc = false;
+  uint32_t item = 0;
+  if (row < n) {
+    item = p.prop[row];
+    to1 = p.logits[row] > p.proposal_threshold;
+    toc = !to1;
+  }
+  append_item(&p.counters[kCtrList1 * kCtrStride], p.list1, to1, item, lane, lt_mask);
+  append_item(&p.counters[kCtrCand * kCtrStride], p.cand, toc, item, lane, lt_mask);
+}
+
+// training rows of this frame: features and whether the item ended up with
+// a visible pixel, appended at dataset row base (rows past cap dropped)
+__global__ __launch_bounds__(kBlock) void k_collect(FrameParams p, uint16_t* __restrict__ X,
+                                                    uint8_t* __restrict__ Y, uint64_t base,
+                                                    uint64_t cap, uint32_t* __restrict__ positives) {
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = min(p.counters[kCtrProp * kCtrStride], p.n_items);
+  bool pos = false;
+  if (row < n && base + row < cap) {
+    const uint32_t item = p.prop[row];
+    pos = p.item_vis[item] == p.frame;
+    const uint4* src = reinterpret_cast<const uint4*>(p.features + (size_t)row * 32);
+    uint4* dst = reinterpret_cast<uint4*>(X + (size_t)(base + row) * 32);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = src[q];
+    Y[base + row] = pos ? 1 : 0;
+  }
+  const unsigned long long m = __ballot(pos);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(positives, (uint32_t)__popcll(m));
 }
 
 // depth pyramid from the phase-1 visibility keys: one workgroup per 32x32
@@ -1875,6 +2026,20 @@ hipError_t launch_resolve(const FrameParams& p, const RasterItem* items, uint32_
 hipError_t launch_cull(const FrameParams& p, hipStream_t s) {
   if (p.n_items == 0) return hipSuccess;
   hipLaunchKernelGGL(k_cull, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_propose(const FrameParams& p, hipStream_t s) {
+  if (p.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_propose, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_collect(const FrameParams& p, uint16_t* X, uint8_t* Y, uint64_t base,
+                          uint64_t cap, uint32_t* positives, hipStream_t s) {
+  if (p.n_items == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_collect, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p, X,
+                     Y, base, cap, positives);
   return hipGetLastError();
 }
 

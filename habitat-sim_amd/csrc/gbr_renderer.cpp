@@ -24,6 +24,8 @@
 
 #include "gbr.h"
 #include "gbr_device.h"
+#include "gbr_host.h"
+#include "gbr_vis.h"
 
 #pragma STDC FP_CONTRACT OFF
 
@@ -41,33 +43,24 @@ hipError_t launch_resolve(const FrameParams& p, const RasterItem* items, uint32_
 hipError_t launch_cull(const FrameParams& p, hipStream_t s);
 hipError_t launch_hiz(const FrameParams& p, uint32_t scenes, hipStream_t s);
 hipError_t launch_cull_hiz(const FrameParams& p, hipStream_t s);
+hipError_t launch_propose(const FrameParams& p, hipStream_t s);
+hipError_t launch_collect(const FrameParams& p, uint16_t* X, uint8_t* Y, uint64_t base,
+                          uint64_t cap, uint32_t* positives, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& g, hipStream_t s);
 }  // namespace gbr
 
 using namespace gbr;
 
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
 
-gbr_status fail(gbr_status s, const std::string& msg) {
+gbr_status gbr::fail(gbr_status s, const std::string& msg) {
   g_error = msg;
   return s;
 }
 
-#define HIP_TRY(expr)                                                                  \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(e_ == hipErrorOutOfMemory ? GBR_ERR_OOM : GBR_ERR_DEVICE,            \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
-  } while (0)
-
-#define TRY(expr)                    \
-  do {                               \
-    gbr_status s_ = (expr);          \
-    if (s_ != GBR_OK) return s_;     \
-  } while (0)
+namespace {
 
 struct Mat4 {
   float m[16];
@@ -98,40 +91,6 @@ Mat4 load4(const float* p) {
   Mat4 r;
   std::memcpy(r.m, p, sizeof r.m);
   return r;
-}
-
-template <class T>
-struct DevBuf {
-  T* ptr = nullptr;
-  size_t cap = 0;
-  gbr_status reserve(size_t n, bool keep = false, hipStream_t s = nullptr) {
-    if (n <= cap && ptr) return GBR_OK;
-    size_t ncap = std::max<size_t>(n + n / 2, 64);
-    T* np = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&np), ncap * sizeof(T));
-    if (e != hipSuccess)
-      return fail(GBR_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    if (keep && ptr && cap) {
-      (void)hipMemcpyAsync(np, ptr, cap * sizeof(T), hipMemcpyDeviceToDevice, s);
-      (void)hipStreamSynchronize(s);
-    }
-    if (ptr) (void)hipFree(ptr);
-    ptr = np;
-    cap = ncap;
-    return GBR_OK;
-  }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    cap = 0;
-  }
-};
-
-template <class T>
-gbr_status upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
-  TRY(b.reserve(std::max<size_t>(v.size(), 1)));
-  if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  return GBR_OK;
 }
 
 struct MeshInfo {
@@ -183,6 +142,7 @@ struct Scene {
   std::vector<Light> lights;
   Mat4 camera = identity4();
   float unproj[2] = {0.0f, 0.0f};
+  float cam_info[8] = {0, 0, 0, 0, 0, -1, 0, 0};  // camera position, forward (world)
   bool xform_dirty = true;
 };
 
@@ -242,6 +202,9 @@ struct gbr_renderer {
   DevBuf<uint32_t> d_list1, d_cand, d_list2, d_item_vis;
   DevBuf<float> d_hiz;
   uint32_t frame_no = 1;
+  DevBuf<float> d_caminfo;
+  VisModel vis;
+  bool collecting = false;
   DevBuf<RasterItem> d_ritems;
   DevBuf<int32_t> d_parents;
   DevBuf<float> d_local, d_abs;
@@ -542,7 +505,7 @@ gbr_status upload_dirty_transforms(gbr_renderer* r) {
 
 gbr_status upload_cameras(gbr_renderer* r) {
   const size_t n = r->scene_count();
-  const size_t need = n * 18;
+  const size_t need = n * 26;  // projection*view, unprojection, camera position + forward
   if (r->cam_event_pending) {
     HIP_TRY(hipEventSynchronize(r->cam_event));
     r->cam_event_pending = false;
@@ -557,11 +520,15 @@ gbr_status upload_cameras(gbr_renderer* r) {
     std::memcpy(r->h_cam + i * 16, r->scenes[i].camera.m, 64);
     r->h_cam[n * 16 + 2 * i] = r->scenes[i].unproj[0];
     r->h_cam[n * 16 + 2 * i + 1] = r->scenes[i].unproj[1];
+    std::memcpy(r->h_cam + n * 18 + 8 * i, r->scenes[i].cam_info, 32);
   }
   TRY(r->d_cameras.reserve(n * 16 + 16));
   TRY(r->d_unproj.reserve(n * 2 + 2));
+  TRY(r->d_caminfo.reserve(n * 8 + 8));
   HIP_TRY(hipMemcpyAsync(r->d_cameras.ptr, r->h_cam, n * 64, hipMemcpyHostToDevice, r->stream));
   HIP_TRY(hipMemcpyAsync(r->d_unproj.ptr, r->h_cam + n * 16, n * 8, hipMemcpyHostToDevice, r->stream));
+  HIP_TRY(hipMemcpyAsync(r->d_caminfo.ptr, r->h_cam + n * 18, n * 32, hipMemcpyHostToDevice,
+                         r->stream));
   HIP_TRY(hipEventRecord(r->cam_event, r->stream));
   r->cam_event_pending = true;
   return GBR_OK;
@@ -659,8 +626,15 @@ gbr_status do_draw(gbr_renderer* r) {
   // previous-frame proposal starts from the last frame drawn
   p.item_vis = r->d_item_vis.ptr;
   p.frame = ++r->frame_no;
-  p.proposal = nullptr;
-  p.proposal_threshold = 0.5f;
+  p.proposal_threshold = r->tun.proposal_threshold;
+  p.cam_info = r->d_caminfo.ptr;
+  const bool learned = p.cull_mode == kCullLearned;
+  if (learned || r->collecting) {
+    TRY(r->vis.reserve_infer(r->n_items));
+    p.features = r->vis.Xi.ptr;
+    p.prop = r->vis.prop.ptr;
+    p.logits = r->vis.logits_i.ptr;
+  }
   p.raster_mode = r->tun.raster_mode;
   {
     const int W = r->cfg.tile_size[0], H = r->cfg.tile_size[1];
@@ -696,11 +670,17 @@ gbr_status do_draw(gbr_renderer* r) {
   const uint32_t setup_grid = (uint32_t)(r->cu_count * r->setup_bpc);
   const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
   const bool culled = p.cull_mode != kCullNone;
-  if (culled) {
+  if (culled || r->collecting) {  // (unculled frames run it for the feature rows only)
     // phase 1: items that pass the frustum test and are proposed visible
     // (all of them in frustum mode); the rest become phase-2 candidates
     StageTimer t(r, GBR_TIMING_CULL);
     HIP_TRY(launch_cull(p, r->stream));
+    if (learned) {
+      const int rows = (int)((r->n_items + 31) & ~31u);
+      TRY(r->vis.forward(r->vis.Xi.ptr, rows, p.counters + kCtrProp * kCtrStride, r->vis.H1i.ptr,
+                         nullptr, r->vis.H2i.ptr, nullptr, r->vis.logits_i.ptr, r->stream));
+      HIP_TRY(launch_propose(p, r->stream));
+    }
   }
   {
     StageTimer t(r, GBR_TIMING_SETUP1);
@@ -735,8 +715,13 @@ gbr_status do_draw(gbr_renderer* r) {
       HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 2, r->stream));
     }
   }
-  StageTimer t(r, GBR_TIMING_RESOLVE);
-  HIP_TRY(launch_resolve(p, r->d_ritems.ptr, r->n_ritems, r->stream));
+  {
+    StageTimer t(r, GBR_TIMING_RESOLVE);
+    HIP_TRY(launch_resolve(p, r->d_ritems.ptr, r->n_ritems, r->stream));
+  }
+  if (r->collecting)
+    HIP_TRY(launch_collect(p, r->vis.Xd.ptr, r->vis.Yd.ptr, r->vis.rows, r->vis.cap,
+                           r->vis.positives.ptr, r->stream));
   return GBR_OK;
 }
 
@@ -859,11 +844,10 @@ static gbr_status validate_tuning(const gbr_tuning* t) {
     return fail(GBR_ERR_ARGUMENT, "tuning: timing must be 0 or 1");
   if (t->queue_grain < 1 || t->queue_grain > 64)
     return fail(GBR_ERR_ARGUMENT, "tuning: queue_grain must be in [1, 64]");
-  if (t->cull_mode == GBR_CULL_LEARNED)
-    return fail(GBR_ERR_UNSUPPORTED, "tuning: GBR_CULL_LEARNED needs a visibility model");
-  if (t->cull_mode != GBR_CULL_NONE && t->cull_mode != GBR_CULL_FRUSTUM &&
-      t->cull_mode != GBR_CULL_HIZ)
+  if (t->cull_mode < GBR_CULL_NONE || t->cull_mode > GBR_CULL_LEARNED)
     return fail(GBR_ERR_ARGUMENT, "tuning: unknown cull_mode");
+  if (!std::isfinite(t->proposal_threshold))
+    return fail(GBR_ERR_ARGUMENT, "tuning: proposal_threshold must be finite");
   return GBR_OK;
 }
 
@@ -947,6 +931,8 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   r->d_ritems.release();
   r->d_parents.release();
   r->d_stats.release();
+  r->d_caminfo.release();
+  r->vis.release();
   r->d_fb_color.release();
   r->d_fb_depth.release();
   r->d_stage.release();
@@ -1315,6 +1301,13 @@ gbr_status gbr_update_camera(gbr_renderer* r, uint32_t scene, const float* proje
   Scene& sc = r->scenes[scene];
   sc.camera = mul4(load4(projection16), load4(view16));
   gbr_calculate_depth_unprojection(projection16, sc.unproj);
+  // camera position -R^T t and forward -R^T e_z of a rigid view (features of
+  // the visibility model)
+  const float* v = view16;
+  for (int i = 0; i < 3; ++i) {
+    sc.cam_info[i] = -(v[i * 4 + 0] * v[12] + v[i * 4 + 1] * v[13] + v[i * 4 + 2] * v[14]);
+    sc.cam_info[3 + i] = -v[i * 4 + 2];
+  }
   return GBR_OK;
 }
 
@@ -1444,8 +1437,15 @@ gbr_status gbr_get_tuning(const gbr_renderer* r, gbr_tuning* out) {
 gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* t) {
   if (!r || !t) return fail(GBR_ERR_ARGUMENT, "set_tuning: null argument");
   TRY(validate_tuning(t));
+  if (t->cull_mode == GBR_CULL_LEARNED && !r->vis.ready)
+    return fail(GBR_ERR_STATE, "tuning: GBR_CULL_LEARNED needs visibility-model weights "
+                               "(gbr_vis_init / gbr_vis_set_weights)");
+  // the frame layout depends on these; culling and launch shapes do not
+  if (t->bin_size != r->tun.bin_size || t->chunk_triangles != r->tun.chunk_triangles ||
+      t->bin_capacity_factor != r->tun.bin_capacity_factor ||
+      t->min_bin_capacity != r->tun.min_bin_capacity)
+    r->structure_dirty = true;
   r->tun = *t;
-  r->structure_dirty = true;
   return GBR_OK;
 }
 
@@ -1485,6 +1485,103 @@ gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
     out->items_phase1 = r->n_items;
   }
   return GBR_OK;
+}
+
+gbr_status gbr_vis_fit_params_init(gbr_vis_fit_params* p) {
+  if (!p) return fail(GBR_ERR_ARGUMENT, "vis_fit_params_init: null");
+  std::memset(p, 0, sizeof *p);
+  p->steps = 1000;
+  p->batch = 8192;
+  p->learning_rate = 1e-3f;
+  p->beta1 = 0.9f;
+  p->beta2 = 0.999f;
+  p->eps = 1e-8f;
+  p->pos_weight = 1.0f;
+  p->seed = 1;
+  return GBR_OK;
+}
+
+gbr_status gbr_vis_init(gbr_renderer* r, uint64_t seed) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "vis_init: null renderer");
+  TRY(set_device(r));
+  return r->vis.init(seed, r->stream);
+}
+
+gbr_status gbr_vis_collect(gbr_renderer* r, uint64_t* rows_added) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "vis_collect: null renderer");
+  TRY(set_device(r));
+  if (r->structure_dirty) {
+    TRY(ensure_planes(r));
+    if (r->assets_dirty) TRY(upload_assets(r));
+    TRY(rebuild_structure(r));
+  }
+  TRY(r->vis.reserve_dataset(r->vis.rows + r->n_items + 1));
+  r->collecting = true;
+  const gbr_status st = do_draw(r);
+  r->collecting = false;
+  TRY(st);
+  uint32_t n = 0;
+  HIP_TRY(hipMemcpyAsync(&n, r->d_counters.ptr + kCtrProp * kCtrStride, sizeof n,
+                         hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  const uint64_t added = std::min<uint64_t>(n, r->vis.cap - r->vis.rows);
+  r->vis.rows += added;
+  if (rows_added) *rows_added = added;
+  return GBR_OK;
+}
+
+gbr_status gbr_vis_clear_rows(gbr_renderer* r) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "vis_clear_rows: null renderer");
+  TRY(set_device(r));
+  r->vis.rows = 0;
+  if (r->vis.positives.ptr) HIP_TRY(hipMemset(r->vis.positives.ptr, 0, sizeof(uint32_t)));
+  return GBR_OK;
+}
+
+gbr_status gbr_vis_rows(gbr_renderer* r, uint64_t* rows, uint64_t* positives) {
+  if (!r || !rows || !positives) return fail(GBR_ERR_ARGUMENT, "vis_rows: null argument");
+  TRY(set_device(r));
+  *rows = r->vis.rows;
+  uint32_t p = 0;
+  if (r->vis.positives.ptr) {
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    HIP_TRY(hipMemcpy(&p, r->vis.positives.ptr, sizeof p, hipMemcpyDeviceToHost));
+  }
+  *positives = p;
+  return GBR_OK;
+}
+
+gbr_status gbr_vis_fit(gbr_renderer* r, const gbr_vis_fit_params* params,
+                       gbr_vis_fit_report* report) {
+  if (!r || !params) return fail(GBR_ERR_ARGUMENT, "vis_fit: null argument");
+  TRY(set_device(r));
+  return r->vis.fit(*params, report, r->stream);
+}
+
+gbr_status gbr_vis_get_weights(gbr_renderer* r, float* out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "vis_get_weights: null argument");
+  TRY(set_device(r));
+  return r->vis.get_weights(out, r->stream);
+}
+
+gbr_status gbr_vis_set_weights(gbr_renderer* r, const float* in) {
+  if (!r || !in) return fail(GBR_ERR_ARGUMENT, "vis_set_weights: null argument");
+  TRY(set_device(r));
+  return r->vis.set_weights(in, r->stream);
+}
+
+gbr_status gbr_vis_loss_grad(gbr_renderer* r, const float* x, const uint8_t* labels,
+                             int32_t rows, float pos_weight, float* loss, float* grad) {
+  if (!r || !x || !labels || !loss || !grad)
+    return fail(GBR_ERR_ARGUMENT, "vis_loss_grad: null argument");
+  TRY(set_device(r));
+  return r->vis.loss_grad(x, labels, rows, pos_weight, loss, grad, r->stream);
+}
+
+gbr_status gbr_vis_predict(gbr_renderer* r, const float* x, int32_t rows, float* logits) {
+  if (!r || !x || !logits) return fail(GBR_ERR_ARGUMENT, "vis_predict: null argument");
+  TRY(set_device(r));
+  return r->vis.predict(x, rows, logits, r->stream);
 }
 
 gbr_status gbr_frame_timings(gbr_renderer* r, float* ms) {

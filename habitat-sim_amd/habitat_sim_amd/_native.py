@@ -49,7 +49,25 @@ class Tuning(ctypes.Structure):
     _fields_ = [("bin_size", ctypes.c_int32), ("chunk_triangles", ctypes.c_int32),
                 ("bin_capacity_factor", ctypes.c_float), ("min_bin_capacity", ctypes.c_int32),
                 ("cull_mode", ctypes.c_int32), ("queue_grain", ctypes.c_int32),
-                ("timing", ctypes.c_int32), ("raster_mode", ctypes.c_int32)]
+                ("timing", ctypes.c_int32), ("raster_mode", ctypes.c_int32),
+                ("proposal_threshold", ctypes.c_float)]
+
+
+VIS_FEATURES = 32
+VIS_PARAMS = 74497
+
+
+class VisFitParams(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+                ("pos_weight", ctypes.c_float), ("reserved", ctypes.c_int32),
+                ("seed", ctypes.c_uint64)]
+
+
+class VisFitReport(ctypes.Structure):
+    _fields_ = [("loss_first", ctypes.c_float), ("loss_last", ctypes.c_float),
+                ("rows", ctypes.c_uint64), ("positives", ctypes.c_uint64)]
 
 
 class FrameStatsInfo(ctypes.Structure):
@@ -162,6 +180,17 @@ SIGNATURES = {
     "gbr_set_tuning": (S, [R, ctypes.POINTER(Tuning)]),
     "gbr_frame_stats": (S, [R, ctypes.POINTER(FrameStatsInfo)]),
     "gbr_frame_timings": (S, [R, ctypes.POINTER(ctypes.c_float)]),
+    "gbr_vis_fit_params_init": (S, [ctypes.POINTER(VisFitParams)]),
+    "gbr_vis_init": (S, [R, ctypes.c_uint64]),
+    "gbr_vis_collect": (S, [R, ctypes.POINTER(ctypes.c_uint64)]),
+    "gbr_vis_clear_rows": (S, [R]),
+    "gbr_vis_rows": (S, [R, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "gbr_vis_fit": (S, [R, ctypes.POINTER(VisFitParams), ctypes.POINTER(VisFitReport)]),
+    "gbr_vis_get_weights": (S, [R, ctypes.c_void_p]),
+    "gbr_vis_set_weights": (S, [R, ctypes.c_void_p]),
+    "gbr_vis_loss_grad": (S, [R, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_float,
+                              ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
+    "gbr_vis_predict": (S, [R, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
 }
 
 _lib = None

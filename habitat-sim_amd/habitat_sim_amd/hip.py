@@ -69,6 +69,12 @@ class DeviceBuffer:
             pass
 
 
+def memcpy_d2h(out, src: int) -> None:
+    """Copies out.nbytes from device address src into the host array out."""
+    _check(hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(src), out.nbytes, D2H),
+           "hipMemcpy D2H")
+
+
 def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
     _check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, D2D), "hipMemcpy D2D")
 

@@ -429,6 +429,66 @@ class RendererStandalone:
             setattr(t, k, v)
         N.call("gbr_set_tuning", self._h, ctypes.byref(t))
 
+    # ---- learned visibility proposal ------------------------------------
+    def vis_init(self, seed: int = 1) -> None:
+        """Random visibility-model weights (He-uniform) from seed."""
+        N.call("gbr_vis_init", self._h, seed)
+
+    def vis_collect(self) -> int:
+        """draw() with the current cameras and append one training row per
+        chunk that passed the frustum test; returns the rows added."""
+        n = ctypes.c_uint64()
+        N.call("gbr_vis_collect", self._h, ctypes.byref(n))
+        return n.value
+
+    def vis_clear_rows(self) -> None:
+        N.call("gbr_vis_clear_rows", self._h)
+
+    def vis_rows(self) -> tuple:
+        """(training rows, rows labelled visible)."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        N.call("gbr_vis_rows", self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def vis_fit(self, **kw) -> dict:
+        """Adam on the training rows; keyword arguments override
+        gbr_vis_fit_params_init() defaults (steps, batch, learning_rate,
+        beta1, beta2, eps, pos_weight, seed)."""
+        p = N.VisFitParams()
+        N.call("gbr_vis_fit_params_init", ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        rep = N.VisFitReport()
+        N.call("gbr_vis_fit", self._h, ctypes.byref(p), ctypes.byref(rep))
+        return {k: getattr(rep, k) for k, _ in rep._fields_}
+
+    def vis_weights(self) -> np.ndarray:
+        out = np.empty(N.VIS_PARAMS, np.float32)
+        N.call("gbr_vis_get_weights", self._h, out.ctypes.data)
+        return out
+
+    def set_vis_weights(self, w) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        if w.size != N.VIS_PARAMS:
+            raise ValueError(f"expected {N.VIS_PARAMS} weights, got {w.size}")
+        N.call("gbr_vis_set_weights", self._h, w.ctypes.data)
+
+    def vis_loss_grad(self, x, labels, pos_weight: float = 1.0):
+        """(mean weighted loss, gradient) of the model on caller rows."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(labels, np.uint8)
+        loss = ctypes.c_float()
+        grad = np.empty(N.VIS_PARAMS, np.float32)
+        N.call("gbr_vis_loss_grad", self._h, x.ctypes.data, y.ctypes.data, len(y), pos_weight,
+               ctypes.byref(loss), grad.ctypes.data)
+        return loss.value, grad
+
+    def vis_predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty(len(x), np.float32)
+        N.call("gbr_vis_predict", self._h, x.ctypes.data, len(x), out.ctypes.data)
+        return out
+
     def frame_timings(self) -> dict:
         """Milliseconds per stage of the last draw() (needs tuning timing=1)."""
         ms = (ctypes.c_float * len(N.TIMING_STAGES))()

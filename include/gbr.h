@@ -78,7 +78,7 @@ enum {
 #define GBR_CULL_FRUSTUM 1 /* per-chunk frustum test before setup (exact) */
 #define GBR_CULL_HIZ 2     /* + two-phase depth-pyramid occlusion test; phase 1 =
                               chunks visible in the previous frame (exact) */
-#define GBR_CULL_LEARNED 3 /* + phase 1 = chunks a visibility model predicts (exact) */
+#define GBR_CULL_LEARNED 3 /* + phase 1 = chunks the visibility model proposes (exact) */
 
 typedef struct gbr_renderer gbr_renderer;
 
@@ -115,6 +115,8 @@ typedef struct gbr_tuning {
   int32_t timing;             /* 1: time each stage of gbr_draw() with HIP events */
   int32_t raster_mode;        /* 0: a lane per binned triangle, 1: a lane per
                                  (triangle, pixel) pair; same output */
+  float proposal_threshold;   /* GBR_CULL_LEARNED: phase 1 = chunks whose logit
+                                 exceeds this */
 } gbr_tuning;
 
 /* stages timed when gbr_tuning.timing is set (gbr_frame_timings) */
@@ -319,6 +321,55 @@ gbr_status gbr_unproject_depth(const float* unprojection2, float* depth, int64_t
 gbr_status gbr_get_tuning(const gbr_renderer* r, gbr_tuning* out);
 gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* tuning);
 gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out);
+/* ---- learned visibility proposal (new) -------------------------------------
+   No counterpart in the reference, whose batch renderer submits every mesh
+   of every scene (Renderer.cpp:1298-1351).  A 32 -> 256 -> 256 -> 1 MLP
+   (BF16 MFMA, FP32 accumulation and master weights) scores each chunk that
+   passes the frustum test from per-(camera, chunk) features; with
+   GBR_CULL_LEARNED the chunks it scores above gbr_tuning.proposal_threshold
+   form phase 1 of the two-phase depth-pyramid cull.  The proposal only
+   orders work: output pixels never depend on it.  Training rows are
+   (features, chunk visible) pairs taken from frames the renderer draws. */
+#define GBR_VIS_FEATURES 32
+#define GBR_VIS_PARAMS 74497 /* W1[256][32] b1[256] W2[256][256] b2[256] w3[256] b3 */
+
+typedef struct gbr_vis_fit_params {
+  int32_t steps;            /* optimiser steps */
+  int32_t batch;            /* rows per step: multiple of 32, 32..65536 */
+  float learning_rate;      /* Adam step size */
+  float beta1, beta2, eps;  /* Adam */
+  float pos_weight;         /* loss weight of rows labelled visible */
+  int32_t reserved;
+  uint64_t seed;            /* minibatch sampling */
+} gbr_vis_fit_params;
+
+typedef struct gbr_vis_fit_report {
+  float loss_first;         /* mean weighted loss of the first step */
+  float loss_last;          /* mean weighted loss over the last tenth of the steps */
+  uint64_t rows;            /* training rows */
+  uint64_t positives;       /* of which labelled visible */
+} gbr_vis_fit_report;
+
+gbr_status gbr_vis_fit_params_init(gbr_vis_fit_params* params);
+/* random (He-uniform) weights from seed; keeps the training rows */
+gbr_status gbr_vis_init(gbr_renderer* r, uint64_t seed);
+/* gbr_draw() with the current cameras, then appends one training row per
+   chunk that passed the frustum test strictly inside the clip volume */
+gbr_status gbr_vis_collect(gbr_renderer* r, uint64_t* rows_added);
+gbr_status gbr_vis_clear_rows(gbr_renderer* r);
+gbr_status gbr_vis_rows(gbr_renderer* r, uint64_t* rows, uint64_t* positives);
+/* Adam on the weighted binary cross-entropy of the training rows */
+gbr_status gbr_vis_fit(gbr_renderer* r, const gbr_vis_fit_params* params,
+                       gbr_vis_fit_report* report);
+gbr_status gbr_vis_get_weights(gbr_renderer* r, float* out /* GBR_VIS_PARAMS */);
+gbr_status gbr_vis_set_weights(gbr_renderer* r, const float* in /* GBR_VIS_PARAMS */);
+/* mean weighted loss and its gradient (GBR_VIS_PARAMS floats) for caller rows
+   x[rows][GBR_VIS_FEATURES] (rounded to BF16) and labels; rows multiple of 32 */
+gbr_status gbr_vis_loss_grad(gbr_renderer* r, const float* x, const uint8_t* labels,
+                             int32_t rows, float pos_weight, float* loss, float* grad);
+/* logits of caller rows x[rows][GBR_VIS_FEATURES] */
+gbr_status gbr_vis_predict(gbr_renderer* r, const float* x, int32_t rows, float* logits);
+
 /* new: milliseconds per GBR_TIMING_* stage, averaged over the gbr_draw()
    calls since the previous call (at most 4096; a stage that did not run
    counts 0); needs gbr_tuning.timing = 1; synchronises the stream */

@@ -128,5 +128,5 @@ def test_learned_mode_needs_a_model():
     r = _renderer(spec, N.CULL_NONE)
     with pytest.raises(N.GbrError) as e:
         r.set_tuning(cull_mode=N.CULL_LEARNED)
-    assert e.value.status == N.ERR_UNSUPPORTED
+    assert e.value.status == N.ERR_STATE
     r.close()

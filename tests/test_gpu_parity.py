@@ -158,3 +158,27 @@ def test_full_size_properties_512_envs():
         np.testing.assert_array_equal(sem[sl], planes["semantic"][k], err_msg=f"env {e} semantic")
         np.testing.assert_array_equal(lin[sl], planes["lindepth"][k], err_msg=f"env {e} depth")
     r.close()
+
+
+def test_sliver_with_degenerate_depth_plane_matches_oracle():
+    """The lattice sliver whose float32 depth plane is NaN (see
+    test_oracle_rules.sliver_file) renders as in the oracle: hidden behind
+    the quad, and alone at a depth between its vertices'."""
+    import synth
+    from test_oracle_rules import sliver_file, WHITE
+    from habitat_sim_amd import math3d as m3
+    meshes = sliver_file()
+    for ms in (meshes, meshes[1:]):
+        fd = synth.mesh_file(ms, [synth.flat(WHITE)])
+        outs = []
+        for make in (lambda: RendererStandalone(RendererConfiguration().set_tile_size_count((64, 64), (1, 1))),
+                     lambda: OracleRenderer((64, 64), (1, 1))):
+            x = make()
+            assert x.add_file_data(fd)
+            for i in range(len(ms)):
+                x.add_node_hierarchy(0, f"m{i}", None, i + 1)
+            x.update_camera(0, m3.identity(), m3.identity())
+            x.draw() if isinstance(x, RendererStandalone) else x.draw(NTHREADS)
+            outs.append(x)
+        _compare(outs[0], outs[1])
+        outs[0].close()

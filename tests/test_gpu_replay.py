@@ -1,0 +1,54 @@
+"""The replay-renderer surface on the GPU: render(color_images,
+depth_images) fills caller arrays with GL rows like the reference's
+AbstractReplayRenderer::render, get_observations() gives habitat's top-down
+observation dict, and both equal the oracle's planes."""
+import numpy as np
+import pytest
+
+from habitat_sim_amd import replay, scenes
+from habitat_sim_amd import math3d as m3
+from oracle.oracle import OracleRenderer
+
+pytestmark = pytest.mark.gpu
+
+
+def test_render_and_observations_match_oracle():
+    spec = scenes.small_rooms(n_envs=5)
+    S = replay.CameraSensorSpec
+    cfg = replay.ReplayRendererConfiguration(
+        5, [S("rgba", "color", (128, 128)), S("depth", "depth", (128, 128)),
+            S("semantic", "semantic", (128, 128))])
+    rr = replay.BatchReplayRenderer(cfg)
+    assert rr.grid == (3, 2)
+    assert rr.add_file_data(spec.file)
+    o = OracleRenderer((128, 128), rr.grid)
+    o.add_file_data(spec.file)
+    for e, env in enumerate(spec.envs):
+        for inst in env.instances:
+            rr.add_instance(e, inst.template, inst.bake, inst.semantic)
+            o.add_node_hierarchy(e, inst.template, inst.bake, inst.semantic)
+    rr.set_sensor_transforms(np.stack([env.camera_pose for env in spec.envs]))
+    proj = S("x").projection()
+    for e, env in enumerate(spec.envs):
+        o.update_camera(e, proj, m3.inverted(env.camera_pose))
+    colors = [np.zeros((128, 128, 4), np.uint8) for _ in range(5)]
+    depths = [np.zeros((128, 128), np.float32) for _ in range(5)]
+    rr.render(colors, depths)
+    planes = o.render(scenes=list(range(5)))
+    obs = rr.get_observations()
+    assert obs["rgba"].dtype == np.uint8 and obs["rgba"].shape == (5, 128, 128, 4)
+    assert obs["depth"].dtype == np.float32 and obs["semantic"].dtype == np.uint32
+    for e in range(5):
+        np.testing.assert_array_equal(colors[e], planes["color"][e])
+        np.testing.assert_array_equal(depths[e], planes["lindepth"][e])
+        # observations: the same, rows top-down
+        np.testing.assert_array_equal(obs["rgba"][e], planes["color"][e][::-1])
+        np.testing.assert_array_equal(obs["depth"][e], planes["lindepth"][e][::-1])
+        np.testing.assert_array_equal(obs["semantic"][e], planes["semantic"][e][::-1])
+    one = rr.get_observations(3)
+    np.testing.assert_array_equal(one["semantic"], obs["semantic"][3])
+    # empty lists skip; wrong counts are errors
+    rr.render([], depths)
+    with pytest.raises(ValueError):
+        rr.render(colors[:2], [])
+    rr.close()

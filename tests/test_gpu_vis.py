@@ -1,0 +1,155 @@
+"""The learned visibility proposal: BF16 MFMA forward/backward against the
+float64 CPU oracle (oracle/mlp_ref.py), training on rows the renderer
+collects, exactness of the learned cull, and its ablation against the
+previous-frame proposal and frustum-only culling."""
+import math
+
+import numpy as np
+import pytest
+
+from habitat_sim_amd import RendererConfiguration, RendererStandalone, scenes
+from habitat_sim_amd import _native as N
+from habitat_sim_amd import math3d as m3
+from habitat_sim_amd import visibility
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+PLANES = ("color", "depth", "linear_depth", "semantic")
+
+
+def _small():
+    spec = scenes.small_rooms(n_envs=1)
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count(spec.tile, spec.grid))
+    scenes.populate(r, spec)
+    return r
+
+
+def _rows(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(0, 1, (n, M.IN)).astype(np.float32)
+    y = (rng.uniform(size=n) < 0.3).astype(np.uint8)
+    return X, y
+
+
+@pytest.mark.parametrize("rows,pos_weight", [(64, 1.0), (512, 2.5)])
+def test_loss_and_gradient_match_oracle(rows, pos_weight):
+    """Tolerance: the GPU sums in float32 (MFMA accumulation order) before
+    rounding activations to bfloat16, the oracle in float64, so an activation
+    next to a rounding boundary may land one bf16 ulp (2^-8 relative) apart;
+    per parameter block the gradient agrees to 1% of its norm and the loss
+    to 1e-4 relative."""
+    r = _small()
+    flat = M.he_uniform(11).astype(np.float32)
+    r.set_vis_weights(flat)
+    X, y = _rows(rows, 3)
+    loss, g = r.vis_loss_grad(X, y, pos_weight)
+    ref_loss, ref_g = M.loss_grad(flat.astype(np.float64), X, y, pos_weight, rounding=True)
+    assert abs(loss - ref_loss) <= 1e-4 * abs(ref_loss)
+    o = 0
+    for name, shape in M.SIZES:
+        n = int(np.prod(shape))
+        a, b = g[o:o + n].astype(np.float64), ref_g[o:o + n]
+        assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b) + 1e-7, name
+        o += n
+    r.close()
+
+
+def test_logits_match_oracle():
+    r = _small()
+    flat = M.he_uniform(5).astype(np.float32)
+    r.set_vis_weights(flat)
+    X, _ = _rows(100, 9)  # not a multiple of 32: padded rows
+    z = r.vis_predict(X)
+    ref, _ = M.forward(flat.astype(np.float64), X, rounding=True)
+    assert np.max(np.abs(z - ref)) <= 1e-2 * (1.0 + np.max(np.abs(ref)))
+    r.close()
+
+
+def test_weights_round_trip_and_init():
+    r = _small()
+    with pytest.raises(N.GbrError):
+        r.vis_weights()  # no model yet
+    with pytest.raises(N.GbrError):
+        r.set_tuning(cull_mode=N.CULL_LEARNED)
+    r.vis_init(7)
+    w = r.vis_weights()
+    assert w.shape == (N.VIS_PARAMS,) and np.isfinite(w).all() and np.abs(w).max() > 0
+    r.set_vis_weights(w * 0.5)
+    np.testing.assert_array_equal(r.vis_weights(), (w * 0.5).astype(np.float32))
+    r.close()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A model trained on seeded apartment walks (not the evaluation walk)."""
+    spec = scenes.apartments(n_envs=32)
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count(spec.tile, spec.grid))
+    scenes.populate(r, spec)
+    report = visibility.train(r, spec, frames=40, steps=400, seed=21)
+    yield spec, r, report
+    r.close()
+
+
+def test_training_lowers_the_loss(trained):
+    _, r, rep = trained
+    rows, pos = r.vis_rows()
+    assert rows > 10000 and 0 < pos < rows
+    assert rep["loss_last"] < 0.7 * rep["loss_first"]
+
+
+def _modes_on_frames(spec, r, frames, modes):
+    """Per mode: planes of every frame and the work items set up."""
+    out = {}
+    for mode in modes:
+        r.set_tuning(cull_mode=mode)
+        planes, items = [], 0
+        for views in frames:
+            for e, v in enumerate(views):
+                r.update_camera(e, spec.envs[e].projection, v)
+            r.draw()
+            st = r.frame_stats()
+            items += st["items_phase1"] + st["items_phase2"]
+            planes.append({n: getattr(r, f"{n}_image")() for n in PLANES})
+        out[mode] = (planes, items)
+    return out
+
+
+def test_learned_cull_is_exact_and_ablation(trained):
+    spec, r, _ = trained
+    walk = visibility.camera_walk(spec, 5, seed=99)
+    res = _modes_on_frames(spec, r, walk, [N.CULL_NONE, N.CULL_FRUSTUM, N.CULL_HIZ, N.CULL_LEARNED])
+    plain = res[N.CULL_NONE][0]
+    for mode in (N.CULL_FRUSTUM, N.CULL_HIZ, N.CULL_LEARNED):
+        for k, pl in enumerate(res[mode][0]):
+            for n in PLANES:
+                np.testing.assert_array_equal(pl[n], plain[k][n], err_msg=f"mode {mode} frame {k} {n}")
+    # the learned proposal sets up no more chunks than frustum culling alone
+    assert res[N.CULL_LEARNED][1] < res[N.CULL_FRUSTUM][1]
+
+
+def test_learned_proposal_without_history(trained):
+    """Teleported cameras: the previous frame says nothing about the new
+    views, the model still proposes from geometry."""
+    spec, r, _ = trained
+    rng = np.random.default_rng(1234)
+    frames = []
+    for _ in range(3):
+        frames.append([m3.inverted_rigid(visibility.random_pose(rng)) for _ in spec.envs])
+    # each teleport follows a frame at other poses, so history is stale
+    res = {}
+    for mode in (N.CULL_FRUSTUM, N.CULL_HIZ, N.CULL_LEARNED):
+        r.set_tuning(cull_mode=mode)
+        items = 0
+        planes = []
+        for views in frames:
+            for e, v in enumerate(views):
+                r.update_camera(e, spec.envs[e].projection, v)
+            r.draw()
+            st = r.frame_stats()
+            items += st["items_phase1"] + st["items_phase2"]
+            planes.append(r.semantic_image())
+        res[mode] = (items, planes)
+    for k in range(len(frames)):
+        np.testing.assert_array_equal(res[N.CULL_LEARNED][1][k], res[N.CULL_FRUSTUM][1][k])
+    assert res[N.CULL_LEARNED][0] < res[N.CULL_FRUSTUM][0]

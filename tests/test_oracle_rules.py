@@ -275,3 +275,39 @@ def test_unproject_depth_reference_cases(depth, expected, near, far):
     d = np.float32((zn + np.float32(1.0)) / np.float32(2.0))  # lerpInverted(-1, 1, z)
     out = unproject_depth(depth_unprojection(P), np.array([d], np.float32))[0]
     assert abs(out - expected) <= depth * 0.0002
+
+
+# A lattice sliver (twice its area is 4 subpixel^2) with one vertex on the
+# centre of pixel (10, 20) and one 4000 px away, inside the guard band: in
+# float32 the area of the depth plane rounds to 0, so its gradient is NaN.
+# Window coordinates (subpixels) and depths of the vertices:
+SLIVER_X = (1051264, 2688, 2684)
+SLIVER_Y = (1053823, 5248, 5244)
+SLIVER_Z = (0.3, 0.7, 0.5)
+
+
+def sliver_file():
+    """The sliver behind a full-tile quad at window depth 0.2 (64x64 tile,
+    identity camera; x_ndc = X/8192 - 1 is exact in float32)."""
+    pos = [(x / 8192.0 - 1.0, y / 8192.0 - 1.0, 2.0 * z - 1.0)
+           for x, y, z in zip(SLIVER_X, SLIVER_Y, SLIVER_Z)]
+    quad = np.array([(-1, -1, -0.6), (1, -1, -0.6), (1, 1, -0.6), (-1, 1, -0.6)], np.float32)
+    return [(quad, [0, 1, 2, 0, 2, 3], None, None, 0), (np.array(pos, np.float32), [0, 1, 2], None, None, 0)]
+
+
+def test_sliver_never_lands_in_front_of_its_triangle():
+    """Every point of a triangle lies at or behind its nearest vertex, so a
+    triangle entirely behind an opaque quad never shows through it, however
+    the depth plane of a sliver rounds."""
+    meshes = sliver_file()
+    # alone, the sliver covers exactly that pixel centre, at a depth between
+    # its vertices' depths
+    _, out = _render((64, 64), meshes[1:], semantics=[2])
+    sem = out["semantic"][0]
+    assert int((sem == 2).sum()) == 1
+    d = out["depth"][0][sem == 2][0]
+    assert 0.3 - 1e-6 <= d <= 0.7 + 1e-6
+    # behind the quad it stays hidden
+    _, out = _render((64, 64), meshes, semantics=[1, 2])
+    assert (out["semantic"][0] == 1).all()
+    assert (out["depth"][0] == out["depth"][0][0, 0]).all()
