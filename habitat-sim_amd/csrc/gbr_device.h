@@ -223,6 +223,7 @@ struct FrameParams {
   uint32_t hiz_stride;
   float proposal_threshold;
   int32_t raster_mode;  // 0: a lane per triangle, 1: a lane per (triangle, pixel)
+  int32_t hiz_fused;    // phase-1 raster writes the depth pyramid (32-pixel bins)
   // learned proposal: feature rows [kCtrProp count][32] bf16 and their items
   const float* cam_info;   // [scene][8]: camera position xyz, forward xyz
   uint16_t* features;      // null: no feature rows this frame

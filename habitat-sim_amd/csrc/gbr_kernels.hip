@@ -1192,6 +1192,7 @@ struct BinInfo {
   SceneDesc sc;
   PixRect bin;
   uint32_t gbin;
+  uint32_t scene;
 };
 
 __device__ __forceinline__ BinInfo bin_info(const FrameParams& p, const RasterItem& it) {
@@ -1204,6 +1205,7 @@ __device__ __forceinline__ BinInfo bin_info(const FrameParams& p, const RasterIt
   b.bin.x1 = min(bx * B + B, b.sc.width) - 1;
   b.bin.y1 = min(by * B + B, b.sc.height) - 1;
   b.gbin = b.sc.bin_base + it.bin;
+  b.scene = it.scene;
   return b;
 }
 
@@ -1283,6 +1285,34 @@ __device__ __forceinline__ void add_raster_stats(const FrameParams& p, uint32_t 
   }
 }
 
+// Depth pyramid of one 32x32 bin (s_vis rows of kMaxBin keys) from its final
+// phase-1 keys in LDS: max
+// depth per 8x8 pixels (level 0) and over the bin (level 1), the same values
+// k_hiz computes from the stored keys.
+__device__ void bin_hiz(const FrameParams& p, const BinInfo& b, const unsigned long long* s_vis) {
+  __shared__ uint32_t s_max[16];
+  if (threadIdx.x < 16) s_max[threadIdx.x] = 0u;
+  __syncthreads();
+  const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
+  for (int l = threadIdx.x; l < kMaxBin * kMaxBin; l += kBlock) {
+    const int lx = l & 31, ly = l >> 5;
+    if (lx < bw && ly < bh)
+      atomicMax(&s_max[(ly >> 3) * 4 + (lx >> 3)], (uint32_t)(s_vis[ly * kMaxBin + lx] >> 32));
+  }
+  __syncthreads();
+  const int rx = b.bin.x0 >> kHiZ1, ry = b.bin.y0 >> kHiZ1;
+  float* h = p.hiz + (size_t)b.scene * p.hiz_stride;
+  if (threadIdx.x < 16) {
+    const int bx = rx * 4 + (int)(threadIdx.x & 3), by = ry * 4 + (int)(threadIdx.x >> 2);
+    if (bx < p.hiz_w0 && by < p.hiz_h0) h[by * p.hiz_w0 + bx] = __uint_as_float(s_max[threadIdx.x]);
+  }
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (int k = 0; k < 16; ++k) m = max(m, s_max[k]);
+    h[p.hiz_w0 * p.hiz_h0 + ry * p.hiz_w1 + rx] = __uint_as_float(m);
+  }
+}
+
 // --------------------------------------------------------------------------
 // k_raster_lane: per (scene, bin), one binned triangle per lane walking its
 // clipped rectangle; rectangles above kSmallRaster pixels are rasterised by
@@ -1343,6 +1373,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
     __syncthreads();
   }
   store_vis(p, b, s_vis, B);
+  if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
   if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
   add_raster_stats(p, pairs, frags);
 }
@@ -1459,7 +1490,8 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
 
 // exact fallback for bins whose list overflowed: every primitive of the scene
 __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
-                                                        const RasterItem* __restrict__ items) {
+                                                        const RasterItem* __restrict__ items,
+                                                        int phase) {
   __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
   __shared__ BigTri s_big[kBigQueue];
   __shared__ uint32_t s_base[kDrawCache];
@@ -1497,6 +1529,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
     __syncthreads();
   }
   store_vis(p, b, s_vis, B);
+  if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
   if (threadIdx.x == 0) {
     add_stat(p.stats, 1, 1ull);
     p.bin_count[b.gbin] = 0;
@@ -2012,7 +2045,7 @@ hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t
     hipLaunchKernelGGL(k_raster_lane, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_raster_full, dim3(n), dim3(kBlock), 0, s, p, items);
+  hipLaunchKernelGGL(k_raster_full, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@
 #include "gbr.h"
 #include "gbr_device.h"
 #include "gbr_host.h"
+#include "gbr_import.h"
 #include "gbr_vis.h"
 
 #pragma STDC FP_CONTRACT OFF
@@ -646,6 +647,8 @@ gbr_status do_draw(gbr_renderer* r) {
     if (p.cull_mode >= kCullHiZ) {
       TRY(r->d_hiz.reserve((size_t)p.hiz_stride * std::max<uint32_t>(r->scene_count(), 1)));
       p.hiz = r->d_hiz.ptr;
+      // the lane rasteriser builds the pyramid from its 32x32 LDS tiles
+      p.hiz_fused = (r->tun.bin_size == 32 && r->tun.raster_mode == 0) ? 1 : 0;
     }
   }
   p.sub_cap = r->sub_cap;
@@ -699,7 +702,7 @@ gbr_status do_draw(gbr_renderer* r) {
     // phase 2: candidates not hidden behind the phase-1 depth pyramid
     {
       StageTimer t(r, GBR_TIMING_HIZ);
-      HIP_TRY(launch_hiz(p, r->scene_count(), r->stream));
+      if (!p.hiz_fused) HIP_TRY(launch_hiz(p, r->scene_count(), r->stream));
       HIP_TRY(launch_cull_hiz(p, r->stream));
     }
     {
@@ -958,11 +961,17 @@ gbr_status gbr_scene_count(const gbr_renderer* r, uint64_t* out) {
   return GBR_OK;
 }
 
-gbr_status gbr_add_file(gbr_renderer* r, const char* filename, uint32_t, const char*) {
+// Renderer::addFile(), Renderer.cpp:354-867: the native importer parses the
+// glTF composite, then the parsed file goes through gbr_add_file_data()
+gbr_status gbr_add_file(gbr_renderer* r, const char* filename, uint32_t flags, const char* name) {
   if (!r || !filename) return fail(GBR_ERR_ARGUMENT, "gbr_add_file: null argument");
-  return fail(GBR_ERR_UNSUPPORTED,
-              "gbr_add_file: native file import is not available; parse the file and call "
-              "gbr_add_file_data()");
+  imp::File f;
+  std::string err;
+  if (!imp::load_gltf(filename, f, err))
+    return fail(GBR_ERR_IO, std::string("Renderer::addFile(): can't open the file ") + filename +
+                                ": " + err);
+  const gbr_file_desc d = f.desc();
+  return gbr_add_file_data(r, &d, flags, name, filename);
 }
 
 // Renderer::addFile(), Renderer.cpp:354-867, on a parsed file
@@ -1481,8 +1490,20 @@ gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
     out->items_phase1 = c[kCtrList1 * kCtrStride];
     out->items_candidates = c[kCtrCand * kCtrStride];
     out->items_phase2 = r->tun.cull_mode >= GBR_CULL_HIZ ? c[kCtrList2 * kCtrStride] : 0;
+    out->triangles_clipped = c[kCtrClipped * kCtrStride];
+    out->clip_fan_triangles = c[kCtrSubTris * kCtrStride];
   } else {
     out->items_phase1 = r->n_items;
+  }
+  if (r->d_counters.ptr && r->tun.cull_mode == GBR_CULL_NONE) {
+    uint32_t c[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&c[0], r->d_counters.ptr + kCtrClipped * kCtrStride, 4,
+                           hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipMemcpyAsync(&c[1], r->d_counters.ptr + kCtrSubTris * kCtrStride, 4,
+                           hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    out->triangles_clipped = c[0];
+    out->clip_fan_triangles = c[1];
   }
   return GBR_OK;
 }

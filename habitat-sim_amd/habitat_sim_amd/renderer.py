@@ -171,7 +171,20 @@ class RendererStandalone:
 
     # -- assets ----------------------------------------------------------------
     def add_file(self, filename: str, flags: int = 0, name: Optional[str] = None) -> bool:
-        """Renderer::addFile(); returns False if the file can't be imported."""
+        """Renderer::addFile() through the library's native glTF importer;
+        returns False if the file can't be imported (as the reference)."""
+        st = N.lib().gbr_add_file(self._h, filename.encode(), flags,
+                                  name.encode() if name else None)
+        if st in (N.ERR_IO, N.ERR_STATE, N.ERR_ARGUMENT):
+            import sys
+            print(N.lib().gbr_last_error().decode(), file=sys.stderr)
+            return False
+        N.check(st)
+        return True
+
+    def add_file_python(self, filename: str, flags: int = 0, name: Optional[str] = None) -> bool:
+        """Renderer::addFile() through the Python importer (assets.load_gltf)
+        and gbr_add_file_data(); same result as add_file()."""
         try:
             fd = load_gltf(filename)
         except (ImportError_, OSError, ValueError, KeyError) as e:

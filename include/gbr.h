@@ -148,6 +148,8 @@ typedef struct gbr_frame_stats_info {
   uint64_t vertices_phase2;
   uint64_t raster_pixels;     /* pixel centres tested against binned triangles */
   uint64_t fragments;         /* of which inside their triangle */
+  uint64_t triangles_clipped; /* triangles through the near/far/guard-band clipper */
+  uint64_t clip_fan_triangles; /* front-facing triangles of their clipped fans */
 } gbr_frame_stats_info;
 
 /* ---- parsed-file description consumed by gbr_add_file_data() (new) ---- */
@@ -225,8 +227,12 @@ gbr_status gbr_get_config(const gbr_renderer* r, gbr_config* out);
 gbr_status gbr_scene_count(const gbr_renderer* r, uint64_t* out);
 
 /* ---- assets: addFile() Renderer.h:517-556 / Renderer.cpp:354-867 ---------- */
-/* Parses glTF 2.0 (+MAGNUMX_mesh_views, KHR_texture_ktx) natively is not yet
-   provided: returns GBR_ERR_UNSUPPORTED; use gbr_add_file_data(). */
+/* Parses a glTF 2.0 / glb composite natively (MAGNUMX_mesh_views extras,
+   KHR_texture_ktx, KHR_texture_transform, KHR_materials_unlit; KTX2 RGBA8/
+   RGB8/RG8/R8 and BC1, PNG images; data URIs and glb chunks) with the
+   conventions of the reference's Magnum importers, then applies addFile()'s
+   logic as gbr_add_file_data().  A file that can't be read or parsed returns
+   GBR_ERR_IO (the reference's `false`). */
 gbr_status gbr_add_file(gbr_renderer* r, const char* filename, uint32_t file_flags,
                         const char* name);
 /* new: addFile() on an already-parsed file; `filename` is the name used for

@@ -61,12 +61,16 @@ def _walk(spec, steps, seed):
     return out
 
 
-@pytest.mark.parametrize("mode,raster_mode", [(N.CULL_FRUSTUM, 0), (N.CULL_HIZ, 0), (N.CULL_HIZ, 1)])
-def test_apartment_walk_matches_unculled(mode, raster_mode):
+@pytest.mark.parametrize("mode,tuning", [
+    (N.CULL_FRUSTUM, {}), (N.CULL_HIZ, {}), (N.CULL_HIZ, dict(raster_mode=1)),
+    # bins overflow: the exact rescan also builds the depth pyramid
+    (N.CULL_HIZ, dict(min_bin_capacity=64, bin_capacity_factor=0.02)),
+    (N.CULL_HIZ, dict(bin_size=16))])
+def test_apartment_walk_matches_unculled(mode, tuning):
     spec = scenes.apartments(n_envs=48)
     plain = _renderer(spec, N.CULL_NONE)
     culled = _renderer(spec, mode)
-    culled.set_tuning(raster_mode=raster_mode)
+    culled.set_tuning(**tuning)
     walk = _walk(spec, 6, seed=11)
     total = skipped = 0
     for k, views in enumerate(walk):
@@ -78,6 +82,8 @@ def test_apartment_walk_matches_unculled(mode, raster_mode):
         st = culled.frame_stats()
         total += st["work_items"]
         skipped += st["work_items"] - st["items_phase1"] - st["items_phase2"]
+        if "bin_capacity_factor" in tuning:
+            assert st["overflowed_bins"] > 0
     # culling is actually exercised
     assert skipped > 0.3 * total
     plain.close()

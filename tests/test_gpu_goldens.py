@@ -1,6 +1,9 @@
 """The HIP renderer reproduces the reference's GfxBatchRendererTest goldens
 and API expectations through the C ABI, and matches the oracle bit for bit on
 every golden scene."""
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -140,3 +143,28 @@ def test_api_errors():
     assert not r.add_file(S.SCENES + "/batch.gltf")  # duplicate names -> false
     assert not r.add_file("/nonexistent.gltf")
     r.close()
+
+
+@pytest.mark.parametrize("gltf", sorted(os.path.basename(p) for p in glob.glob(S.SCENES + "/*.gltf")))
+def test_native_importer_matches_python_importer(gltf):
+    """gbr_add_file (native glTF/KTX2/PNG import) renders every reference
+    fixture exactly like the Python importer + gbr_add_file_data."""
+    from habitat_sim_amd import math3d as m3
+    outs, oks = [], []
+    for how in ("native", "python"):
+        r = make((64, 64), (1, 1))
+        ok = (r.add_file if how == "native" else r.add_file_python)(S.SCENES + "/" + gltf,
+                                                                    flags=1, name="f")
+        oks.append(ok)
+        if ok:
+            P, V = S.ortho_4_3()
+            r.update_camera(0, P, V)
+            r.add_node_hierarchy(0, "f", m3.scaling(0.5))
+            r.draw()
+            outs.append((r.color_image(), r.depth_image(), r.semantic_image()))
+        r.close()
+    assert oks[0] == oks[1]
+    if oks[0]:
+        for a, b in zip(*outs):
+            np.testing.assert_array_equal(a, b)
+        assert (outs[0][2] != 0).any()  # something was drawn
