@@ -3,6 +3,7 @@
 // as vectors.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace gbr {
@@ -72,31 +73,54 @@ struct alignas(16) WorkItem {
 };
 
 // Per-primitive records written by setup for triangles that reach a bin,
-// indexed by (scene record base + primitive order).  Bin entries are the
-// primitive order, or kSubTri | index of a clipped fan triangle in the
-// sub-triangle buffer.
+// indexed by (scene record base + primitive order), in two arrays so that
+// the rasteriser and the resolve pass each read one aligned record.  Bin
+// entries are the primitive order, or kSubTri | index of a clipped fan
+// triangle in the sub-triangle buffers.
 constexpr uint32_t kSubTri = 0x80000000u;
 
-struct alignas(16) RasterRec {
-  int32_t X0, Y0, X1, Y1;
-  int32_t X2, Y2;
-  uint32_t prim;    // primitive order
-  uint32_t draw;    // global draw index
-  float z0, dzdx, dzdy;
-  float zlo;        // smallest vertex depth: fragments never land in front of it
+// Raster part (32 bytes): the snapped window coordinates X0 Y0 X1 Y1 X2 Y2
+// (sub-pixel units) as 24-bit two's complement fields at bit offsets 0, 24,
+// ..., 120 of xy[], and the vertices' window depths.  The depth plane is
+// recomputed from these where it is used.
+struct alignas(32) RasterRec {
+  uint32_t xy[5];
+  float z[3];
 };
 
-// A primitive's record: the raster part (snapped window coordinates and
-// depth plane) followed by the resolve part (clip-space x, y, w and the
-// texture coordinates of the unclipped triangle).
-struct alignas(16) PrimRec {
-  RasterRec r;
+// Resolve part (64 bytes): clip-space x, y, w and texture coordinates of the
+// unclipped triangle, and its work item (whose draw gives the material).
+struct alignas(64) ResolveRec {
   float cx0, cy0, cw0, cx1;
   float cy1, cw1, cx2, cy2;
   float cw2, s0, t0, s1;
   float t1, s2, t2;
-  uint32_t item;    // work item (chunk) of the primitive
+  uint32_t item;
 };
+
+// 24-bit fields of RasterRec::xy; window coordinates stay within the guard
+// band, |X|, |Y| < 2^23
+__host__ __device__ inline void pack_xy(const int32_t* X, const int32_t* Y, uint32_t* xy) {
+  const uint32_t v[6] = {(uint32_t)X[0] & 0xFFFFFFu, (uint32_t)Y[0] & 0xFFFFFFu,
+                         (uint32_t)X[1] & 0xFFFFFFu, (uint32_t)Y[1] & 0xFFFFFFu,
+                         (uint32_t)X[2] & 0xFFFFFFu, (uint32_t)Y[2] & 0xFFFFFFu};
+  xy[0] = v[0] | (v[1] << 24);
+  xy[1] = (v[1] >> 8) | (v[2] << 16);
+  xy[2] = (v[2] >> 16) | (v[3] << 8);
+  xy[3] = v[4] | (v[5] << 24);
+  xy[4] = v[5] >> 8;
+}
+
+__host__ __device__ inline int32_t sext24(uint32_t v) { return (int32_t)(v << 8) >> 8; }
+
+__host__ __device__ inline void unpack_xy(const uint32_t* xy, int32_t* X, int32_t* Y) {
+  X[0] = sext24(xy[0]);
+  Y[0] = sext24((xy[0] >> 24) | (xy[1] << 8));
+  X[1] = sext24((xy[1] >> 16) | (xy[2] << 16));
+  Y[1] = sext24(xy[2] >> 8);
+  X[2] = sext24(xy[3]);
+  Y[2] = sext24((xy[3] >> 24) | (xy[4] << 8));
+}
 
 // A primitive that needs clipping (crosses the near/far plane or the guard band).
 struct alignas(16) PrimRef {
@@ -200,8 +224,10 @@ struct FrameParams {
   // binning
   uint32_t* bin_count;
   uint32_t* bin_entries;
-  PrimRec* recs;                // per primitive, indexed by rec_base + prim
+  RasterRec* rrecs;             // per primitive, indexed by rec_base + prim
+  ResolveRec* srecs;            // per primitive, indexed by rec_base + prim
   RasterRec* sub_recs;          // clipped fan triangles
+  uint32_t* sub_prims;          // their primitive order
   PrimRef* clipped;             // primitives that need clipping
   uint32_t* counters;           // kCtr*
   unsigned long long* vis;      // per-pixel visibility keys, scene planes

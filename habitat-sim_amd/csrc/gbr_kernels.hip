@@ -504,13 +504,13 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
           }
         }
         if (active) {
-          PrimRec rec;
-          rec.r.X0 = X[0]; rec.r.Y0 = Y[0]; rec.r.X1 = X[1]; rec.r.Y1 = Y[1];
-          rec.r.X2 = X[2]; rec.r.Y2 = Y[2];
-          rec.r.prim = prim;
-          rec.r.draw = wi.draw;
-          depth_plane(X, Y, Z, rec.r.z0, rec.r.dzdx, rec.r.dzdy);
-          rec.r.zlo = min_depth(Z);
+          RasterRec rr;
+          pack_xy(X, Y, rr.xy);
+          rr.z[0] = Z[0];
+          rr.z[1] = Z[1];
+          rr.z[2] = Z[2];
+          p.rrecs[(size_t)sc.rec_base + prim] = rr;
+          ResolveRec rec;
           const uint32_t l0 = vi[0] - wi.vmin, l1 = vi[1] - wi.vmin, l2 = vi[2] - wi.vmin;
           rec.cx0 = s_cx[l0]; rec.cy0 = s_cy[l0]; rec.cw0 = s_cw[l0];
           rec.cx1 = s_cx[l1]; rec.cy1 = s_cy[l1]; rec.cw1 = s_cw[l1];
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
           rec.s1 = s_s[l1]; rec.t1 = s_t[l1];
           rec.s2 = s_s[l2]; rec.t2 = s_t[l2];
           rec.item = item;
-          p.recs[(size_t)sc.rec_base + prim] = rec;
+          p.srecs[(size_t)sc.rec_base + prim] = rec;
           // count into the LDS per-bin counters; pairs beyond the LDS list
           // go straight to the global bin
           for (int by = pr.y0 >> sh; by <= (pr.y1 >> sh); ++by)
@@ -677,6 +677,58 @@ __device__ __forceinline__ void raster_tri(const RasterCtx& rc, const int32_t* X
   float z0, dzdx, dzdy;
   depth_plane(X, Y, Z, z0, dzdx, dzdy);
   raster_planed(rc, X, Y, z0, dzdx, dzdy, min_depth(Z), prim);
+}
+
+constexpr int kLaneRaster = 4;  // hybrid mode: rectangles a lane walks alone
+
+// hybrid mode: the wave's medium triangles (rectangles of 5..64 pixels), one
+// per iteration, each lane testing one pixel of it; the triangle is
+// broadcast from its lane with readlane (all lanes of the wave call this)
+__device__ void wave_medium(const RasterCtx& rc, bool medium, const int32_t* X, const int32_t* Y,
+                            float z0, float dzdx, float dzdy, float zlo, uint32_t prim,
+                            const PixRect& r) {
+  unsigned long long m = __ballot(medium);
+  const int lane = threadIdx.x & 63;
+  while (m) {
+    const int j = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    int32_t tX[3], tY[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      tX[q] = __builtin_amdgcn_readlane(X[q], j);
+      tY[q] = __builtin_amdgcn_readlane(Y[q], j);
+    }
+    const float tz0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(z0), j));
+    const float tdx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dzdx), j));
+    const float tdy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dzdy), j));
+    const float tlo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zlo), j));
+    const uint32_t tprim = (uint32_t)__builtin_amdgcn_readlane((int)prim, j);
+    const int x0 = __builtin_amdgcn_readlane(r.x0, j), y0 = __builtin_amdgcn_readlane(r.y0, j);
+    const int w = __builtin_amdgcn_readlane(r.x1, j) - x0 + 1;
+    const int npx = w * (__builtin_amdgcn_readlane(r.y1, j) - y0 + 1);
+    if (lane == 0) *rc.pairs += (uint32_t)npx;
+    if (lane >= npx) continue;
+    const int ly = lane / w;
+    const int px = x0 + lane - ly * w, py = y0 + ly;
+    bool in = true;
+    if (small_box(tX, tY)) {
+      const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int k = (i + 1) % 3;
+        const int32_t dx = tX[k] - tX[i], dy = tY[k] - tY[i];
+        const int32_t bias = (dy < 0 || (dy == 0 && dx > 0)) ? 0 : -1;
+        in = in && (dx * (Py - tY[i]) - dy * (Px - tX[i]) + bias >= 0);
+      }
+    } else {
+      TriSetup s;
+      setup_edges_only(tX, tY, s);
+      in = covers(s, tX, tY, (long long)px * 256 + 128, (long long)py * 256 + 128);
+    }
+    if (!in) continue;
+    ++*rc.frags;
+    depth_key_min(rc, tz0, tdx, tdy, tlo, tX, tY, tprim, px, py);
+  }
 }
 
 __device__ void raster_big(const RasterCtx& rc, const BigTri& b) {
@@ -1151,27 +1203,25 @@ __global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p, int phase)
       const uint32_t slot = atomicAdd(&p.counters[kCtrSubTris * kCtrStride], 1u);
       if (slot < p.sub_cap) {
         RasterRec rr;
-        rr.X0 = tx[0]; rr.Y0 = ty[0]; rr.X1 = tx[1]; rr.Y1 = ty[1]; rr.X2 = tx[2]; rr.Y2 = ty[2];
-        rr.prim = ref.prim;
-        rr.draw = ref.draw;
-        depth_plane(tx, ty, tz, rr.z0, rr.dzdx, rr.dzdy);
-        rr.zlo = min_depth(tz);
+        pack_xy(tx, ty, rr.xy);
+        rr.z[0] = tz[0];
+        rr.z[1] = tz[1];
+        rr.z[2] = tz[2];
         p.sub_recs[slot] = rr;
+        p.sub_prims[slot] = ref.prim;
         bin_rect_atomic(p, sc, r, kSubTri | slot);
       } else {
         overflow_rect(p, sc, r);
       }
     }
     if (any) {
-      // resolve part of the primitive's record (the raster part stays unused:
-      // its fan triangles carry their own)
+      // resolve record of the primitive (its fan triangles carry their own
+      // raster records)
       const MaterialDesc& mat = p.materials[d.material];
       float tm[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) tm[q] = mat.texmat[q];
-      PrimRec rec = {};
-      rec.r.prim = ref.prim;
-      rec.r.draw = ref.draw;
+      ResolveRec rec;
       rec.item = ref.item;
       rec.cx0 = c[0].x; rec.cy0 = c[0].y; rec.cw0 = c[0].w;
       rec.cx1 = c[1].x; rec.cy1 = c[1].y; rec.cw1 = c[1].w;
@@ -1179,7 +1229,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p, int phase)
       vertex_uv(p, d.flags, ip[0], tm, rec.s0, rec.t0);
       vertex_uv(p, d.flags, ip[1], tm, rec.s1, rec.t1);
       vertex_uv(p, d.flags, ip[2], tm, rec.s2, rec.t2);
-      p.recs[(size_t)sc.rec_base + ref.prim] = rec;
+      p.srecs[(size_t)sc.rec_base + ref.prim] = rec;
     }
   }
 }
@@ -1207,6 +1257,19 @@ __device__ __forceinline__ BinInfo bin_info(const FrameParams& p, const RasterIt
   b.gbin = b.sc.bin_base + it.bin;
   b.scene = it.scene;
   return b;
+}
+
+// a bin entry's triangle: window coordinates, depth plane (same op order as
+// everywhere) and nearest vertex depth; returns its primitive order
+__device__ __forceinline__ uint32_t load_raster_rec(const FrameParams& p, const SceneDesc& sc,
+                                                    uint32_t e, int32_t* X, int32_t* Y, float& z0,
+                                                    float& dzdx, float& dzdy, float& zlo) {
+  const bool sub = (e & kSubTri) != 0;
+  const RasterRec rr = sub ? p.sub_recs[e & ~kSubTri] : p.rrecs[(size_t)sc.rec_base + e];
+  unpack_xy(rr.xy, X, Y);
+  depth_plane(X, Y, rr.z, z0, dzdx, dzdy);
+  zlo = min_depth(rr.z);
+  return sub ? p.sub_prims[e & ~kSubTri] : e;
 }
 
 __device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b,
@@ -1355,16 +1418,25 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
   __syncthreads();
   const uint32_t* entries =
       p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
+  const bool hybrid = p.raster_mode == 2;
   for (uint32_t base = 0; base < n; base += kBlock) {
     const uint32_t k = base + threadIdx.x;
+    int32_t X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
+    float z0 = 0.0f, dzdx = 0.0f, dzdy = 0.0f, zlo = 0.0f;
+    uint32_t prim = 0;
+    PixRect r = {0, 0, -1, -1};
+    bool medium = false;
     if (k < n) {
       const uint32_t e = entries[k];
-      const RasterRec rr = (e & kSubTri) ? p.sub_recs[e & ~kSubTri]
-                                         : p.recs[(size_t)b.sc.rec_base + e].r;
-      const int32_t X[3] = {rr.X0, rr.X1, rr.X2};
-      const int32_t Y[3] = {rr.Y0, rr.Y1, rr.Y2};
-      raster_planed(rc, X, Y, rr.z0, rr.dzdx, rr.dzdy, rr.zlo, rr.prim);
+      prim = load_raster_rec(p, b.sc, e, X, Y, z0, dzdx, dzdy, zlo);
+      if (hybrid) {
+        r = tri_pixels(X, Y, b.bin);
+        const int npx = (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
+        medium = r.x0 <= r.x1 && r.y0 <= r.y1 && npx > kLaneRaster && npx <= 64;
+      }
+      if (!medium) raster_planed(rc, X, Y, z0, dzdx, dzdy, zlo, prim);
     }
+    if (hybrid) wave_medium(rc, medium, X, Y, z0, dzdx, dzdy, zlo, prim, r);
     __syncthreads();
     const int nb = min(s_nbig, kBigQueue);
     for (int q = 0; q < nb; ++q) raster_big(rc, s_big[q]);
@@ -1426,10 +1498,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
     uint32_t npx = 0;
     if (k < n) {
       const uint32_t e = entries[k];
-      const RasterRec rr = (e & kSubTri) ? p.sub_recs[e & ~kSubTri]
-                                         : p.recs[(size_t)b.sc.rec_base + e].r;
-      const int32_t X[3] = {rr.X0, rr.X1, rr.X2};
-      const int32_t Y[3] = {rr.Y0, rr.Y1, rr.Y2};
+      int32_t X[3], Y[3];
+      float z0, dzdx, dzdy, zlo;
+      const uint32_t prim = load_raster_rec(p, b.sc, e, X, Y, z0, dzdx, dzdy, zlo);
       const PixRect r = tri_pixels(X, Y, b.bin);
       if (r.x0 <= r.x1 && r.y0 <= r.y1) {
         const int w = r.x1 - r.x0 + 1;
@@ -1440,11 +1511,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
           t.X[q] = X[q];
           t.Y[q] = Y[q];
         }
-        t.z0 = rr.z0;
-        t.dzdx = rr.dzdx;
-        t.dzdy = rr.dzdy;
-        t.zlo = rr.zlo;
-        t.prim = rr.prim;
+        t.z0 = z0;
+        t.dzdx = dzdx;
+        t.dzdy = dzdy;
+        t.zlo = zlo;
+        t.prim = prim;
         t.x0 = (int16_t)r.x0;
         t.y0 = (int16_t)r.y0;
         t.w = (uint16_t)w;
@@ -1569,7 +1640,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
         valid ? p.vis[(size_t)sc.plane_offset + (size_t)py * sc.width + px] : kClearKey;
     const bool drawn = key != kClearKey;
     const uint32_t prim = (uint32_t)key;
-    PrimRec rs;
+    ResolveRec rs;
     DrawDesc d;
     MaterialDesc mat;
     TriAttr A;
@@ -1577,10 +1648,13 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     float s = 0.0f, t = 0.0f;
     bool textured = false;
     if (drawn) {
-      rs = p.recs[(size_t)sc.rec_base + prim];
-      d = p.draws[rs.r.draw];
+      rs = p.srecs[(size_t)sc.rec_base + prim];
+      // every drawn primitive's record was written this frame; the clamp
+      // only keeps the lookups in bounds
+      const uint32_t item = min(rs.item, p.n_items - 1u);
+      d = p.draws[p.items[item].draw];
       mat = p.materials[d.material];
-      if (p.item_vis && rs.item < p.n_items) p.item_vis[rs.item] = p.frame;  // feedback for the next frame
+      if (p.item_vis) p.item_vis[item] = p.frame;  // visibility feedback for the next frame
       A.cx[0] = rs.cx0; A.cy[0] = rs.cy0; A.cw[0] = rs.cw0;
       A.cx[1] = rs.cx1; A.cy[1] = rs.cy1; A.cw[1] = rs.cw1;
       A.cx[2] = rs.cx2; A.cy[2] = rs.cy2; A.cw[2] = rs.cw2;
@@ -2039,7 +2113,7 @@ hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
-  if (p.raster_mode == 1)
+  if (p.raster_mode == 1)  // (2 = size classes inside k_raster_lane)
     hipLaunchKernelGGL(k_raster_pairs, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   else
     hipLaunchKernelGGL(k_raster_lane, dim3(n), dim3(kBlock), 0, s, p, items, phase);

@@ -211,8 +211,10 @@ struct gbr_renderer {
   DevBuf<float> d_local, d_abs;
   DevBuf<float> d_cameras, d_unproj;
   DevBuf<uint32_t> d_bin_count, d_bin_entries;
-  DevBuf<PrimRec> d_recs;
+  DevBuf<RasterRec> d_rrecs;
+  DevBuf<ResolveRec> d_srecs;
   DevBuf<RasterRec> d_sub_recs;
+  DevBuf<uint32_t> d_sub_prims;
   DevBuf<PrimRef> d_clipped;
   DevBuf<uint32_t> d_counters;
   DevBuf<unsigned long long> d_vis;
@@ -480,13 +482,15 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   r->n_bins = bins;
   TRY(r->d_bin_entries.reserve(std::max<uint64_t>(entries, 1)));
   r->entries_cap = entries;
-  TRY(r->d_recs.reserve(std::max<uint64_t>(tris, 1)));
+  TRY(r->d_rrecs.reserve(std::max<uint64_t>(tris, 1)));
+  TRY(r->d_srecs.reserve(std::max<uint64_t>(tris, 1)));
   // every primitive is listed at most once; the sub-triangle pool is bounded
   // (bins of an unstored fan fall back to the exact rescan)
   r->clipped_cap = (uint32_t)std::max<uint64_t>(tris, 1);
   r->sub_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(tris, 1024), 1u << 21);
   TRY(r->d_clipped.reserve(r->clipped_cap));
   TRY(r->d_sub_recs.reserve(r->sub_cap));
+  TRY(r->d_sub_prims.reserve(r->sub_cap));
   HIP_TRY(hipStreamSynchronize(s));
   r->structure_dirty = false;
   return GBR_OK;
@@ -608,8 +612,10 @@ gbr_status do_draw(gbr_renderer* r) {
   p.abs_nodes = r->d_abs.ptr;
   p.bin_count = r->d_bin_count.ptr;
   p.bin_entries = r->d_bin_entries.ptr;
-  p.recs = r->d_recs.ptr;
+  p.rrecs = r->d_rrecs.ptr;
+  p.srecs = r->d_srecs.ptr;
   p.sub_recs = r->d_sub_recs.ptr;
+  p.sub_prims = r->d_sub_prims.ptr;
   p.clipped = r->d_clipped.ptr;
   p.counters = r->d_counters.ptr;
   p.vis = r->d_vis.ptr;
@@ -648,7 +654,7 @@ gbr_status do_draw(gbr_renderer* r) {
       TRY(r->d_hiz.reserve((size_t)p.hiz_stride * std::max<uint32_t>(r->scene_count(), 1)));
       p.hiz = r->d_hiz.ptr;
       // the lane rasteriser builds the pyramid from its 32x32 LDS tiles
-      p.hiz_fused = (r->tun.bin_size == 32 && r->tun.raster_mode == 0) ? 1 : 0;
+      p.hiz_fused = (r->tun.bin_size == 32 && r->tun.raster_mode != 1) ? 1 : 0;
     }
   }
   p.sub_cap = r->sub_cap;
@@ -841,8 +847,8 @@ static gbr_status validate_tuning(const gbr_tuning* t) {
     return fail(GBR_ERR_ARGUMENT, "tuning: chunk_triangles must be in [1, 1024]");
   if (!(t->bin_capacity_factor > 0.0f) || t->min_bin_capacity < 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: capacity must be positive");
-  if (t->raster_mode != 0 && t->raster_mode != 1)
-    return fail(GBR_ERR_ARGUMENT, "tuning: raster_mode must be 0 or 1");
+  if (t->raster_mode < 0 || t->raster_mode > 2)
+    return fail(GBR_ERR_ARGUMENT, "tuning: raster_mode must be 0, 1 or 2");
   if (t->timing != 0 && t->timing != 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: timing must be 0 or 1");
   if (t->queue_grain < 1 || t->queue_grain > 64)
@@ -924,7 +930,9 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   r->d_textures.release();
   r->d_materials.release();
   r->d_draws.release();
-  r->d_recs.release();
+  r->d_rrecs.release();
+  r->d_srecs.release();
+  r->d_sub_prims.release();
   r->d_sub_recs.release();
   r->d_clipped.release();
   r->d_counters.release();

@@ -114,7 +114,8 @@ typedef struct gbr_tuning {
   int32_t queue_grain;        /* setup work items claimed per queue fetch: 1..64 */
   int32_t timing;             /* 1: time each stage of gbr_draw() with HIP events */
   int32_t raster_mode;        /* 0: a lane per binned triangle, 1: a lane per
-                                 (triangle, pixel) pair; same output */
+                                 (triangle, pixel) pair, 2: by rectangle size
+                                 (lane / wave / workgroup); same output */
   float proposal_threshold;   /* GBR_CULL_LEARNED: phase 1 = chunks whose logit
                                  exceeds this */
 } gbr_tuning;
