@@ -1245,6 +1245,21 @@ gbr_status gbr_add_node_hierarchy(gbr_renderer* r, uint32_t scene, const char* n
   return gbr_add_node_hierarchy_semantic(r, scene, name, bake, 0u, out_node);
 }
 
+gbr_status gbr_set_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, uint64_t node,
+                                           uint32_t semantic) {
+  TRY(check_scene(r, scene, "setNodeHierarchySemantic()"));
+  Scene& sc = r->scenes[scene];
+  if (node >= sc.parents.size())
+    return fail(GBR_ERR_RANGE, "setNodeHierarchySemantic(): node " + std::to_string(node) +
+                                   " out of range for " + std::to_string(sc.parents.size()) +
+                                   " nodes");
+  // the hierarchy's draws sit on the top-level node's children
+  for (Draw& d : sc.draws)
+    if (d.node == node || sc.parents[d.node] == (int32_t)node) d.semantic = semantic;
+  r->structure_dirty = true;
+  return GBR_OK;
+}
+
 gbr_status gbr_add_empty_node(gbr_renderer* r, uint32_t scene, uint64_t* out_node) {
   TRY(check_scene(r, scene, "addEmptyNode()"));
   Scene& sc = r->scenes[scene];

@@ -181,6 +181,7 @@ SIGNATURES = {
     "gbr_set_tuning": (S, [R, ctypes.POINTER(Tuning)]),
     "gbr_frame_stats": (S, [R, ctypes.POINTER(FrameStatsInfo)]),
     "gbr_frame_timings": (S, [R, ctypes.POINTER(ctypes.c_float)]),
+    "gbr_set_node_hierarchy_semantic": (S, [R, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32]),
     "gbr_vis_fit_params_init": (S, [ctypes.POINTER(VisFitParams)]),
     "gbr_vis_init": (S, [R, ctypes.c_uint64]),
     "gbr_vis_collect": (S, [R, ctypes.POINTER(ctypes.c_uint64)]),

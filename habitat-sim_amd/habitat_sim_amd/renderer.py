@@ -290,6 +290,10 @@ class RendererStandalone:
                None if b is None else b.ctypes.data, semantic_id, ctypes.byref(out))
         return out.value
 
+    def set_node_hierarchy_semantic(self, scene: int, node: int, semantic_id: int) -> None:
+        """Semantic id of every draw of the hierarchy under top-level `node`."""
+        N.call("gbr_set_node_hierarchy_semantic", self._h, scene, node, semantic_id)
+
     def add_empty_node(self, scene: int) -> int:
         out = ctypes.c_uint64()
         N.call("gbr_add_empty_node", self._h, scene, ctypes.byref(out))

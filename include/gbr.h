@@ -253,6 +253,11 @@ gbr_status gbr_add_node_hierarchy(gbr_renderer* r, uint32_t scene, const char* n
 gbr_status gbr_add_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, const char* name,
                                            const float* bake, uint32_t semantic_id,
                                            uint64_t* out_node);
+/* new: sets the semantic id of every draw of the hierarchy whose top-level
+   node is `node` (a gfx-replay instance's metadata, which the reference's
+   BatchPlayerImplementation drops, BatchPlayerImplementation.cpp:51-91) */
+gbr_status gbr_set_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, uint64_t node,
+                                           uint32_t semantic_id);
 /* addEmptyNode() Renderer.h:598 */
 gbr_status gbr_add_empty_node(gbr_renderer* r, uint32_t scene, uint64_t* out_node);
 /* addLight() Renderer.h:619-621: needs max_light_count > 0 (not supported) */

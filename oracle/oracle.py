@@ -315,6 +315,14 @@ class OracleRenderer:
             sc.draws.append(_Draw(b, nid, mat, mesh, off, cnt, semantic_id))
         return top
 
+    def set_node_hierarchy_semantic(self, scene_id: int, node: int, semantic_id: int) -> None:
+        """Semantic id of the draws under top-level `node` (the renderer's
+        gbr_set_node_hierarchy_semantic; no reference counterpart)."""
+        sc = self.scenes[scene_id]
+        for d in sc.draws:
+            if d.node == node or sc.parents[d.node] == node:
+                d.semantic = semantic_id
+
     def add_empty_node(self, scene_id: int) -> int:
         sc = self.scenes[scene_id]
         sc.parents.append(-1)

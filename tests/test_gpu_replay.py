@@ -52,3 +52,30 @@ def test_render_and_observations_match_oracle():
     with pytest.raises(ValueError):
         rr.render(colors[:2], [])
     rr.close()
+
+
+def test_keyframe_player_on_the_gpu_matches_oracle():
+    """The gfx-replay deletion sequence played into the HIP renderer and into
+    the oracle: identical node layouts and identical pixels per keyframe."""
+    from habitat_sim_amd import RendererConfiguration, RendererStandalone
+    from habitat_sim_amd.replay_player import BatchPlayer
+    from test_replay_player import deletion_keyframes
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count((64, 64), (1, 1)))
+    o = OracleRenderer((64, 64), (1, 1))
+    P = m3.orthographic_projection((6.0, 6.0), 0.1, 10.0)
+    V = m3.inverted_rigid(m3.translation((0.5, 0.5, 5.0)))
+    players = []
+    for x in (r, o):
+        x.update_camera(0, P, V)
+        p = BatchPlayer(x, 0)
+        for k in deletion_keyframes():
+            p.append_keyframe(k)
+        players.append(p)
+    for idx in (0, 1, 2, 0):
+        for p in players:
+            p.set_keyframe_index(idx)
+        r.draw()
+        o.draw()
+        np.testing.assert_array_equal(r.color_image(), o.color_image(), err_msg=f"keyframe {idx}")
+        np.testing.assert_array_equal(r.depth_image(), o.depth_image(), err_msg=f"keyframe {idx}")
+    r.close()
