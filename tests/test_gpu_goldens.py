@@ -159,7 +159,7 @@ def test_native_importer_matches_python_importer(gltf):
         if ok:
             P, V = S.ortho_4_3()
             r.update_camera(0, P, V)
-            r.add_node_hierarchy(0, "f", m3.scaling(0.5))
+            r.add_node_hierarchy(0, "f", m3.scaling(0.5), semantic_id=1)
             r.draw()
             outs.append((r.color_image(), r.depth_image(), r.semantic_image()))
         r.close()
