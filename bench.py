@@ -193,10 +193,16 @@ def main():
     ap.add_argument("--train-frames", type=int, default=64,
                     help="frames of training rows for the visibility model")
     ap.add_argument("--train-steps", type=int, default=600)
+    ap.add_argument("--modes", default=",".join(m for m, _ in MODES),
+                    help="culling modes to time, comma-separated (profiling runs time "
+                         "fewer; the speedup needs plain and the headline mode)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (the reference kernel "
                          "for scaling hardware byte counters) and exit")
     args = ap.parse_args()
+    modes = [(m, c) for m, c in MODES if m in args.modes.split(",")]
+    if not modes:
+        ap.error(f"--modes: none of {[m for m, _ in MODES]}")
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "environments shard across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
@@ -223,29 +229,36 @@ def main():
 
     # the visibility model trains on its own walks (random starts, other
     # seeds) before the learned mode is timed
-    from habitat_sim_amd import visibility
-    t0 = time.perf_counter()
-    training = visibility.train(r, spec, frames=args.train_frames, steps=args.train_steps, seed=3)
-    training["seconds"] = time.perf_counter() - t0
-    training["frames"] = args.train_frames
-    training["poses"] = args.train_frames * len(spec.envs)
-    training["steps"] = args.train_steps
+    training = None
+    if any(m == "learned" for m, _ in modes):
+        from habitat_sim_amd import visibility
+        t0 = time.perf_counter()
+        training = visibility.train(r, spec, frames=args.train_frames, steps=args.train_steps,
+                                    seed=3)
+        training["seconds"] = time.perf_counter() - t0
+        training["frames"] = args.train_frames
+        training["poses"] = args.train_frames * len(spec.envs)
+        training["steps"] = args.train_steps
+        print(f"bench: visibility model trained in {training['seconds']:.1f} s", file=sys.stderr,
+              flush=True)
     runs = {}
-    for name, mode in MODES:
+    for name, mode in modes:
         runs[name] = run_mode(r, mode, spec, views, proj, args.warmup, args.steps)
-    base = runs["plain"]
-    head = runs[HEADLINE]
+        print(f"bench: {name} {runs[name]['gpu_ms']:.3f} ms/frame", file=sys.stderr, flush=True)
+    base = runs.get("plain")
+    headline = HEADLINE if HEADLINE in runs else modes[-1][0]
+    head = runs[headline]
     n_items = head["stats"]["work_items"]
     n_nodes = sum(2 + len(env.instances) for env in spec.envs)  # roots + children (upper bound)
 
     cull = {}
-    for name, _ in MODES:
+    for name, _ in modes:
         x = runs[name]
         cull[name] = {
             "env_frames_per_s": n * args.steps / x["wall"],
             "gpu_ms_per_frame": x["gpu_ms"],
-            "speedup_vs_plain": base["wall"] / x["wall"],
-            "diff_pixels_vs_plain": diff_pixels(x["planes"], base["planes"]),
+            "speedup_vs_plain": base["wall"] / x["wall"] if base else None,
+            "diff_pixels_vs_plain": diff_pixels(x["planes"], base["planes"]) if base else None,
             "items_phase1": x["stats"]["items_phase1"],
             "items_phase2": x["stats"]["items_phase2"],
             "triangles_setup": x["stats"]["triangles_setup"],
@@ -257,7 +270,7 @@ def main():
     # stream): the frame formula's bytes over the frame's GPU time
     stages = head["stages"]
     dom = max(stages, key=lambda k: stages[k])
-    learned = HEADLINE == "learned"
+    learned = headline == "learned"
     bytes_frame = sum(stage_bytes(k, spec, head["stats"], n_items, n_nodes, learned)
                       for k in stages)
     frame_ms = head["gpu_ms"]
@@ -267,7 +280,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        traffic = tj.get("frame_bytes", {}).get(HEADLINE)
+        traffic = tj.get("frame_bytes", {}).get(headline)
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(spec, args.cpu_seconds)
     value = n * args.steps / head["wall"]
@@ -288,7 +301,7 @@ def main():
                    "envs": n, "resolution": [args.res, args.res],
                    "triangles_per_frame": int(head["stats"]["triangles"]),
                    "sensors": ["rgba8", "linear_depth_f32", "semantic_u32"],
-                   "cull": HEADLINE, "parallelism": "single-gpu"},
+                   "cull": headline, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "frame", "algorithmic_bytes_per_launch": bytes_frame,
@@ -301,7 +314,7 @@ def main():
                                  head["stats"]["triangles_setup"] / (frame_ms * 1e-3)}},
         "cpu_baseline": cpu,
         "batch_steps_per_s": value / n,
-        "learned_cull_speedup": cull["learned"]["speedup_vs_plain"],
+        "learned_cull_speedup": cull["learned"]["speedup_vs_plain"] if "learned" in cull else None,
         "vis_training": training,
         "cull": cull,
         "frame_stats": head["stats"],

@@ -14,7 +14,7 @@ constexpr int kChunkTris = 1024;   // triangles per setup work item (4 per lane)
 constexpr int kVertexCache = 640;  // vertices per setup work item held in LDS
 constexpr int kMaxSceneBins = 1024;  // bins per scene (LDS counters in setup, <= 2^11)
 constexpr int kPairCap = 2048;     // (bin, triangle) pairs aggregated in LDS per item (<= 2^11)
-constexpr int kBigQueue = 192;     // cooperative-raster queue entries per bin
+constexpr int kBigQueue = 128;     // cooperative-raster queue entries per bin
 constexpr int kDrawCache = 1024;   // draw prim bases cached in LDS per bin
 constexpr float kSubpix = 256.0f;  // 8 sub-pixel bits
 constexpr float kGuardPx = 8192.0f;
@@ -179,10 +179,18 @@ enum : int32_t { kCullNone = 0, kCullFrustum = 1, kCullHiZ = 2, kCullLearned = 3
 constexpr int kHiZ0 = 3, kHiZ1 = 5;  // log2 block edges
 
 
+// One (scene, bin) of the raster and resolve grids, with the scene fields
+// those kernels need, so that a workgroup's first load is its last
+// dependent one before the bin's entries.
 struct alignas(16) RasterItem {
   uint32_t scene;
-  uint32_t bin;         // scene-local bin index
-  uint32_t pad[2];
+  uint32_t gbin;            // global bin index (scene bin_base + bin)
+  int16_t x0, y0, x1, y1;   // the bin's pixels (inclusive)
+  uint32_t bin_cap;         // entries per bin of the scene
+  uint32_t plane_offset;    // first pixel of the scene's planes
+  int32_t width, height;    // scene size in pixels
+  unsigned long long entries;   // the bin's first entry in bin_entries
+  unsigned long long rec_base;  // the scene's first primitive record
 };
 
 struct alignas(16) MaterialDesc {
@@ -249,6 +257,7 @@ struct FrameParams {
   uint32_t hiz_stride;
   float proposal_threshold;
   int32_t raster_mode;  // 0: a lane per triangle, 1: a lane per (triangle, pixel)
+  int32_t lane_pixels;  // mode 0: rectangles above this go to the workgroup queue
   int32_t hiz_fused;    // phase-1 raster writes the depth pyramid (32-pixel bins)
   // learned proposal: feature rows [kCtrProp count][32] bf16 and their items
   const float* cam_info;   // [scene][8]: camera position xyz, forward xyz

@@ -41,7 +41,6 @@ namespace gbr {
 
 constexpr unsigned long long kClearKey = 0x3F800000FFFFFFFFull;  // depth 1.0, no primitive
 constexpr uint32_t kClearColor = 0xFF1F1F1Fu;                      // 0x1f1f1f, alpha 255
-constexpr int kSmallRaster = 48;  // pixels a single lane rasterises alone
 
 // --------------------------------------------------------------------------
 // Arithmetic shared by setup, raster and resolve (op order = the oracle's)
@@ -167,6 +166,39 @@ __device__ int clip_poly(float4* poly, int n, float gx, float gy) {
     if (n < 3) return 0;
     for (int i = 0; i < n; ++i) poly[i] = tmp[i];
   }
+  return n;
+}
+
+// The same clipper over polygons kept in LDS, vertex i of the calling thread
+// at poly[i * S]: the dynamically indexed arrays stay out of scratch memory.
+// Ping-pongs between poly and tmp; returns the vertex count and the buffer
+// that holds the result in *out.
+template <int S>
+__device__ int clip_poly_lds(float4* poly, float4* tmp, int n, float gx, float gy, float4** out) {
+  for (int plane = 0; plane < 6; ++plane) {
+    int m = 0;
+    const float4 first = poly[0];
+    const float dfirst = plane_dist(first, plane, gx, gy);
+    float4 a = first;
+    float da = dfirst;
+    for (int i = 0; i < n; ++i) {
+      const bool wrap = i + 1 == n;
+      const float4 b = wrap ? first : poly[(i + 1) * S];
+      const float db = wrap ? dfirst : plane_dist(b, plane, gx, gy);
+      const bool ina = da >= 0.0f, inb = db >= 0.0f;
+      if (ina && m < kMaxPoly) tmp[(m++) * S] = a;
+      if (ina != inb && m < kMaxPoly)
+        tmp[(m++) * S] = ina ? lerp_from(a, b, da, db) : lerp_from(b, a, db, da);
+      a = b;
+      da = db;
+    }
+    n = m;
+    if (n < 3) return 0;
+    float4* t = poly;
+    poly = tmp;
+    tmp = t;
+  }
+  *out = poly;
   return n;
 }
 
@@ -573,11 +605,19 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
 // Rasterisation helpers: LDS visibility tile with 64-bit atomicMin keys
 // --------------------------------------------------------------------------
 
+// A triangle the whole workgroup rasterises, set up once by the lane that
+// queued it: its three edge functions (fill-rule bias included) at the bin's
+// first pixel centre, exact in 64 bits, and their steps per pixel right and
+// down; narrow = every value over the bin (and a bin past it) fits in 32 bits.
 struct BigTri {
-  int32_t X[3], Y[3];
+  long long e[3];
+  int32_t sx[3], sy[3];  // -dy * 256, dx * 256
   float z0, dzdx, dzdy, zlo;
+  int32_t X0, Y0;        // vertex 0, the depth plane's origin
   uint32_t prim;
-  PixRect r;
+  uint8_t x0, y0, x1, y1;  // covered pixel rectangle, relative to the bin
+  uint16_t npx;            // its pixel count
+  uint16_t narrow;
 };
 
 struct RasterCtx {
@@ -586,6 +626,7 @@ struct RasterCtx {
   int* nbig;
   PixRect bin;
   int B;
+  int lane_limit;       // rectangles above this many pixels go to the queue
   uint32_t* pairs;      // per-lane counters: pixels tested, fragments covered
   uint32_t* frags;
 };
@@ -608,24 +649,109 @@ __device__ __forceinline__ void depth_key_min(const RasterCtx& rc, float z0, flo
   }
 }
 
+// queue entry of a triangle with a known depth plane and pixel rectangle r
+// (inside the bin)
+__device__ __forceinline__ void make_big(const RasterCtx& rc, const int32_t* X, const int32_t* Y,
+                                         float z0, float dzdx, float dzdy, float zlo,
+                                         uint32_t prim, const PixRect& r, int npx, BigTri& b) {
+  const long long Px0 = (long long)rc.bin.x0 * 256 + 128, Py0 = (long long)rc.bin.y0 * 256 + 128;
+  bool narrow = true;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3;
+    const int32_t dx = X[j] - X[i], dy = Y[j] - Y[i];
+    const long long bias = (dy < 0 || (dy == 0 && dx > 0)) ? 0 : -1;
+    const long long e = (long long)dx * (Py0 - Y[i]) - (long long)dy * (Px0 - X[i]) + bias;
+    // window coordinates stay within the guard band (|X|, |Y| < 2^22), so
+    // |d| * 256 < 2^31
+    b.e[i] = e;
+    b.sx[i] = -dy * 256;
+    b.sy[i] = dx * 256;
+    const long long reach = (llabs((long long)dx) + llabs((long long)dy)) * 256 * 64;
+    narrow = narrow && llabs(e) + reach < (1ll << 31);
+  }
+  b.z0 = z0; b.dzdx = dzdx; b.dzdy = dzdy; b.zlo = zlo;
+  b.X0 = X[0];
+  b.Y0 = Y[0];
+  b.prim = prim;
+  b.x0 = (uint8_t)(r.x0 - rc.bin.x0);
+  b.y0 = (uint8_t)(r.y0 - rc.bin.y0);
+  b.x1 = (uint8_t)(r.x1 - rc.bin.x0);
+  b.y1 = (uint8_t)(r.y1 - rc.bin.y0);
+  b.npx = (uint16_t)npx;
+  b.narrow = narrow ? 1 : 0;
+}
+
+// The queued triangles, by the whole workgroup: each thread owns the pixels
+// (lx, ly0 + k * step) of the bin and tests them against every triangle, with
+// the edge values stepped by additions (exact: 32-bit when narrow).
+__device__ void raster_big_all(const RasterCtx& rc, const BigTri* big, int nb) {
+  const int B = rc.B;
+  const int lx = (int)threadIdx.x % B, ly0 = (int)threadIdx.x / B, step = kBlock / B;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < nb; ++q) *rc.pairs += big[q].npx;
+  if (ly0 >= B) return;  // (8-pixel bins: the first 64 threads cover the bin)
+  const int px = rc.bin.x0 + lx;
+  const long long Px = (long long)px * 256 + 128;
+  for (int q = 0; q < nb; ++q) {
+    const BigTri& t = big[q];
+    if (lx < t.x0 || lx > t.x1) continue;
+    const int ylo = t.y0, yhi = t.y1;
+    if (t.narrow) {
+      uint32_t e[3], d[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        e[i] = (uint32_t)t.e[i] + (uint32_t)t.sx[i] * (uint32_t)lx + (uint32_t)t.sy[i] * (uint32_t)ly0;
+        d[i] = (uint32_t)t.sy[i] * (uint32_t)step;
+      }
+      for (int ly = ly0; ly < B; ly += step) {
+        if (ly >= ylo && ly <= yhi && (int32_t)(e[0] | e[1] | e[2]) >= 0) {
+          ++*rc.frags;
+          const int py = rc.bin.y0 + ly;
+          const float z = depth_at(t.z0, t.dzdx, t.dzdy, t.zlo, t.X0, t.Y0, Px,
+                                   (long long)py * 256 + 128);
+          if (z < 1.0f)
+            atomicMin(&rc.vis[ly * B + lx], ((unsigned long long)__float_as_uint(z) << 32) | t.prim);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) e[i] += d[i];
+      }
+    } else {
+      long long e[3], d[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        e[i] = t.e[i] + (long long)t.sx[i] * lx + (long long)t.sy[i] * ly0;
+        d[i] = (long long)t.sy[i] * step;
+      }
+      for (int ly = ly0; ly < B; ly += step) {
+        if (ly >= ylo && ly <= yhi && (e[0] | e[1] | e[2]) >= 0) {
+          ++*rc.frags;
+          const int py = rc.bin.y0 + ly;
+          const float z = depth_at(t.z0, t.dzdx, t.dzdy, t.zlo, t.X0, t.Y0, Px,
+                                   (long long)py * 256 + 128);
+          if (z < 1.0f)
+            atomicMin(&rc.vis[ly * B + lx], ((unsigned long long)__float_as_uint(z) << 32) | t.prim);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) e[i] += d[i];
+      }
+    }
+  }
+}
+
 // rasterise one front-facing triangle with a known depth plane: this lane
 // walks the clipped rectangle row by row with edge functions stepped by
-// additions; rectangles above kSmallRaster pixels go to the block's
-// cooperative queue
+// additions; rectangles above rc.lane_limit pixels go to the workgroup's
+// queue (a full queue leaves them to the lane)
 __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32_t* Y, float z0,
                               float dzdx, float dzdy, float zlo, uint32_t prim) {
   PixRect r = tri_pixels(X, Y, rc.bin);
   if (r.x0 > r.x1 || r.y0 > r.y1) return;
   const int npx = (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
-  if (npx > kSmallRaster) {
+  if (npx > rc.lane_limit) {
     const int slot = atomicAdd(rc.nbig, 1);
     if (slot < kBigQueue) {
-      BigTri& b = rc.big[slot];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) { b.X[k] = X[k]; b.Y[k] = Y[k]; }
-      b.z0 = z0; b.dzdx = dzdx; b.dzdy = dzdy; b.zlo = zlo;
-      b.prim = prim;
-      b.r = r;
+      make_big(rc, X, Y, z0, dzdx, dzdy, zlo, prim, r, npx, rc.big[slot]);
       return;
     }
   }
@@ -677,94 +803,6 @@ __device__ __forceinline__ void raster_tri(const RasterCtx& rc, const int32_t* X
   float z0, dzdx, dzdy;
   depth_plane(X, Y, Z, z0, dzdx, dzdy);
   raster_planed(rc, X, Y, z0, dzdx, dzdy, min_depth(Z), prim);
-}
-
-constexpr int kLaneRaster = 4;  // hybrid mode: rectangles a lane walks alone
-
-// hybrid mode: the wave's medium triangles (rectangles of 5..64 pixels), one
-// per iteration, each lane testing one pixel of it; the triangle is
-// broadcast from its lane with readlane (all lanes of the wave call this)
-__device__ void wave_medium(const RasterCtx& rc, bool medium, const int32_t* X, const int32_t* Y,
-                            float z0, float dzdx, float dzdy, float zlo, uint32_t prim,
-                            const PixRect& r) {
-  unsigned long long m = __ballot(medium);
-  const int lane = threadIdx.x & 63;
-  while (m) {
-    const int j = __ffsll((long long)m) - 1;
-    m &= m - 1;
-    int32_t tX[3], tY[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      tX[q] = __builtin_amdgcn_readlane(X[q], j);
-      tY[q] = __builtin_amdgcn_readlane(Y[q], j);
-    }
-    const float tz0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(z0), j));
-    const float tdx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dzdx), j));
-    const float tdy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dzdy), j));
-    const float tlo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zlo), j));
-    const uint32_t tprim = (uint32_t)__builtin_amdgcn_readlane((int)prim, j);
-    const int x0 = __builtin_amdgcn_readlane(r.x0, j), y0 = __builtin_amdgcn_readlane(r.y0, j);
-    const int w = __builtin_amdgcn_readlane(r.x1, j) - x0 + 1;
-    const int npx = w * (__builtin_amdgcn_readlane(r.y1, j) - y0 + 1);
-    if (lane == 0) *rc.pairs += (uint32_t)npx;
-    if (lane >= npx) continue;
-    const int ly = lane / w;
-    const int px = x0 + lane - ly * w, py = y0 + ly;
-    bool in = true;
-    if (small_box(tX, tY)) {
-      const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const int k = (i + 1) % 3;
-        const int32_t dx = tX[k] - tX[i], dy = tY[k] - tY[i];
-        const int32_t bias = (dy < 0 || (dy == 0 && dx > 0)) ? 0 : -1;
-        in = in && (dx * (Py - tY[i]) - dy * (Px - tX[i]) + bias >= 0);
-      }
-    } else {
-      TriSetup s;
-      setup_edges_only(tX, tY, s);
-      in = covers(s, tX, tY, (long long)px * 256 + 128, (long long)py * 256 + 128);
-    }
-    if (!in) continue;
-    ++*rc.frags;
-    depth_key_min(rc, tz0, tdx, tdy, tlo, tX, tY, tprim, px, py);
-  }
-}
-
-__device__ void raster_big(const RasterCtx& rc, const BigTri& b) {
-  const int w = b.r.x1 - b.r.x0 + 1;
-  const int n = w * (b.r.y1 - b.r.y0 + 1);
-  *rc.pairs += (uint32_t)((n - (int)threadIdx.x + kBlock - 1) / kBlock);
-  if (small_box(b.X, b.Y)) {
-    int32_t dx[3], dy[3], bias[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const int j = (i + 1) % 3;
-      dx[i] = b.X[j] - b.X[i];
-      dy[i] = b.Y[j] - b.Y[i];
-      bias[i] = (dy[i] < 0 || (dy[i] == 0 && dx[i] > 0)) ? 0 : -1;
-    }
-    for (int k = threadIdx.x; k < n; k += kBlock) {
-      const int px = b.r.x0 + k % w, py = b.r.y0 + k / w;
-      const int32_t Px = px * 256 + 128, Py = py * 256 + 128;
-      int32_t e[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) e[i] = dx[i] * (Py - b.Y[i]) - dy[i] * (Px - b.X[i]) + bias[i];
-      if ((e[0] | e[1] | e[2]) < 0) continue;
-      ++*rc.frags;
-      depth_key_min(rc, b.z0, b.dzdx, b.dzdy, b.zlo, b.X, b.Y, b.prim, px, py);
-    }
-    return;
-  }
-  TriSetup s;
-  setup_edges_only(b.X, b.Y, s);
-  for (int k = threadIdx.x; k < n; k += kBlock) {
-    const int px = b.r.x0 + k % w, py = b.r.y0 + k / w;
-    const long long Px = (long long)px * 256 + 128, Py = (long long)py * 256 + 128;
-    if (!covers(s, b.X, b.Y, Px, Py)) continue;
-    ++*rc.frags;
-    depth_key_min(rc, b.z0, b.dzdx, b.dzdy, b.zlo, b.X, b.Y, b.prim, px, py);
-  }
 }
 
 __device__ __forceinline__ uint32_t find_draw(const FrameParams& p, const SceneDesc& sc,
@@ -1060,11 +1098,10 @@ struct QuadST {
   bool h00, h10, h01;
 };
 
-__device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, const SceneDesc& sc,
+__device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw, float hh,
                                const TriAttr& A, const float* b, float s, float t,
                                const MaterialDesc& mat, bool textured, const QuadST& q,
                                const DrawDesc& d, uint32_t prim, int px, int py) {
-  const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
   const float ihw = 1.0f / hw, ihh = 1.0f / hh;
   float c[4] = {mat.ambient[0], mat.ambient[1], mat.ambient[2], mat.ambient[3]};
   if (textured) {
@@ -1137,15 +1174,26 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, const Sc
 // each binned on its own (entry = kSubTri | record).
 // --------------------------------------------------------------------------
 
+// The bins of a rectangle in groups of 8 whose counter atomics are all in
+// flight together (each returns the entry's slot; taken one at a time, their
+// round trips add up along a thread).
 __device__ __forceinline__ void bin_rect_atomic(const FrameParams& p, const SceneDesc& sc,
                                                 const PixRect& r, uint32_t entry) {
   const int sh = p.bin_shift;
-  for (int by = r.y0 >> sh; by <= (r.y1 >> sh); ++by)
-    for (int bx = r.x0 >> sh; bx <= (r.x1 >> sh); ++bx) {
-      const uint32_t b = (uint32_t)(by * sc.bins_x + bx);
-      const uint32_t slot = atomicAdd(&p.bin_count[sc.bin_base + b], 1u);
-      if (slot < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b * sc.bin_cap + slot] = entry;
+  const int bx0 = r.x0 >> sh, by0 = r.y0 >> sh;
+  const int nx = (r.x1 >> sh) - bx0 + 1, nb = nx * ((r.y1 >> sh) - by0 + 1);
+  for (int j0 = 0; j0 < nb; j0 += 8) {
+    uint32_t b[8], slot[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const int j = j0 + g;
+      b[g] = (uint32_t)((by0 + j / nx) * sc.bins_x + bx0 + j % nx);
+      slot[g] = j < nb ? atomicAdd(&p.bin_count[sc.bin_base + b[g]], 1u) : sc.bin_cap;
     }
+#pragma unroll
+    for (int g = 0; g < 8; ++g)
+      if (slot[g] < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b[g] * sc.bin_cap + slot[g]] = entry;
+  }
 }
 
 // forces the exact rescan for every bin the rectangle touches
@@ -1158,63 +1206,107 @@ __device__ __forceinline__ void overflow_rect(const FrameParams& p, const SceneD
 }
 
 // phase 1 takes the clipped list from its start and leaves its end in
-// kCtrClipMark; phase 2 continues from there
-__global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p, int phase) {
+// kCtrClipMark; phase 2 continues from there.  One thread per primitive;
+// its polygon and projected fan vertices live in LDS.
+constexpr int kClipBlock = 64;
+
+__global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int phase) {
+  __shared__ float4 s_poly[2][kMaxPoly * kClipBlock];
+  __shared__ int32_t s_px[kMaxPoly * kClipBlock], s_py[kMaxPoly * kClipBlock];
+  __shared__ float s_pz[kMaxPoly * kClipBlock];
   const uint32_t n = min(p.counters[kCtrClipped * kCtrStride], p.clipped_cap);
   const uint32_t start = phase == 1 ? 0u : p.counters[kCtrClipMark * kCtrStride];
   if (phase == 1 && blockIdx.x == 0 && threadIdx.x == 0) p.counters[kCtrClipMark * kCtrStride] = n;
-  for (uint32_t i = start + blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const PrimRef ref = p.clipped[i];
-    const DrawDesc d = p.draws[ref.draw];
-    const SceneDesc sc = p.scenes[d.scene];
-    float M[16], PV[16];
-    load_mat(p.abs_nodes + (size_t)d.node * 16, M);
-    load_mat(p.cameras + (size_t)d.scene * 16, PV);
-    const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
-    const float gx = kGuardPx / hw, gy = kGuardPx / hh;
-    const uint32_t* ip = p.indices + d.index_base + 3 * (ref.prim - d.prim_base);
-    float4 c[3], poly[kMaxPoly];
+  const int tid = threadIdx.x;
+  int32_t* PX = s_px + tid;
+  int32_t* PY = s_py + tid;
+  float* PZ = s_pz + tid;
+  // wave-uniform trip count: the fan triangles of all lanes take their
+  // records with one counter atomic per wave
+  for (uint32_t base = start + blockIdx.x * kClipBlock; base < n; base += gridDim.x * kClipBlock) {
+    const uint32_t i = base + tid;
+    uint32_t fans = 0;  // bit q: fan triangle (0, q, q+1) is front-facing and covers a pixel
+    PrimRef ref = {};
+    DrawDesc d = {};
+    SceneDesc sc = {};
+    float4 c[3] = {};
+    const uint32_t* ip = nullptr;
+    if (i < n) {
+      ref = p.clipped[i];
+      d = p.draws[ref.draw];
+      sc = p.scenes[d.scene];
+      float M[16], PV[16];
+      load_mat(p.abs_nodes + (size_t)d.node * 16, M);
+      load_mat(p.cameras + (size_t)d.scene * 16, PV);
+      const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
+      const float gx = kGuardPx / hw, gy = kGuardPx / hh;
+      ip = p.indices + d.index_base + 3 * (ref.prim - d.prim_base);
+      float4* poly = s_poly[0] + tid;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const uint32_t v = ip[q];
-      c[q] = vertex_clip(M, PV, p.pos_x[v], p.pos_y[v], p.pos_z[v]);
-      poly[q] = c[q];
-    }
-    const int nv = clip_poly(poly, 3, gx, gy);
-    if (nv < 3) continue;
-    int32_t PX[kMaxPoly], PY[kMaxPoly];
-    float PZ[kMaxPoly];
-    bool ok = true;
-    for (int q = 0; q < nv && ok; ++q) {
-      if (!(poly[q].w > 0.0f)) ok = false;
-      else project(poly[q], hw, hh, PX[q], PY[q], PZ[q]);
-    }
-    if (!ok) continue;
-    const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
-    bool any = false;
-    for (int q = 1; q + 1 < nv; ++q) {
-      const int32_t tx[3] = {PX[0], PX[q], PX[q + 1]};
-      const int32_t ty[3] = {PY[0], PY[q], PY[q + 1]};
-      const float tz[3] = {PZ[0], PZ[q], PZ[q + 1]};
-      if (tri_area(tx, ty) <= 0) continue;
-      const PixRect r = tri_pixels(tx, ty, tile);
-      if (r.x0 > r.x1 || r.y0 > r.y1) continue;
-      any = true;
-      const uint32_t slot = atomicAdd(&p.counters[kCtrSubTris * kCtrStride], 1u);
-      if (slot < p.sub_cap) {
-        RasterRec rr;
-        pack_xy(tx, ty, rr.xy);
-        rr.z[0] = tz[0];
-        rr.z[1] = tz[1];
-        rr.z[2] = tz[2];
-        p.sub_recs[slot] = rr;
-        p.sub_prims[slot] = ref.prim;
-        bin_rect_atomic(p, sc, r, kSubTri | slot);
-      } else {
-        overflow_rect(p, sc, r);
+      for (int q = 0; q < 3; ++q) {
+        const uint32_t v = ip[q];
+        c[q] = vertex_clip(M, PV, p.pos_x[v], p.pos_y[v], p.pos_z[v]);
+        poly[q * kClipBlock] = c[q];
+      }
+      const int nv = clip_poly_lds<kClipBlock>(poly, s_poly[1] + tid, 3, gx, gy, &poly);
+      bool ok = nv >= 3;
+      for (int q = 0; q < nv && ok; ++q) {
+        const float4 v = poly[q * kClipBlock];
+        if (!(v.w > 0.0f)) {
+          ok = false;
+        } else {
+          int32_t x, y;
+          float z;
+          project(v, hw, hh, x, y, z);
+          PX[q * kClipBlock] = x;
+          PY[q * kClipBlock] = y;
+          PZ[q * kClipBlock] = z;
+        }
+      }
+      const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
+      for (int q = 1; ok && q + 1 < nv; ++q) {
+        const int a = q * kClipBlock, b = (q + 1) * kClipBlock;
+        const int32_t tx[3] = {PX[0], PX[a], PX[b]};
+        const int32_t ty[3] = {PY[0], PY[a], PY[b]};
+        if (tri_area(tx, ty) <= 0) continue;
+        const PixRect r = tri_pixels(tx, ty, tile);
+        if (r.x0 > r.x1 || r.y0 > r.y1) continue;
+        fans |= 1u << q;
       }
     }
-    if (any) {
+    // exclusive wave prefix of the fan counts, one atomic for the wave
+    const uint32_t cnt = (uint32_t)__popc(fans);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(incl, o);
+      if (tid >= o) incl += t;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t wbase = 0;
+    if (tid == 63 && total) wbase = atomicAdd(&p.counters[kCtrSubTris * kCtrStride], total);
+    uint32_t slot = __shfl(wbase, 63) + incl - cnt;
+    if (fans) {
+      const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
+      for (uint32_t f = fans; f; f &= f - 1u, ++slot) {
+        const int q = __ffs(f) - 1;
+        const int a = q * kClipBlock, b = (q + 1) * kClipBlock;
+        const int32_t tx[3] = {PX[0], PX[a], PX[b]};
+        const int32_t ty[3] = {PY[0], PY[a], PY[b]};
+        const PixRect r = tri_pixels(tx, ty, tile);
+        if (slot < p.sub_cap) {
+          RasterRec rr;
+          pack_xy(tx, ty, rr.xy);
+          rr.z[0] = PZ[0];
+          rr.z[1] = PZ[a];
+          rr.z[2] = PZ[b];
+          p.sub_recs[slot] = rr;
+          p.sub_prims[slot] = ref.prim;
+          bin_rect_atomic(p, sc, r, kSubTri | slot);
+        } else {
+          overflow_rect(p, sc, r);
+        }
+      }
       // resolve record of the primitive (its fan triangles carry their own
       // raster records)
       const MaterialDesc& mat = p.materials[d.material];
@@ -1239,37 +1331,53 @@ __global__ __launch_bounds__(kBlock) void k_setup_clip(FrameParams p, int phase)
 // --------------------------------------------------------------------------
 
 struct BinInfo {
-  SceneDesc sc;
   PixRect bin;
-  uint32_t gbin;
-  uint32_t scene;
+  uint32_t gbin, scene, bin_cap, plane_offset;
+  int width, height;
+  unsigned long long entries, rec_base;
 };
 
-__device__ __forceinline__ BinInfo bin_info(const FrameParams& p, const RasterItem& it) {
+__device__ __forceinline__ BinInfo bin_info(const RasterItem& it) {
   BinInfo b;
-  b.sc = p.scenes[it.scene];
-  const int B = p.bin_size;
-  const int bx = (int)(it.bin % (uint32_t)b.sc.bins_x), by = (int)(it.bin / (uint32_t)b.sc.bins_x);
-  b.bin.x0 = bx * B;
-  b.bin.y0 = by * B;
-  b.bin.x1 = min(bx * B + B, b.sc.width) - 1;
-  b.bin.y1 = min(by * B + B, b.sc.height) - 1;
-  b.gbin = b.sc.bin_base + it.bin;
+  b.bin.x0 = it.x0;
+  b.bin.y0 = it.y0;
+  b.bin.x1 = it.x1;
+  b.bin.y1 = it.y1;
+  b.gbin = it.gbin;
   b.scene = it.scene;
+  b.bin_cap = it.bin_cap;
+  b.plane_offset = it.plane_offset;
+  b.width = it.width;
+  b.height = it.height;
+  b.entries = it.entries;
+  b.rec_base = it.rec_base;
   return b;
 }
 
 // a bin entry's triangle: window coordinates, depth plane (same op order as
 // everywhere) and nearest vertex depth; returns its primitive order
-__device__ __forceinline__ uint32_t load_raster_rec(const FrameParams& p, const SceneDesc& sc,
-                                                    uint32_t e, int32_t* X, int32_t* Y, float& z0,
-                                                    float& dzdx, float& dzdy, float& zlo) {
+__device__ __forceinline__ uint32_t fetch_raster_rec(const FrameParams& p,
+                                                     unsigned long long rec_base, uint32_t e,
+                                                     RasterRec& rr) {
   const bool sub = (e & kSubTri) != 0;
-  const RasterRec rr = sub ? p.sub_recs[e & ~kSubTri] : p.rrecs[(size_t)sc.rec_base + e];
+  rr = sub ? p.sub_recs[e & ~kSubTri] : p.rrecs[(size_t)rec_base + e];
+  return sub ? p.sub_prims[e & ~kSubTri] : e;
+}
+
+__device__ __forceinline__ void decode_raster_rec(const RasterRec& rr, int32_t* X, int32_t* Y,
+                                                  float& z0, float& dzdx, float& dzdy, float& zlo) {
   unpack_xy(rr.xy, X, Y);
   depth_plane(X, Y, rr.z, z0, dzdx, dzdy);
   zlo = min_depth(rr.z);
-  return sub ? p.sub_prims[e & ~kSubTri] : e;
+}
+
+__device__ __forceinline__ uint32_t load_raster_rec(const FrameParams& p, unsigned long long rec_base,
+                                                    uint32_t e, int32_t* X, int32_t* Y, float& z0,
+                                                    float& dzdx, float& dzdy, float& zlo) {
+  RasterRec rr;
+  const uint32_t prim = fetch_raster_rec(p, rec_base, e, rr);
+  decode_raster_rec(rr, X, Y, z0, dzdx, dzdy, zlo);
+  return prim;
 }
 
 __device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b,
@@ -1277,7 +1385,7 @@ __device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b
   const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
   for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
     const int lx = l % bw, ly = l / bw;
-    p.vis[(size_t)b.sc.plane_offset + (size_t)(b.bin.y0 + ly) * b.sc.width + b.bin.x0 + lx] =
+    p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.width + b.bin.x0 + lx] =
         s_vis[ly * B + lx];
   }
 }
@@ -1378,8 +1486,8 @@ __device__ void bin_hiz(const FrameParams& p, const BinInfo& b, const unsigned l
 
 // --------------------------------------------------------------------------
 // k_raster_lane: per (scene, bin), one binned triangle per lane walking its
-// clipped rectangle; rectangles above kSmallRaster pixels are rasterised by
-// the whole block from an LDS queue.  Phase 1 starts every bin from the
+// clipped rectangle; rectangles above the lane limit (gbr_tuning.lane_pixels)
+// are rasterised by the whole block from an LDS queue.  Phase 1 starts every bin from the
 // clear key; phase 2 merges into the phase-1 keys and leaves bins without
 // new entries untouched.
 // --------------------------------------------------------------------------
@@ -1390,9 +1498,13 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
   __shared__ BigTri s_big[kBigQueue];
   __shared__ int s_nbig;
 
-  const BinInfo b = bin_info(p, items[blockIdx.x]);
+  const BinInfo b = bin_info(items[blockIdx.x]);
+  const uint32_t* entries = p.bin_entries + b.entries;
+  // the first entries load beside the count (both addresses are known; an
+  // entry at or past the count is never used)
+  const uint32_t e_first = threadIdx.x < b.bin_cap ? entries[threadIdx.x] : 0u;
   const uint32_t n = p.bin_count[b.gbin];
-  if (n > b.sc.bin_cap) return;  // overflowed: k_raster_full rescans this bin
+  if (n > b.bin_cap) return;  // overflowed: k_raster_full rescans this bin
   if (phase == 2 && n == 0) return;
   const int B = p.bin_size;
   uint32_t pairs = 0, frags = 0;
@@ -1401,45 +1513,53 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
   rc.big = s_big;
   rc.nbig = &s_nbig;
   rc.B = B;
+  rc.lane_limit = p.lane_pixels;
   rc.bin = b.bin;
   rc.pairs = &pairs;
   rc.frags = &frags;
+  // the first batch's records are in flight while the tile is set up
+  RasterRec rr_first = {};
+  uint32_t prim_first = 0;
+  if (threadIdx.x < n) prim_first = fetch_raster_rec(p, b.rec_base, e_first, rr_first);
+  const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
   if (phase == 2) {
-    const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
-    for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
-      const int lx = l % bw, ly = l / bw;
-      s_vis[ly * B + lx] =
-          p.vis[(size_t)b.sc.plane_offset + (size_t)(b.bin.y0 + ly) * b.sc.width + b.bin.x0 + lx];
+    constexpr int kPer = kMaxBin * kMaxBin / kBlock;
+    unsigned long long keys[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int l = threadIdx.x + q * kBlock;
+      const int ly = l / bw, lx = l - ly * bw;
+      keys[q] = l < bw * bh
+                    ? p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.width + b.bin.x0 + lx]
+                    : 0ull;
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int l = threadIdx.x + q * kBlock;
+      const int ly = l / bw, lx = l - ly * bw;
+      if (l < bw * bh) s_vis[ly * B + lx] = keys[q];
     }
   } else {
     for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
   }
   if (threadIdx.x == 0) s_nbig = 0;
   __syncthreads();
-  const uint32_t* entries =
-      p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
-  const bool hybrid = p.raster_mode == 2;
   for (uint32_t base = 0; base < n; base += kBlock) {
     const uint32_t k = base + threadIdx.x;
-    int32_t X[3] = {0, 0, 0}, Y[3] = {0, 0, 0};
-    float z0 = 0.0f, dzdx = 0.0f, dzdy = 0.0f, zlo = 0.0f;
-    uint32_t prim = 0;
-    PixRect r = {0, 0, -1, -1};
-    bool medium = false;
     if (k < n) {
-      const uint32_t e = entries[k];
-      prim = load_raster_rec(p, b.sc, e, X, Y, z0, dzdx, dzdy, zlo);
-      if (hybrid) {
-        r = tri_pixels(X, Y, b.bin);
-        const int npx = (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
-        medium = r.x0 <= r.x1 && r.y0 <= r.y1 && npx > kLaneRaster && npx <= 64;
+      int32_t X[3], Y[3];
+      float z0, dzdx, dzdy, zlo;
+      uint32_t prim;
+      if (base == 0) {
+        prim = prim_first;
+        decode_raster_rec(rr_first, X, Y, z0, dzdx, dzdy, zlo);
+      } else {
+        prim = load_raster_rec(p, b.rec_base, entries[k], X, Y, z0, dzdx, dzdy, zlo);
       }
-      if (!medium) raster_planed(rc, X, Y, z0, dzdx, dzdy, zlo, prim);
+      raster_planed(rc, X, Y, z0, dzdx, dzdy, zlo, prim);
     }
-    if (hybrid) wave_medium(rc, medium, X, Y, z0, dzdx, dzdy, zlo, prim, r);
     __syncthreads();
-    const int nb = min(s_nbig, kBigQueue);
-    for (int q = 0; q < nb; ++q) raster_big(rc, s_big[q]);
+    raster_big_all(rc, s_big, min(s_nbig, kBigQueue));
     __syncthreads();
     if (threadIdx.x == 0) s_nbig = 0;
     __syncthreads();
@@ -1466,9 +1586,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
   __shared__ uint32_t s_pref[kBlock];
   __shared__ uint32_t s_next;
 
-  const BinInfo b = bin_info(p, items[blockIdx.x]);
+  const BinInfo b = bin_info(items[blockIdx.x]);
   const uint32_t n = p.bin_count[b.gbin];
-  if (n > b.sc.bin_cap) return;  // overflowed: k_raster_full rescans this bin
+  if (n > b.bin_cap) return;  // overflowed: k_raster_full rescans this bin
   if (phase == 2 && n == 0) return;
   const int B = p.bin_size;
   if (phase == 2) {
@@ -1476,7 +1596,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
     for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
       const int lx = l % bw, ly = l / bw;
       s_vis[ly * B + lx] =
-          p.vis[(size_t)b.sc.plane_offset + (size_t)(b.bin.y0 + ly) * b.sc.width + b.bin.x0 + lx];
+          p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.width + b.bin.x0 + lx];
     }
   } else {
     for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
@@ -1487,8 +1607,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
   const int lane = threadIdx.x & 63;
   const int wbase = threadIdx.x & ~63;
   uint32_t pairs = 0, frags = 0;
-  const uint32_t* entries =
-      p.bin_entries + b.sc.entry_base + (size_t)(b.gbin - b.sc.bin_base) * b.sc.bin_cap;
+  const uint32_t* entries = p.bin_entries + b.entries;
   while (true) {
     uint32_t chunk = 0;
     if (lane == 0) chunk = atomicAdd(&s_next, 64u);
@@ -1500,7 +1619,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
       const uint32_t e = entries[k];
       int32_t X[3], Y[3];
       float z0, dzdx, dzdy, zlo;
-      const uint32_t prim = load_raster_rec(p, b.sc, e, X, Y, z0, dzdx, dzdy, zlo);
+      const uint32_t prim = load_raster_rec(p, b.rec_base, e, X, Y, z0, dzdx, dzdy, zlo);
       const PixRect r = tri_pixels(X, Y, b.bin);
       if (r.x0 <= r.x1 && r.y0 <= r.y1) {
         const int w = r.x1 - r.x0 + 1;
@@ -1560,50 +1679,70 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
 }
 
 // exact fallback for bins whose list overflowed: every primitive of the scene
+// One workgroup checks kFullGroup bins and rescans the overflowed ones among
+// them in turn (overflow is rare: most workgroups exit after the check).
+constexpr int kFullGroup = 64;
+
 __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
                                                         const RasterItem* __restrict__ items,
-                                                        int phase) {
+                                                        uint32_t n, int phase) {
   __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
   __shared__ BigTri s_big[kBigQueue];
   __shared__ uint32_t s_base[kDrawCache];
   __shared__ int s_nbig;
+  __shared__ uint32_t s_over[kFullGroup];
+  __shared__ int s_nover;
 
-  const RasterItem it = items[blockIdx.x];
-  const BinInfo b = bin_info(p, it);
-  if (p.bin_count[b.gbin] <= b.sc.bin_cap) return;
+  if (threadIdx.x == 0) s_nover = 0;
+  __syncthreads();
+  const uint32_t first = blockIdx.x * kFullGroup;
+  if (threadIdx.x < kFullGroup && first + threadIdx.x < n) {
+    const RasterItem it = items[first + threadIdx.x];
+    if (p.bin_count[it.gbin] > it.bin_cap)
+      s_over[atomicAdd(&s_nover, 1)] = first + threadIdx.x;
+  }
+  __syncthreads();
+  const int nover = s_nover;
+  if (nover == 0) return;  // (uniform: read after the barrier)
   const int B = p.bin_size;
   uint32_t pairs = 0, frags = 0;
-  RasterCtx rc;
-  rc.vis = s_vis;
-  rc.big = s_big;
-  rc.nbig = &s_nbig;
-  rc.B = B;
-  rc.bin = b.bin;
-  rc.pairs = &pairs;
-  rc.frags = &frags;
-  for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
-  if (b.sc.draw_count <= (uint32_t)kDrawCache)
-    for (uint32_t i = threadIdx.x; i < b.sc.draw_count; i += kBlock)
-      s_base[i] = p.draws[b.sc.draw_begin + i].prim_base;
-  if (threadIdx.x == 0) s_nbig = 0;
-  float PV[16];
-  load_mat(p.cameras + (size_t)it.scene * 16, PV);
-  __syncthreads();
-  for (uint32_t base = 0; base < b.sc.prim_count; base += kBlock) {
-    const uint32_t k = base + threadIdx.x;
-    if (k < b.sc.prim_count) raster_prim(p, b.sc, PV, rc, s_base, k);
-    __syncthreads();
-    const int nb = min(s_nbig, kBigQueue);
-    for (int q = 0; q < nb; ++q) raster_big(rc, s_big[q]);
-    __syncthreads();
+  for (int o = 0; o < nover; ++o) {
+    const RasterItem it = items[s_over[o]];
+    const BinInfo b = bin_info(it);
+    const SceneDesc sc = p.scenes[it.scene];
+    RasterCtx rc;
+    rc.vis = s_vis;
+    rc.big = s_big;
+    rc.nbig = &s_nbig;
+    rc.B = B;
+    rc.lane_limit = p.lane_pixels;
+    rc.bin = b.bin;
+    rc.pairs = &pairs;
+    rc.frags = &frags;
+    for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
+    if (sc.draw_count <= (uint32_t)kDrawCache)
+      for (uint32_t i = threadIdx.x; i < sc.draw_count; i += kBlock)
+        s_base[i] = p.draws[sc.draw_begin + i].prim_base;
     if (threadIdx.x == 0) s_nbig = 0;
+    float PV[16];
+    load_mat(p.cameras + (size_t)it.scene * 16, PV);
     __syncthreads();
-  }
-  store_vis(p, b, s_vis, B);
-  if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
-  if (threadIdx.x == 0) {
-    add_stat(p.stats, 1, 1ull);
-    p.bin_count[b.gbin] = 0;
+    for (uint32_t base = 0; base < sc.prim_count; base += kBlock) {
+      const uint32_t k = base + threadIdx.x;
+      if (k < sc.prim_count) raster_prim(p, sc, PV, rc, s_base, k);
+      __syncthreads();
+      raster_big_all(rc, s_big, min(s_nbig, kBigQueue));
+      __syncthreads();
+      if (threadIdx.x == 0) s_nbig = 0;
+      __syncthreads();
+    }
+    store_vis(p, b, s_vis, B);
+    if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
+    if (threadIdx.x == 0) {
+      add_stat(p.stats, 1, 1ull);
+      p.bin_count[b.gbin] = 0;
+    }
+    __syncthreads();  // the next bin reuses the LDS tiles
   }
   add_raster_stats(p, pairs, frags);
 }
@@ -1620,8 +1759,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
   if (threadIdx.x < 256) s_lut[threadIdx.x] = p.unorm_lut[threadIdx.x];
   __syncthreads();
   const RasterItem it = items[blockIdx.x];
-  const BinInfo b = bin_info(p, it);
-  const SceneDesc& sc = b.sc;
+  const BinInfo b = bin_info(it);
   const float un0 = p.unprojections[2 * it.scene], un1 = p.unprojections[2 * it.scene + 1];
   const float far_depth = un1 / (1.0f + un0);
   const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
@@ -1630,14 +1768,14 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
   // of one wave (rows of bw pixels, bw in {8, 16, 32}, even bin origin)
   const bool quads = bw == p.bin_size && bh == p.bin_size;
   const int lane = threadIdx.x & 63;
-  const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
+  const float hw = (float)b.width * 0.5f, hh = (float)b.height * 0.5f;
   const float ihw = 1.0f / hw, ihh = 1.0f / hh;
   for (int base = 0; base < npx; base += kBlock) {
     const int l = base + (int)threadIdx.x;
     const bool valid = l < npx;
     const int px = b.bin.x0 + l % bw, py = b.bin.y0 + l / bw;
     const unsigned long long key =
-        valid ? p.vis[(size_t)sc.plane_offset + (size_t)py * sc.width + px] : kClearKey;
+        valid ? p.vis[(size_t)b.plane_offset + (size_t)py * b.width + px] : kClearKey;
     const bool drawn = key != kClearKey;
     const uint32_t prim = (uint32_t)key;
     ResolveRec rs;
@@ -1648,7 +1786,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     float s = 0.0f, t = 0.0f;
     bool textured = false;
     if (drawn) {
-      rs = p.srecs[(size_t)sc.rec_base + prim];
+      rs = p.srecs[(size_t)b.rec_base + prim];
       // every drawn primitive's record was written this frame; the clamp
       // only keeps the lookups in bounds
       const uint32_t item = min(rs.item, p.n_items - 1u);
@@ -1682,8 +1820,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
       q.h01 = textured && g01 == prim;
     }
     if (!valid) continue;
-    const int row = p.top_down ? (sc.height - 1 - py) : py;
-    const size_t o = (size_t)sc.plane_offset + (size_t)row * sc.width + px;
+    const int row = p.top_down ? (b.height - 1 - py) : py;
+    const size_t o = (size_t)b.plane_offset + (size_t)row * b.width + px;
     if (!drawn) {
       if (p.out_color) p.out_color[o] = kClearColor;
       if (p.out_depth) p.out_depth[o] = 1.0f;
@@ -1693,7 +1831,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     const float z = __uint_as_float((uint32_t)(key >> 32));
     if (p.out_color)
-      p.out_color[o] = shade_pixel(p, s_lut, sc, A, bc, s, t, mat, textured, q, d, prim, px, py);
+      p.out_color[o] = shade_pixel(p, s_lut, hw, hh, A, bc, s, t, mat, textured, q, d, prim, px, py);
     if (p.out_depth) p.out_depth[o] = z;
     if (p.out_lindepth) {
       const float lin = un1 / (z + un0);
@@ -1738,28 +1876,40 @@ __device__ __forceinline__ void abs_bound(const float* M, const float* PV, float
   }
 }
 
-// wave-aggregated append of one item index per flagged lane
-__device__ __forceinline__ void append_item(uint32_t* counter, uint32_t* list, bool flag,
-                                            uint32_t item, int lane, unsigned long long lt_mask) {
-  const unsigned long long m = __ballot(flag);
-  if (m == 0ull) return;
-  const int leader = __ffsll((long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  if (flag) list[base + (uint32_t)__popcll(m & lt_mask)] = item;
-}
+// Block-aggregated appends to L lists: one device atomic per list and block.
+// (Single-address atomics from every wave serialise at the memory side and
+// cost more than the work they count.)  Every thread of the block calls it
+// once, with its flags; slot[l] is valid where flag[l] is set.
+constexpr int kListBlock = 1024;
 
-// wave-aggregated append returning the flagged lane's slot
-__device__ __forceinline__ uint32_t append_slot(uint32_t* counter, bool flag, int lane,
-                                                unsigned long long lt_mask) {
-  const unsigned long long m = __ballot(flag);
-  if (m == 0ull) return 0u;
-  const int leader = __ffsll((long long)m) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(m & lt_mask);
+template <int L>
+__device__ __forceinline__ void block_slots(uint32_t* const (&counter)[L], const bool (&flag)[L],
+                                            uint32_t (&slot)[L]) {
+  __shared__ uint32_t s_ofs[L][kListBlock / 64];
+  __shared__ uint32_t s_base[L];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int waves = (int)(blockDim.x >> 6);
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long m[L];
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    m[l] = __ballot(flag[l]);
+    if (lane == 0) s_ofs[l][wave] = (uint32_t)__popcll(m[l]);
+  }
+  __syncthreads();
+  if (threadIdx.x < L) {
+    const int l = threadIdx.x;
+    uint32_t total = 0;
+    for (int w = 0; w < waves; ++w) {
+      const uint32_t c = s_ofs[l][w];
+      s_ofs[l][w] = total;
+      total += c;
+    }
+    s_base[l] = total ? atomicAdd(counter[l], total) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < L; ++l) slot[l] = s_base[l] + s_ofs[l][wave] + (uint32_t)__popcll(m[l] & lt_mask);
 }
 
 __device__ __forceinline__ uint32_t f2bf_bits(float f) {
@@ -1840,10 +1990,8 @@ __device__ void item_features(const FrameParams& p, const WorkItem& wi, const Sc
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
-  const uint32_t item = blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+__global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
+  const uint32_t item = blockIdx.x * kListBlock + threadIdx.x;
   bool to1 = false, toc = false, feat = false;
   ItemCull icf = {};
   float wmin_f = 0.0f, wmax_f = 0.0f;
@@ -1929,10 +2077,15 @@ __global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
       }
     }
   }
-  append_item(&p.counters[kCtrList1 * kCtrStride], p.list1, to1, item, lane, lt_mask);
-  append_item(&p.counters[kCtrCand * kCtrStride], p.cand, toc, item, lane, lt_mask);
+  uint32_t* const ctr[3] = {&p.counters[kCtrList1 * kCtrStride], &p.counters[kCtrCand * kCtrStride],
+                            &p.counters[kCtrProp * kCtrStride]};
+  const bool flag[3] = {to1, toc, feat};
+  uint32_t slot[3];
+  block_slots<3>(ctr, flag, slot);
+  if (to1) p.list1[slot[0]] = item;
+  if (toc) p.cand[slot[1]] = item;
   if (p.features) {
-    const uint32_t row = append_slot(&p.counters[kCtrProp * kCtrStride], feat, lane, lt_mask);
+    const uint32_t row = slot[2];
     if (feat) {
       p.prop[row] = item;
       const WorkItem wi = p.items[item];
@@ -1947,11 +2100,10 @@ __global__ __launch_bounds__(kBlock) void k_cull(FrameParams p) {
 
 // learned mode: the model's score splits the feature rows into phase 1 and
 // the depth-pyramid candidates
-__global__ __launch_bounds__(kBlock) void k_propose(FrameParams p) {
-  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+__global__ __launch_bounds__(kListBlock) void k_propose(FrameParams p) {
+  const uint32_t row = blockIdx.x * kListBlock + threadIdx.x;
   const uint32_t n = min(p.counters[kCtrProp * kCtrStride], p.n_items);
+  if (blockIdx.x * kListBlock >= n) return;  // (whole block: no barrier is skipped)
   bool to1 = false, toc = false;
   uint32_t item = 0;
   if (row < n) {
@@ -1959,8 +2111,12 @@ __global__ __launch_bounds__(kBlock) void k_propose(FrameParams p) {
     to1 = p.logits[row] > p.proposal_threshold;
     toc = !to1;
   }
-  append_item(&p.counters[kCtrList1 * kCtrStride], p.list1, to1, item, lane, lt_mask);
-  append_item(&p.counters[kCtrCand * kCtrStride], p.cand, toc, item, lane, lt_mask);
+  uint32_t* const ctr[2] = {&p.counters[kCtrList1 * kCtrStride], &p.counters[kCtrCand * kCtrStride]};
+  const bool flag[2] = {to1, toc};
+  uint32_t slot[2];
+  block_slots<2>(ctr, flag, slot);
+  if (to1) p.list1[slot[0]] = item;
+  if (toc) p.cand[slot[1]] = item;
 }
 
 // training rows of this frame: features and whether the item ended up with
@@ -2019,11 +2175,10 @@ __global__ __launch_bounds__(kBlock) void k_hiz(FrameParams p, uint32_t regions_
 }
 
 // candidates against the depth pyramid -> phase-2 list
-__global__ __launch_bounds__(kBlock) void k_cull_hiz(FrameParams p) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+__global__ __launch_bounds__(kListBlock) void k_cull_hiz(FrameParams p) {
+  const uint32_t i = blockIdx.x * kListBlock + threadIdx.x;
   const uint32_t n = min(p.counters[kCtrCand * kCtrStride], p.n_items);
+  if (blockIdx.x * kListBlock >= n) return;  // (whole block: no barrier is skipped)
   bool keep = false;
   uint32_t item = 0;
   if (i < n) {
@@ -2043,7 +2198,11 @@ __global__ __launch_bounds__(kBlock) void k_cull_hiz(FrameParams p) {
       for (int bx = bx0; bx <= bx1; ++bx)
         if (!(ic.zmin > h[by * w + bx])) { keep = true; break; }
   }
-  append_item(&p.counters[kCtrList2 * kCtrStride], p.list2, keep, item, lane, lt_mask);
+  uint32_t* const ctr[1] = {&p.counters[kCtrList2 * kCtrStride]};
+  const bool flag[1] = {keep};
+  uint32_t slot[1];
+  block_slots<1>(ctr, flag, slot);
+  if (keep) p.list2[slot[0]] = item;
 }
 
 // --------------------------------------------------------------------------
@@ -2106,20 +2265,21 @@ hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t lis
 }
 
 hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s) {
-  hipLaunchKernelGGL(k_setup_clip, dim3(grid), dim3(kBlock), 0, s, p, phase);
+  hipLaunchKernelGGL(k_setup_clip, dim3(grid), dim3(kClipBlock), 0, s, p, phase);
   return hipGetLastError();
 }
 
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
-  if (p.raster_mode == 1)  // (2 = size classes inside k_raster_lane)
+  if (p.raster_mode == 1)
     hipLaunchKernelGGL(k_raster_pairs, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   else
     hipLaunchKernelGGL(k_raster_lane, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_raster_full, dim3(n), dim3(kBlock), 0, s, p, items, phase);
+  hipLaunchKernelGGL(k_raster_full, dim3((n + kFullGroup - 1) / kFullGroup), dim3(kBlock), 0, s, p,
+                     items, n, phase);
   return hipGetLastError();
 }
 
@@ -2132,13 +2292,15 @@ hipError_t launch_resolve(const FrameParams& p, const RasterItem* items, uint32_
 
 hipError_t launch_cull(const FrameParams& p, hipStream_t s) {
   if (p.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cull, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(k_cull, dim3((p.n_items + kListBlock - 1) / kListBlock), dim3(kListBlock), 0, s,
+                     p);
   return hipGetLastError();
 }
 
 hipError_t launch_propose(const FrameParams& p, hipStream_t s) {
   if (p.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_propose, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(k_propose, dim3((p.n_items + kListBlock - 1) / kListBlock), dim3(kListBlock), 0, s,
+                     p);
   return hipGetLastError();
 }
 
@@ -2159,7 +2321,8 @@ hipError_t launch_hiz(const FrameParams& p, uint32_t scenes, hipStream_t s) {
 
 hipError_t launch_cull_hiz(const FrameParams& p, hipStream_t s) {
   if (p.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cull_hiz, dim3((p.n_items + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(k_cull_hiz, dim3((p.n_items + kListBlock - 1) / kListBlock), dim3(kListBlock), 0, s,
+                     p);
   return hipGetLastError();
 }
 

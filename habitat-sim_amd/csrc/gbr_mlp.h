@@ -48,6 +48,13 @@ hipError_t mlp_gather(const uint16_t* X, const uint8_t* Y, uint64_t total, uint6
                       hipStream_t s);
 hipError_t mlp_to_bf16(const float* in, size_t n, uint16_t* out, uint16_t* out_t, int rows,
                        int cols, hipStream_t s);
+// logits of rows [0, min(rows, *rows_dev)) of X for rendering, in one fused
+// pass with no intermediates in memory; at most max_blocks workgroups (0: one
+// per 128 rows), each looping over 32-row tiles.  H1 and H2 are the training
+// forward's bit for bit; the logit sums in another order.
+hipError_t mlp_score(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
+                     const float* b2, const uint16_t* w3, const float* b3, int rows,
+                     const uint32_t* rows_dev, float* logits, int max_blocks, hipStream_t s);
 hipError_t mlp_adam(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1,
                     float b2, float eps, float bc1, float bc2, hipStream_t s);
 

@@ -15,6 +15,8 @@
 
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "gbr_mlp.h"
 
 namespace gbr {
@@ -279,6 +281,165 @@ hipError_t mlp_adam(float* w, const float* g, float* m, float* v, size_t n, floa
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, eps,
                      bc1, bc2);
+  return hipGetLastError();
+}
+
+}  // namespace gbr
+
+namespace gbr {
+
+// --------------------------------------------------------------------------
+// Fused scorer for rendering: logits of up to min(rows, *rows_dev) feature
+// rows in one pass, one wave per 32 rows.  Layer 1 (K = 32) and layer 2
+// (K = 256) on v_mfma_f32_32x32x16_bf16 in two halves of 128 columns; H1
+// (bf16, after bias and ReLU) is staged in LDS as the A operand of layer 2;
+// H2 is rounded to bf16 like the training forward, multiplied by w3 and
+// reduced across the 32 lanes that hold a row's columns.
+// --------------------------------------------------------------------------
+
+constexpr int kScoreRows = 32;              // rows per wave and round
+constexpr int kH1Stride = kMlpHidden + 8;   // padded LDS row (bank spread)
+constexpr int kW2Half = kMlpHidden / 2;     // W2 rows staged in LDS at a time
+
+// Per round a workgroup scores 4 tiles of 32 rows (one per wave): layer 1
+// from L2-resident W1 into the wave's H1 tile in LDS, then layer 2 in two
+// halves of 128 output columns, each half of W2 staged in LDS once for the
+// 4 waves; the output layer folds into the layer-2 epilogue.
+__global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ X,
+                                                   const uint16_t* __restrict__ W1,
+                                                   const float* __restrict__ b1,
+                                                   const uint16_t* __restrict__ W2,
+                                                   const float* __restrict__ b2,
+                                                   const uint16_t* __restrict__ w3,
+                                                   const float* __restrict__ b3, int rows,
+                                                   const uint32_t* __restrict__ rows_dev,
+                                                   float* __restrict__ logits) {
+  __shared__ uint16_t s_h1[4][kScoreRows * kH1Stride];
+  __shared__ uint16_t s_w2[kW2Half * kH1Stride];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n_rows = rows_of(rows, rows_dev);
+  const int n_tiles = (n_rows + kScoreRows - 1) / kScoreRows;
+  uint16_t* h1 = s_h1[wave];
+  const int lr = lane & 31, kh = 8 * (lane >> 5);
+  // block-uniform rounds: every wave reaches every barrier
+  for (int tb = blockIdx.x * 4; tb < n_tiles; tb += gridDim.x * 4) {
+    const int row0 = (tb + wave) * kScoreRows;
+    const bool active = tb + wave < n_tiles;
+    if (active) {
+      // layer 1: A fragments of the wave's 32 rows (zero past the last row)
+      bf16x8 xa[2];
+      const int xr = row0 + lr;
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        bf16x8 z = {};
+        xa[s] = xr < n_rows ? *reinterpret_cast<const bf16x8*>(X + (size_t)xr * kMlpIn + 16 * s + kh)
+                            : z;
+      }
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        floatx16 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          acc[t] = floatx16{};
+          const int n = (half * 4 + t) * 32 + lr;
+#pragma unroll
+          for (int s = 0; s < 2; ++s) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(W1 + (size_t)n * kMlpIn + 16 * s + kh);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[s], b, acc[t], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int n = (half * 4 + t) * 32 + lr;
+          const float bias = b1[n];
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int m = 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+            float x = acc[t][v] + bias;
+            x = x > 0.0f ? x : 0.0f;
+            h1[m * kH1Stride + n] = f2bf(x);
+          }
+        }
+      }
+    }
+    // layer 2 and the output layer
+    float part[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) part[v] = 0.0f;
+#pragma unroll 1
+    for (int half = 0; half < 2; ++half) {
+      __syncthreads();  // H1 written; the previous half's readers are done with s_w2
+      {
+        // all 16 loads of a thread in flight before its LDS stores
+        constexpr int kStage = kW2Half * (kMlpHidden / 8) / 256;
+        bf16x8 buf[kStage];
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+          const int c = threadIdx.x + 256 * j;
+          const int r = c / (kMlpHidden / 8), k = (c % (kMlpHidden / 8)) * 8;
+          buf[j] = *reinterpret_cast<const bf16x8*>(W2 + (size_t)(half * kW2Half + r) * kMlpHidden + k);
+        }
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+          const int c = threadIdx.x + 256 * j;
+          const int r = c / (kMlpHidden / 8), k = (c % (kMlpHidden / 8)) * 8;
+          *reinterpret_cast<bf16x8*>(s_w2 + r * kH1Stride + k) = buf[j];
+        }
+      }
+      __syncthreads();
+      if (active) {
+        floatx16 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = floatx16{};
+#pragma unroll 4
+        for (int s = 0; s < kMlpHidden / 16; ++s) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(h1 + lr * kH1Stride + 16 * s + kh);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const bf16x8 b =
+                *reinterpret_cast<const bf16x8*>(s_w2 + (t * 32 + lr) * kH1Stride + 16 * s + kh);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int n = half * kW2Half + t * 32 + lr;
+          const float bias = b2[n], wn = bf2f(w3[n]);
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            float x = acc[t][v] + bias;
+            x = x > 0.0f ? x : 0.0f;
+            part[v] += bf2f(f2bf(x)) * wn;
+          }
+        }
+      }
+    }
+    if (active) {
+      // sum each row over the 32 lanes of its half-wave
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+        for (int o = 1; o < 32; o <<= 1) part[v] += __shfl_xor(part[v], o);
+      if (lr == 0) {
+        const float bb = *b3;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int m = row0 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+          if (m < n_rows) logits[m] = part[v] + bb;
+        }
+      }
+    }
+    __syncthreads();  // the next round rewrites the H1 tiles
+  }
+}
+
+hipError_t mlp_score(const uint16_t* X, const uint16_t* W1, const float* b1, const uint16_t* W2,
+                     const float* b2, const uint16_t* w3, const float* b3, int rows,
+                     const uint32_t* rows_dev, float* logits, int max_blocks, hipStream_t s) {
+  if (rows == 0) return hipSuccess;
+  const int tiles = (rows + kScoreRows - 1) / kScoreRows;
+  const int blocks = max_blocks > 0 ? std::min((tiles + 3) / 4, max_blocks) : (tiles + 3) / 4;
+  hipLaunchKernelGGL(k_mlp_score, dim3(blocks), dim3(256), 0, s, X, W1, b1, W2, b2, w3, b3, rows,
+                     rows_dev, logits);
   return hipGetLastError();
 }
 

@@ -450,7 +450,23 @@ gbr_status rebuild_structure(gbr_renderer* r) {
     sd.entry_base = entries;
     bins += nb;
     entries += (uint64_t)nb * cap;
-    for (uint32_t b = 0; b < nb; ++b) ritems.push_back(RasterItem{si, b, {0, 0}});
+    for (uint32_t b = 0; b < nb; ++b) {
+      RasterItem ri{};
+      ri.scene = si;
+      ri.gbin = sd.bin_base + b;
+      const int bx = (int)(b % (uint32_t)bins_x), by = (int)(b / (uint32_t)bins_x);
+      ri.x0 = (int16_t)(bx * B);
+      ri.y0 = (int16_t)(by * B);
+      ri.x1 = (int16_t)(std::min(bx * B + B, W) - 1);
+      ri.y1 = (int16_t)(std::min(by * B + B, H) - 1);
+      ri.bin_cap = cap;
+      ri.plane_offset = sd.plane_offset;
+      ri.width = W;
+      ri.height = H;
+      ri.entries = sd.entry_base + (uint64_t)b * cap;
+      ri.rec_base = sd.rec_base;
+      ritems.push_back(ri);
+    }
   }
   r->n_nodes = (uint32_t)parents.size();
   r->n_items = (uint32_t)items.size();
@@ -643,6 +659,7 @@ gbr_status do_draw(gbr_renderer* r) {
     p.logits = r->vis.logits_i.ptr;
   }
   p.raster_mode = r->tun.raster_mode;
+  p.lane_pixels = r->tun.lane_pixels;
   {
     const int W = r->cfg.tile_size[0], H = r->cfg.tile_size[1];
     p.hiz_w0 = (W + 7) >> 3;
@@ -686,8 +703,8 @@ gbr_status do_draw(gbr_renderer* r) {
     HIP_TRY(launch_cull(p, r->stream));
     if (learned) {
       const int rows = (int)((r->n_items + 31) & ~31u);
-      TRY(r->vis.forward(r->vis.Xi.ptr, rows, p.counters + kCtrProp * kCtrStride, r->vis.H1i.ptr,
-                         nullptr, r->vis.H2i.ptr, nullptr, r->vis.logits_i.ptr, r->stream));
+      TRY(r->vis.score(r->vis.Xi.ptr, rows, p.counters + kCtrProp * kCtrStride,
+                       r->vis.logits_i.ptr, r->cu_count, r->stream));
       HIP_TRY(launch_propose(p, r->stream));
     }
   }
@@ -837,6 +854,7 @@ gbr_status gbr_tuning_init(gbr_tuning* t) {
   t->min_bin_capacity = 256;
   t->cull_mode = GBR_CULL_NONE;
   t->queue_grain = 4;
+  t->lane_pixels = 96;
   return GBR_OK;
 }
 
@@ -847,8 +865,10 @@ static gbr_status validate_tuning(const gbr_tuning* t) {
     return fail(GBR_ERR_ARGUMENT, "tuning: chunk_triangles must be in [1, 1024]");
   if (!(t->bin_capacity_factor > 0.0f) || t->min_bin_capacity < 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: capacity must be positive");
-  if (t->raster_mode < 0 || t->raster_mode > 2)
-    return fail(GBR_ERR_ARGUMENT, "tuning: raster_mode must be 0, 1 or 2");
+  if (t->raster_mode < 0 || t->raster_mode > 1)
+    return fail(GBR_ERR_ARGUMENT, "tuning: raster_mode must be 0 or 1");
+  if (t->lane_pixels < 1 || t->lane_pixels > kMaxBin * kMaxBin)
+    return fail(GBR_ERR_ARGUMENT, "tuning: lane_pixels must be in [1, 1024]");
   if (t->timing != 0 && t->timing != 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: timing must be 0 or 1");
   if (t->queue_grain < 1 || t->queue_grain > 64)

@@ -39,7 +39,7 @@ GemmArgs gemm(const uint16_t* A, int lda, const uint16_t* Bt, int ldb, int M, in
 void VisModel::release() {
   for (DevBuf<float>* b : {&w, &g, &m, &v, &logits, &dl, &lossr, &red, &logits_i}) b->release();
   for (DevBuf<uint16_t>* b : {&W1b, &W2b, &W2tb, &w3b, &Xd, &Xb, &Xbt, &H1, &H1t, &H2, &H2t, &dH2,
-                              &dH2t, &dH1, &dH1t, &Xi, &H1i, &H2i})
+                              &dH2t, &dH1, &dH1t, &Xi})
     b->release();
   Yd.release();
   Yb.release();
@@ -118,8 +118,6 @@ gbr_status VisModel::reserve_dataset(uint64_t rows_cap) {
 gbr_status VisModel::reserve_infer(size_t n) {
   n = std::max<size_t>((n + 31) & ~size_t(31), 32);
   TRY(Xi.reserve(n * kMlpIn));
-  TRY(H1i.reserve(n * kMlpHidden));
-  TRY(H2i.reserve(n * kMlpHidden));
   TRY(logits_i.reserve(n));
   TRY(prop.reserve(n));
   return GBR_OK;
@@ -163,6 +161,13 @@ gbr_status VisModel::forward(const uint16_t* X, int nrows, const uint32_t* rows_
   g2.lddt = nrows;
   HIP_TRY(mlp_gemm(g2, s));
   HIP_TRY(mlp_last_layer(H2o, w3b.ptr, w.ptr + kVisB3, out, nrows, rows_dev, s));
+  return GBR_OK;
+}
+
+gbr_status VisModel::score(const uint16_t* X, int nrows, const uint32_t* rows_dev, float* out,
+                           int max_blocks, hipStream_t s) {
+  HIP_TRY(mlp_score(X, W1b.ptr, w.ptr + kVisB1, W2b.ptr, w.ptr + kVisB2, w3b.ptr, w.ptr + kVisB3,
+                    nrows, rows_dev, out, max_blocks, s));
   return GBR_OK;
 }
 
@@ -275,7 +280,9 @@ gbr_status VisModel::predict(const float* X, int n, float* out, hipStream_t s) {
   std::vector<uint16_t> xb((size_t)np * kMlpIn, 0);
   for (size_t i = 0; i < (size_t)n * kMlpIn; ++i) xb[i] = host_f2bf(X[i]);
   HIP_TRY(hipMemcpyAsync(Xb.ptr, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, s));
-  TRY(forward(Xb.ptr, np, nullptr, H1.ptr, nullptr, H2.ptr, nullptr, logits.ptr, s));
+  // the renderer's fused scorer, on few workgroups so that each loops over
+  // several row tiles as it does on a frame's candidate rows
+  TRY(score(Xb.ptr, n, nullptr, logits.ptr, 16, s));
   HIP_TRY(hipMemcpyAsync(out, logits.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return GBR_OK;

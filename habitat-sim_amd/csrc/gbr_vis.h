@@ -40,7 +40,7 @@ struct VisModel {
   DevBuf<float> logits, dl, lossr, red;
 
   // per-frame inference scratch (rows = candidate work items)
-  DevBuf<uint16_t> Xi, H1i, H2i;
+  DevBuf<uint16_t> Xi;
   DevBuf<float> logits_i;
   DevBuf<uint32_t> prop;
 
@@ -52,6 +52,9 @@ struct VisModel {
   // forward over rows [0, min(rows, *rows_dev)) of X into logits
   gbr_status forward(const uint16_t* X, int rows, const uint32_t* rows_dev, uint16_t* H1o,
                      uint16_t* H1to, uint16_t* H2o, uint16_t* H2to, float* out, hipStream_t s);
+  // rendering: logits of rows [0, min(rows, *rows_dev)) of X, fused
+  gbr_status score(const uint16_t* X, int rows, const uint32_t* rows_dev, float* out,
+                   int max_blocks, hipStream_t s);
   // loss and gradients (into g) of the batch already in Xb/Xbt/Yb
   gbr_status backward(int batch, float pos_weight, hipStream_t s);
   gbr_status adam_step(const gbr_vis_fit_params& fp, hipStream_t s);

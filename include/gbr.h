@@ -114,10 +114,12 @@ typedef struct gbr_tuning {
   int32_t queue_grain;        /* setup work items claimed per queue fetch: 1..64 */
   int32_t timing;             /* 1: time each stage of gbr_draw() with HIP events */
   int32_t raster_mode;        /* 0: a lane per binned triangle, 1: a lane per
-                                 (triangle, pixel) pair, 2: by rectangle size
-                                 (lane / wave / workgroup); same output */
+                                 (triangle, pixel) pair; same output */
   float proposal_threshold;   /* GBR_CULL_LEARNED: phase 1 = chunks whose logit
                                  exceeds this */
+  int32_t lane_pixels;        /* raster_mode 0: triangles covering more than this
+                                 many pixels of a bin are rasterised by the whole
+                                 workgroup, the rest by one lane each: 1..1024 */
 } gbr_tuning;
 
 /* stages timed when gbr_tuning.timing is set (gbr_frame_timings) */

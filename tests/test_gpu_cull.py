@@ -65,7 +65,8 @@ def _walk(spec, steps, seed):
     (N.CULL_FRUSTUM, {}), (N.CULL_HIZ, {}), (N.CULL_HIZ, dict(raster_mode=1)),
     # bins overflow: the exact rescan also builds the depth pyramid
     (N.CULL_HIZ, dict(min_bin_capacity=64, bin_capacity_factor=0.02)),
-    (N.CULL_HIZ, dict(bin_size=16)), (N.CULL_HIZ, dict(raster_mode=2))])
+    (N.CULL_HIZ, dict(bin_size=16)), (N.CULL_HIZ, dict(lane_pixels=1)),
+    (N.CULL_HIZ, dict(lane_pixels=1024))])
 def test_apartment_walk_matches_unculled(mode, tuning):
     spec = scenes.apartments(n_envs=48)
     plain = _renderer(spec, N.CULL_NONE)

@@ -52,7 +52,11 @@ def test_small_rooms_16_envs_bit_exact():
                                     dict(bin_size=16, min_bin_capacity=64, bin_capacity_factor=0.01),
                                     dict(raster_mode=1), dict(raster_mode=1, bin_size=8),
                                     dict(raster_mode=1, queue_grain=7, cull_mode=2),
-                                    dict(raster_mode=2), dict(raster_mode=2, bin_size=16, cull_mode=2)])
+                                    # every rectangle above one pixel to the workgroup
+                                    # queue (which overflows), or none of them
+                                    dict(lane_pixels=1), dict(lane_pixels=1, bin_size=8),
+                                    dict(lane_pixels=2, bin_size=16, cull_mode=2),
+                                    dict(lane_pixels=1024)])
 def test_launch_parameters_do_not_change_output(tuning):
     spec = scenes.small_rooms(n_envs=6)
     r, o = _render_both(spec, tuning)

@@ -55,11 +55,14 @@ def test_loss_and_gradient_match_oracle(rows, pos_weight):
     r.close()
 
 
-def test_logits_match_oracle():
+@pytest.mark.parametrize("rows", [100, 20000])
+def test_logits_match_oracle(rows):
+    # the renderer's fused scorer: 100 rows are a ragged last tile; 20000 rows
+    # on its 16 workgroups make every wave loop over ten tiles
     r = _small()
     flat = M.he_uniform(5).astype(np.float32)
     r.set_vis_weights(flat)
-    X, _ = _rows(100, 9)  # not a multiple of 32: padded rows
+    X, _ = _rows(rows, 9)
     z = r.vis_predict(X)
     ref, _ = M.forward(flat.astype(np.float64), X, rounding=True)
     assert np.max(np.abs(z - ref)) <= 1e-2 * (1.0 + np.max(np.abs(ref)))

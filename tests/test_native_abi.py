@@ -45,6 +45,15 @@ def test_abi_version_and_config_defaults():
     assert c.planes == N.PLANE_ALL
 
 
+def test_tuning_defaults():
+    lib = N.lib()
+    t = N.Tuning()
+    N.check(lib.gbr_tuning_init(ctypes.byref(t)))
+    assert (t.bin_size, t.chunk_triangles, t.queue_grain) == (32, 1024, 4)
+    assert (t.cull_mode, t.raster_mode, t.timing) == (N.CULL_NONE, 0, 0)
+    assert t.lane_pixels == 96 and t.proposal_threshold == 0.0
+
+
 def test_null_arguments_are_errors_not_crashes():
     lib = N.lib()
     assert lib.gbr_create(None, None) == N.ERR_ARGUMENT
