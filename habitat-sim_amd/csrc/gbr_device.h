@@ -135,8 +135,8 @@ struct alignas(16) PrimRef {
 enum : uint32_t {
   kCtrClipped = 0,
   kCtrSubTris = 1,
-  kCtrQueue1 = 2,    // next setup work item, phase 1 (dynamic work queue)
-  kCtrQueue2 = 3,    // same, phase 2
+  kCtrQueue1 = 2,    // phase-1 setup queue: first of kQueueStripes counters
+  kCtrQueue2 = 3,    // phase 2 (the stripes live at kCtrStripes + ...)
   kCtrList1 = 4,     // work items set up in phase 1
   kCtrCand = 5,      // work items to test against the phase-1 depth pyramid
   kCtrList2 = 6,     // work items set up in phase 2
@@ -145,6 +145,13 @@ enum : uint32_t {
   kCtrCount = 9
 };
 constexpr uint32_t kCtrStride = 32;
+// The setup work queue of each phase is split into kQueueStripes contiguous
+// ranges of the item list, each with its own claim counter: single-address
+// atomics serialise at the memory side (~20 ns each), so one counter for
+// thousands of claims would cost more than the claims save.
+constexpr uint32_t kQueueStripes = 16;
+constexpr uint32_t kCtrStripes = kCtrCount;                       // [phase][stripe]
+constexpr uint32_t kCtrTotal = kCtrStripes + 2 * kQueueStripes;   // counters reset per frame
 
 // frame statistics (64-bit): bin entries, overflowed bins, then triangles and
 // vertex-range entries set up in phase 1, the same for phase 2, then pixels
@@ -258,6 +265,7 @@ struct FrameParams {
   float proposal_threshold;
   int32_t raster_mode;  // 0: a lane per triangle, 1: a lane per (triangle, pixel)
   int32_t lane_pixels;  // mode 0: rectangles above this go to the workgroup queue
+  uint32_t lds_bins;    // setup: scenes with at most this many bins count in LDS
   int32_t hiz_fused;    // phase-1 raster writes the depth pyramid (32-pixel bins)
   // learned proposal: feature rows [kCtrProp count][32] bf16 and their items
   const float* cam_info;   // [scene][8]: camera position xyz, forward xyz

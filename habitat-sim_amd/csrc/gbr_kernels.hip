@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
                                                   unsigned long long* __restrict__ stats) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   // first kernel of every frame: reset the frame's counters
-  if (i < kCtrCount) counters[i * kCtrStride] = 0u;
+  if (i < kCtrTotal) counters[i * kCtrStride] = 0u;
   for (uint32_t j = i; j < kStatCount * kStatStripes; j += gridDim.x * kBlock) stats[j] = 0ull;
   if (i >= n) return;
   int depth = 0;
@@ -393,6 +393,42 @@ __device__ __forceinline__ void vertex_uv(const FrameParams& p, uint32_t flags, 
 
 // wave-aggregated append of one PrimRef per flagged lane (ballot + prefix
 // popcount, one atomic per wave)
+// Depth-pyramid test: true iff every fragment with depth >= zmin inside the
+// pixel rectangle [x0, x1] x [y0, y1] of the scene fails LESS against the
+// phase-1 depths (zmin above the largest of them over every covering tile).
+// Level 0 (8x8 tiles) when the rectangle touches at most 16 of them, else
+// level 1 (32x32).
+__device__ __forceinline__ bool hiz_hidden(const FrameParams& p, uint32_t scene, int x0, int y0,
+                                           int x1, int y1, float zmin) {
+  const float* h = p.hiz + (size_t)scene * p.hiz_stride;
+  int w = p.hiz_w0;
+  int bx0 = x0 >> kHiZ0, bx1 = x1 >> kHiZ0, by0 = y0 >> kHiZ0, by1 = y1 >> kHiZ0;
+  if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > 16) {
+    w = p.hiz_w1;
+    h += p.hiz_w0 * p.hiz_h0;
+    bx0 = x0 >> kHiZ1; bx1 = x1 >> kHiZ1; by0 = y0 >> kHiZ1; by1 = y1 >> kHiZ1;
+  }
+  for (int by = by0; by <= by1; ++by)
+    for (int bx = bx0; bx <= bx1; ++bx)
+      if (!(zmin > h[by * w + bx])) return false;
+  return true;
+}
+
+// A load of data the kernel never writes, at an address uniform over the
+// wave: through the constant address space it becomes a scalar load into
+// SGPRs (no VGPRs, scalar-cache latency).
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* ptr, uint32_t i) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+  const ConstWords w = (ConstWords)(ptr + __builtin_amdgcn_readfirstlane(i));
+  T out;
+  uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) o[k] = w[k];
+  return out;
+}
+
 __device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counter, bool flag,
                                            PrimRef ref, PrimRef* list, uint32_t cap, int lane,
                                            unsigned long long lt_mask) {
@@ -434,8 +470,10 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
   __shared__ float s_cw[kVertexCache];
   __shared__ float s_s[kVertexCache];
   __shared__ float s_t[kVertexCache];
-  __shared__ uint32_t s_bcount[kMaxSceneBins];
-  __shared__ uint32_t s_bbase[kMaxSceneBins];
+  // per-bin counters and bases: 2 * p.lds_bins words of dynamic LDS
+  extern __shared__ uint32_t s_bins[];
+  uint32_t* const s_bcount = s_bins;
+  uint32_t* const s_bbase = s_bins + p.lds_bins;
   __shared__ uint32_t s_pair[kPairCap];  // bin << 21 | local slot << 10 | triangle in chunk
   __shared__ uint32_t s_npairs;
   __shared__ uint32_t s_item;
@@ -445,29 +483,68 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
   uint32_t written = 0;
   unsigned long long tri_acc = 0, vtx_acc = 0;  // triangles / vertices set up (uniform)
   uint32_t grab = 0, grab_end = 0;  // items claimed from the queue (uniform)
+  const bool hiz_test = queue_ctr == kCtrQueue2 && p.hiz != nullptr;
   // the whole frame, or the work items listed for this phase
   const uint32_t n_items = list ? min(p.counters[list_ctr * kCtrStride], p.n_items) : p.n_items;
 
+  bool have_pre = false;
+  uint32_t pre_item = 0;
+  WorkItem pre_wi = {};
+  // items per claim: queue_grain, down to 1 when there are fewer than 8
+  // claims' worth per workgroup (a short list balances better item by item)
+  const uint32_t grain =
+      max(1u, min((uint32_t)p.queue_grain, n_items / (gridDim.x * 8u)));
+  // this workgroup's stripe of the list and its claim counter (every stripe
+  // has workgroups: a grid smaller than kQueueStripes uses fewer stripes)
+  const uint32_t nstripes = min(kQueueStripes, gridDim.x);
+  const uint32_t stripe = blockIdx.x % nstripes;
+  const uint32_t s_begin = (uint32_t)((unsigned long long)n_items * stripe / nstripes);
+  const uint32_t s_end = (uint32_t)((unsigned long long)n_items * (stripe + 1) / nstripes);
+  uint32_t* const claim_ctr =
+      &p.counters[(kCtrStripes + (queue_ctr == kCtrQueue2 ? kQueueStripes : 0u) + stripe) * kCtrStride];
+  // thread 0 claims the next chunk one chunk ahead, so the claim's round
+  // trip overlaps the current chunk (the last, unused claim only moves the
+  // counter past the end)
+  uint32_t next_claim = 0;
+  if (threadIdx.x == 0) next_claim = s_begin + atomicAdd(claim_ctr, grain);
   while (true) {
     if (grab == grab_end) {
-      if (threadIdx.x == 0) s_item = atomicAdd(&p.counters[queue_ctr * kCtrStride], p.queue_grain);
+      if (threadIdx.x == 0) {
+        s_item = next_claim;
+        if (next_claim < s_end) next_claim = s_begin + atomicAdd(claim_ctr, grain);
+      }
       __syncthreads();
       grab = s_item;
-      grab_end = min(grab + p.queue_grain, n_items);
+      grab_end = min(grab + grain, s_end);
       __syncthreads();  // s_item may be rewritten only after everyone read it
-      if (grab >= n_items) break;
+      if (grab >= s_end) break;
     }
-    const uint32_t item = list ? list[grab] : grab;
+    uint32_t item;
+    WorkItem wi;
+    if (have_pre) {  // fetched during the previous item
+      item = pre_item;
+      wi = pre_wi;
+    } else {
+      item = list ? uniform_load(list, grab) : grab;
+      wi = uniform_load(p.items, item);
+    }
     ++grab;
-    const WorkItem wi = p.items[item];
+    have_pre = false;
+    // the next item of the chunk: its index now, its descriptor after the
+    // triangle loop, each load overlapping this item's work
+    const bool prefetch = grab < grab_end;
+    if (prefetch) pre_item = list ? uniform_load(list, grab) : grab;
     tri_acc += wi.tri_count;
     vtx_acc += wi.vcount;
-    const SceneDesc sc = p.scenes[wi.scene];
-    const DrawDesc d = p.draws[wi.draw];
-    const MaterialDesc& mat = p.materials[d.material];
+    const SceneDesc sc = uniform_load(p.scenes, wi.scene);
+    const DrawDesc d = uniform_load(p.draws, wi.draw);
+    const MaterialDesc mat = uniform_load(p.materials, d.material);
     float M[16], PV[16], tm[9];
-    load_mat(p.abs_nodes + (size_t)wi.node * 16, M);
-    load_mat(p.cameras + (size_t)wi.scene * 16, PV);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      M[q] = uniform_load(p.abs_nodes, wi.node * 16u + q);
+      PV[q] = uniform_load(p.cameras, wi.scene * 16u + q);
+    }
 #pragma unroll
     for (int q = 0; q < 9; ++q) tm[q] = mat.texmat[q];
     const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
@@ -476,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
     // scenes with more bins than the LDS counters hold append every pair
     // straight to the global bins
     const int nbins = sc.bins_x * sc.bins_y;
-    const int nlb = nbins <= kMaxSceneBins ? nbins : 0;
+    const int nlb = nbins <= (int)p.lds_bins ? nbins : 0;
 
     for (int b = threadIdx.x; b < nlb; b += kBlock) s_bcount[b] = 0u;
     if (threadIdx.x == 0) s_npairs = nlb ? 0u : (uint32_t)kPairCap;
@@ -530,6 +607,11 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
             if (tri_area(X, Y) > 0) {
               pr = tri_pixels(X, Y, tile);
               active = pr.x0 <= pr.x1 && pr.y0 <= pr.y1;
+              // phase 2: a triangle whose nearest vertex depth (every
+              // fragment's lower bound) lies behind the phase-1 pyramid over
+              // its rectangle cannot change a pixel
+              if (active && hiz_test && hiz_hidden(p, wi.scene, pr.x0, pr.y0, pr.x1, pr.y1, min_depth(Z)))
+                active = false;
             }
           } else {
             clip_me = true;  // rare: k_setup_clip handles the fan
@@ -571,6 +653,10 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
       }
       append_ref(p, kCtrClipped * kCtrStride, clip_me, PrimRef{wi.draw, prim, item, 0u}, p.clipped,
                  p.clipped_cap, lane, lt_mask);
+    }
+    if (prefetch) {
+      pre_wi = uniform_load(p.items, pre_item);
+      have_pre = true;
     }
     __syncthreads();
     // one global reservation per touched bin
@@ -2184,19 +2270,7 @@ __global__ __launch_bounds__(kListBlock) void k_cull_hiz(FrameParams p) {
   if (i < n) {
     item = p.cand[i];
     const ItemCull ic = p.cull[item];
-    const float* h = p.hiz + (size_t)ic.pad[0] * p.hiz_stride;
-    int lvl = kHiZ0, w = p.hiz_w0;
-    int bx0 = ic.x0 >> kHiZ0, bx1 = ic.x1 >> kHiZ0, by0 = ic.y0 >> kHiZ0, by1 = ic.y1 >> kHiZ0;
-    if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > 16) {
-      lvl = kHiZ1;
-      w = p.hiz_w1;
-      h += p.hiz_w0 * p.hiz_h0;
-      bx0 = ic.x0 >> kHiZ1; bx1 = ic.x1 >> kHiZ1; by0 = ic.y0 >> kHiZ1; by1 = ic.y1 >> kHiZ1;
-    }
-    (void)lvl;
-    for (int by = by0; by <= by1 && !keep; ++by)
-      for (int bx = bx0; bx <= bx1; ++bx)
-        if (!(ic.zmin > h[by * w + bx])) { keep = true; break; }
+    keep = !hiz_hidden(p, ic.pad[0], ic.x0, ic.y0, ic.x1, ic.y1, ic.zmin);
   }
   uint32_t* const ctr[1] = {&p.counters[kCtrList2 * kCtrStride]};
   const bool flag[1] = {keep};
@@ -2247,9 +2321,11 @@ hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_o
   return hipGetLastError();
 }
 
-int setup_blocks_per_cu() {
+int setup_blocks_per_cu(uint32_t lds_bins) {
   int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_setup_bin, kBlock, 0) != hipSuccess || b < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_setup_bin, kBlock,
+                                                   2 * lds_bins * sizeof(uint32_t)) != hipSuccess ||
+      b < 1)
     b = 4;
   return b;
 }
@@ -2259,8 +2335,8 @@ int setup_blocks_per_cu() {
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
                         uint32_t queue_ctr, uint32_t grid, hipStream_t s) {
   if (p.n_items == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_setup_bin, dim3(grid < p.n_items ? grid : p.n_items), dim3(kBlock), 0, s, p,
-                     list, list_ctr, queue_ctr);
+  hipLaunchKernelGGL(k_setup_bin, dim3(grid < p.n_items ? grid : p.n_items), dim3(kBlock),
+                     2 * p.lds_bins * sizeof(uint32_t), s, p, list, list_ctr, queue_ctr);
   return hipGetLastError();
 }
 

@@ -34,7 +34,7 @@ namespace gbr {
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
                         uint32_t* counters, unsigned long long* stats, hipStream_t s);
 hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s);
-int setup_blocks_per_cu();
+int setup_blocks_per_cu(uint32_t lds_bins);
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
                         uint32_t queue_ctr, uint32_t grid, hipStream_t s);
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
@@ -156,7 +156,8 @@ struct gbr_renderer {
   gbr_tuning tun{};
   int device = 0;
   int cu_count = 256;
-  int setup_bpc = 4;
+  int setup_bpc = 4;      // setup workgroups per CU (occupancy with lds_bins)
+  uint32_t lds_bins = 0;  // bins per scene counted in setup's LDS (0: none)
   hipStream_t stream = nullptr;
 
   // assets (host mirrors)
@@ -369,6 +370,10 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   const int B = r->tun.bin_size;
   const int bins_x = (W + B - 1) / B, bins_y = (H + B - 1) / B;
   const uint32_t chunk = (uint32_t)r->tun.chunk_triangles;
+  // every scene has the same bin grid; larger grids than the LDS counters
+  // hold append each pair straight to the global bins
+  r->lds_bins = bins_x * bins_y <= kMaxSceneBins ? (uint32_t)(bins_x * bins_y) : 0u;
+  r->setup_bpc = setup_blocks_per_cu(r->lds_bins);
 
   std::vector<int32_t> parents;
   std::vector<float> local;
@@ -604,7 +609,7 @@ gbr_status do_draw(gbr_renderer* r) {
   else TRY(upload_dirty_transforms(r));
   TRY(upload_cameras(r));
   TRY(r->d_stats.reserve(kStatCount * kStatStripes));
-  TRY(r->d_counters.reserve(kCtrCount * kCtrStride));
+  TRY(r->d_counters.reserve(kCtrTotal * kCtrStride));
   TRY(r->d_vis.reserve((size_t)r->scene_count() * r->tile_pixels()));
   TRY(ensure_timing_events(r));
 
@@ -685,6 +690,7 @@ gbr_status do_draw(gbr_renderer* r) {
   int sh = 0;
   while ((1 << sh) < r->tun.bin_size) ++sh;
   p.bin_shift = sh;
+  p.lds_bins = r->lds_bins;
   p.top_down = r->cfg.top_down ? 1 : 0;
   p.no_textures = (r->cfg.flags & GBR_FLAG_NO_TEXTURES) ? 1 : 0;
 
@@ -905,7 +911,6 @@ gbr_status gbr_create(const gbr_config* config, gbr_renderer** out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
       r->cu_count = cus;
-    r->setup_bpc = setup_blocks_per_cu();
   }
   gbr_tuning_init(&r->tun);
   const uint64_t n = (uint64_t)config->tile_count[0] * config->tile_count[1];
