@@ -133,7 +133,7 @@ def stage_bytes(stage, spec, st, n_items, n_nodes, learned=False):
         return 12 * t2 + 16 * v2 + 64 * st["items_phase2"]
     if stage == "cull":  # work item, bounds, transform; learned: feature rows + bf16 weights
         rows = st.get("items_phase1", 0) + st.get("items_candidates", 0)
-        return n_items * (32 + 24 + 64) + (64 * rows + 2 * 74497 if learned else 0)
+        return n_items * (32 + 24 + 64) + (64 * rows + 2 * 20865 if learned else 0)
     if stage == "hiz":  # depth pyramid: read the phase-1 depth, write 4/3 of a float per 16 px
         return envs * (w * h * 8 + w * h * 4 // 12)
     if stage == "resolve":

@@ -396,14 +396,14 @@ __device__ __forceinline__ void vertex_uv(const FrameParams& p, uint32_t flags, 
 // Depth-pyramid test: true iff every fragment with depth >= zmin inside the
 // pixel rectangle [x0, x1] x [y0, y1] of the scene fails LESS against the
 // phase-1 depths (zmin above the largest of them over every covering tile).
-// Level 0 (8x8 tiles) when the rectangle touches at most 16 of them, else
-// level 1 (32x32).
+// Level 0 (8x8 tiles) when the rectangle touches at most max_tiles of them,
+// else level 1 (32x32).
 __device__ __forceinline__ bool hiz_hidden(const FrameParams& p, uint32_t scene, int x0, int y0,
-                                           int x1, int y1, float zmin) {
+                                           int x1, int y1, float zmin, int max_tiles) {
   const float* h = p.hiz + (size_t)scene * p.hiz_stride;
   int w = p.hiz_w0;
   int bx0 = x0 >> kHiZ0, bx1 = x1 >> kHiZ0, by0 = y0 >> kHiZ0, by1 = y1 >> kHiZ0;
-  if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > 16) {
+  if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > max_tiles) {
     w = p.hiz_w1;
     h += p.hiz_w0 * p.hiz_h0;
     bx0 = x0 >> kHiZ1; bx1 = x1 >> kHiZ1; by0 = y0 >> kHiZ1; by1 = y1 >> kHiZ1;
@@ -610,7 +610,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
               // phase 2: a triangle whose nearest vertex depth (every
               // fragment's lower bound) lies behind the phase-1 pyramid over
               // its rectangle cannot change a pixel
-              if (active && hiz_test && hiz_hidden(p, wi.scene, pr.x0, pr.y0, pr.x1, pr.y1, min_depth(Z)))
+              if (active && hiz_test &&
+                  hiz_hidden(p, wi.scene, pr.x0, pr.y0, pr.x1, pr.y1, min_depth(Z), 16))
                 active = false;
             }
           } else {
@@ -2270,7 +2271,7 @@ __global__ __launch_bounds__(kListBlock) void k_cull_hiz(FrameParams p) {
   if (i < n) {
     item = p.cand[i];
     const ItemCull ic = p.cull[item];
-    keep = !hiz_hidden(p, ic.pad[0], ic.x0, ic.y0, ic.x1, ic.y1, ic.zmin);
+    keep = !hiz_hidden(p, ic.pad[0], ic.x0, ic.y0, ic.x1, ic.y1, ic.zmin, 16);
   }
   uint32_t* const ctr[1] = {&p.counters[kCtrList2 * kCtrStride]};
   const bool flag[1] = {keep};

@@ -1,4 +1,4 @@
-// Chunk-visibility MLP (32 -> 256 -> 256 -> 1, ReLU) on gfx950 MFMA:
+// Chunk-visibility MLP (32 -> 128 -> 128 -> 1, ReLU) on gfx950 MFMA:
 // BF16 operands, FP32 accumulation, FP32 master weights with Adam.
 //
 // Every matrix product is an "NT" GEMM, C[M][N] = sum_k A[M][K] * Bt[N][K],
@@ -299,12 +299,13 @@ namespace gbr {
 
 constexpr int kScoreRows = 32;              // rows per wave and round
 constexpr int kH1Stride = kMlpHidden + 8;   // padded LDS row (bank spread)
-constexpr int kW2Half = kMlpHidden / 2;     // W2 rows staged in LDS at a time
+constexpr int kW2Group = 128;               // W2 rows (output columns) staged at a time
+static_assert(kMlpHidden % kW2Group == 0, "hidden width in groups of 4 tiles");
 
 // Per round a workgroup scores 4 tiles of 32 rows (one per wave): layer 1
-// from L2-resident W1 into the wave's H1 tile in LDS, then layer 2 in two
-// halves of 128 output columns, each half of W2 staged in LDS once for the
-// 4 waves; the output layer folds into the layer-2 epilogue.
+// from L2-resident W1 into the wave's H1 tile in LDS, then layer 2 in groups
+// of 128 output columns, each group of W2 rows staged in LDS once for the 4
+// waves; the output layer folds into the layer-2 epilogue.
 __global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ X,
                                                    const uint16_t* __restrict__ W1,
                                                    const float* __restrict__ b1,
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ 
                                                    const uint32_t* __restrict__ rows_dev,
                                                    float* __restrict__ logits) {
   __shared__ uint16_t s_h1[4][kScoreRows * kH1Stride];
-  __shared__ uint16_t s_w2[kW2Half * kH1Stride];
+  __shared__ uint16_t s_w2[kW2Group * kH1Stride];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n_rows = rows_of(rows, rows_dev);
   const int n_tiles = (n_rows + kScoreRows - 1) / kScoreRows;
@@ -336,12 +337,12 @@ __global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ 
                             : z;
       }
 #pragma unroll
-      for (int half = 0; half < 2; ++half) {
+      for (int grp = 0; grp < kMlpHidden / kW2Group; ++grp) {
         floatx16 acc[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           acc[t] = floatx16{};
-          const int n = (half * 4 + t) * 32 + lr;
+          const int n = grp * kW2Group + t * 32 + lr;
 #pragma unroll
           for (int s = 0; s < 2; ++s) {
             const bf16x8 b = *reinterpret_cast<const bf16x8*>(W1 + (size_t)n * kMlpIn + 16 * s + kh);
@@ -350,7 +351,7 @@ __global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ 
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const int n = (half * 4 + t) * 32 + lr;
+          const int n = grp * kW2Group + t * 32 + lr;
           const float bias = b1[n];
 #pragma unroll
           for (int v = 0; v < 16; ++v) {
@@ -367,17 +368,17 @@ __global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ 
 #pragma unroll
     for (int v = 0; v < 16; ++v) part[v] = 0.0f;
 #pragma unroll 1
-    for (int half = 0; half < 2; ++half) {
-      __syncthreads();  // H1 written; the previous half's readers are done with s_w2
+    for (int grp = 0; grp < kMlpHidden / kW2Group; ++grp) {
+      __syncthreads();  // H1 written; the previous group's readers are done with s_w2
       {
-        // all 16 loads of a thread in flight before its LDS stores
-        constexpr int kStage = kW2Half * (kMlpHidden / 8) / 256;
+        // all loads of a thread in flight before its LDS stores
+        constexpr int kStage = kW2Group * (kMlpHidden / 8) / 256;
         bf16x8 buf[kStage];
 #pragma unroll
         for (int j = 0; j < kStage; ++j) {
           const int c = threadIdx.x + 256 * j;
           const int r = c / (kMlpHidden / 8), k = (c % (kMlpHidden / 8)) * 8;
-          buf[j] = *reinterpret_cast<const bf16x8*>(W2 + (size_t)(half * kW2Half + r) * kMlpHidden + k);
+          buf[j] = *reinterpret_cast<const bf16x8*>(W2 + (size_t)(grp * kW2Group + r) * kMlpHidden + k);
         }
 #pragma unroll
         for (int j = 0; j < kStage; ++j) {
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ 
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const int n = half * kW2Half + t * 32 + lr;
+          const int n = grp * kW2Group + t * 32 + lr;
           const float bias = b2[n], wn = bf2f(w3[n]);
 #pragma unroll
           for (int v = 0; v < 16; ++v) {

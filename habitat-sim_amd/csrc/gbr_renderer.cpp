@@ -710,7 +710,7 @@ gbr_status do_draw(gbr_renderer* r) {
     if (learned) {
       const int rows = (int)((r->n_items + 31) & ~31u);
       TRY(r->vis.score(r->vis.Xi.ptr, rows, p.counters + kCtrProp * kCtrStride,
-                       r->vis.logits_i.ptr, r->cu_count, r->stream));
+                       r->vis.logits_i.ptr, r->cu_count * 2, r->stream));
       HIP_TRY(launch_propose(p, r->stream));
     }
   }

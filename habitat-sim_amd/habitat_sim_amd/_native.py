@@ -54,7 +54,7 @@ class Tuning(ctypes.Structure):
 
 
 VIS_FEATURES = 32
-VIS_PARAMS = 74497
+VIS_PARAMS = 20865
 
 
 class VisFitParams(ctypes.Structure):

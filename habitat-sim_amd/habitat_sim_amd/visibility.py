@@ -1,7 +1,7 @@
 """Training the learned visibility proposal of the batch renderer.
 
 The renderer scores every chunk that passes the frustum test with a small
-MLP (32 features -> 256 -> 256 -> 1); in cull mode CULL_LEARNED the chunks
+MLP (32 features -> 128 -> 128 -> 1); in cull mode CULL_LEARNED the chunks
 scored visible are drawn first and the rest are tested against their depth
 pyramid.  The proposal only orders work, so a poor model costs time, never
 pixels.  Training rows are (features, chunk had a visible pixel) pairs taken

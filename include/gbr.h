@@ -337,7 +337,7 @@ gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* tuning);
 gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out);
 /* ---- learned visibility proposal (new) -------------------------------------
    No counterpart in the reference, whose batch renderer submits every mesh
-   of every scene (Renderer.cpp:1298-1351).  A 32 -> 256 -> 256 -> 1 MLP
+   of every scene (Renderer.cpp:1298-1351).  A 32 -> 128 -> 128 -> 1 MLP
    (BF16 MFMA, FP32 accumulation and master weights) scores each chunk that
    passes the frustum test from per-(camera, chunk) features; with
    GBR_CULL_LEARNED the chunks it scores above gbr_tuning.proposal_threshold
@@ -345,7 +345,7 @@ gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out);
    orders work: output pixels never depend on it.  Training rows are
    (features, chunk visible) pairs taken from frames the renderer draws. */
 #define GBR_VIS_FEATURES 32
-#define GBR_VIS_PARAMS 74497 /* W1[256][32] b1[256] W2[256][256] b2[256] w3[256] b3 */
+#define GBR_VIS_PARAMS 20865 /* W1[128][32] b1[128] W2[128][128] b2[128] w3[128] b3 */
 
 typedef struct gbr_vis_fit_params {
   int32_t steps;            /* optimiser steps */
