@@ -507,7 +507,8 @@ class RendererStandalone:
         return out
 
     def frame_timings(self) -> dict:
-        """Milliseconds per stage of the last draw() (needs tuning timing=1)."""
+        """Milliseconds per stage, averaged over the draw() calls since the
+        previous call (needs tuning timing=1)."""
         ms = (ctypes.c_float * len(N.TIMING_STAGES))()
         N.call("gbr_frame_timings", self._h, ms)
         return dict(zip(N.TIMING_STAGES, list(ms)))
