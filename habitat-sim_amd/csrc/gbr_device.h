@@ -258,7 +258,7 @@ struct FrameParams {
   uint32_t* list2;              // work items of phase 2
   uint32_t* item_vis;           // frame number in which the item was last visible
   float* hiz;                   // depth pyramid, hiz_stride floats per scene
-  uint32_t frame;               // frame number (>= 2)
+  uint32_t* frame;              // device frame number (>= 2 while drawing), advanced by k_nodes
   int32_t cull_mode;
   int32_t hiz_w0, hiz_h0, hiz_w1, hiz_h1;  // pyramid level sizes in blocks
   uint32_t hiz_stride;

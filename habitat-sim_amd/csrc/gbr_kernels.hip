@@ -268,10 +268,13 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
                                                   const float* __restrict__ local,
                                                   float* __restrict__ abs_out, uint32_t n,
                                                   uint32_t* __restrict__ counters,
-                                                  unsigned long long* __restrict__ stats) {
+                                                  unsigned long long* __restrict__ stats,
+                                                  uint32_t* __restrict__ frame) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  // first kernel of every frame: reset the frame's counters
+  // first kernel of every frame: reset the frame's counters, advance the
+  // frame number (kept on the device, so a captured frame replays as is)
   if (i < kCtrTotal) counters[i * kCtrStride] = 0u;
+  if (i == 0) *frame += 1u;
   for (uint32_t j = i; j < kStatCount * kStatStripes; j += gridDim.x * kBlock) stats[j] = 0ull;
   if (i >= n) return;
   int depth = 0;
@@ -1847,6 +1850,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
   __syncthreads();
   const RasterItem it = items[blockIdx.x];
   const BinInfo b = bin_info(it);
+  const uint32_t frame = *p.frame;
   const float un0 = p.unprojections[2 * it.scene], un1 = p.unprojections[2 * it.scene + 1];
   const float far_depth = un1 / (1.0f + un0);
   const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
@@ -1879,7 +1883,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
       const uint32_t item = min(rs.item, p.n_items - 1u);
       d = p.draws[p.items[item].draw];
       mat = p.materials[d.material];
-      if (p.item_vis) p.item_vis[item] = p.frame;  // visibility feedback for the next frame
+      if (p.item_vis) p.item_vis[item] = frame;  // visibility feedback for the next frame
       A.cx[0] = rs.cx0; A.cy[0] = rs.cy0; A.cw[0] = rs.cw0;
       A.cx[1] = rs.cx1; A.cy[1] = rs.cy1; A.cw[1] = rs.cw1;
       A.cx[2] = rs.cx2; A.cy[2] = rs.cy2; A.cw[2] = rs.cw2;
@@ -2023,10 +2027,10 @@ __device__ void item_features(const FrameParams& p, const WorkItem& wi, const Sc
                               const ItemCull& ic, float wmin, float wmax, uint32_t item,
                               uint16_t* row) {
   float f[32];
-  const uint32_t seen = p.item_vis[item];
-  f[0] = (seen + 1u == p.frame) ? 1.0f : 0.0f;
-  f[1] = (seen != 0u && p.frame - seen <= 8u) ? 1.0f : 0.0f;
-  f[2] = seen == 0u ? 1.0f : fminf((float)(p.frame - seen), 64.0f) / 64.0f;
+  const uint32_t seen = p.item_vis[item], frame = *p.frame;
+  f[0] = (seen + 1u == frame) ? 1.0f : 0.0f;
+  f[1] = (seen != 0u && frame - seen <= 8u) ? 1.0f : 0.0f;
+  f[2] = seen == 0u ? 1.0f : fminf((float)(frame - seen), 64.0f) / 64.0f;
   const float iW = 1.0f / (float)sc.width, iH = 1.0f / (float)sc.height;
   f[3] = ic.x0 * iW;
   f[4] = ic.y0 * iH;
@@ -2152,7 +2156,7 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
             p.cull[item] = ic;  // k_propose decides from the model's score
           } else {
             const bool proposed =
-                p.cull_mode == kCullFrustum || p.item_vis[item] + 1u == p.frame;
+                p.cull_mode == kCullFrustum || p.item_vis[item] + 1u == *p.frame;
             if (proposed) {
               to1 = true;
             } else {
@@ -2216,7 +2220,7 @@ __global__ __launch_bounds__(kBlock) void k_collect(FrameParams p, uint16_t* __r
   bool pos = false;
   if (row < n && base + row < cap) {
     const uint32_t item = p.prop[row];
-    pos = p.item_vis[item] == p.frame;
+    pos = p.item_vis[item] == *p.frame;
     const uint4* src = reinterpret_cast<const uint4*>(p.features + (size_t)row * 32);
     uint4* dst = reinterpret_cast<uint4*>(X + (size_t)(base + row) * 32);
 #pragma unroll
@@ -2315,10 +2319,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g) {
 // --------------------------------------------------------------------------
 
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
-                        uint32_t* counters, unsigned long long* stats, hipStream_t s) {
+                        uint32_t* counters, unsigned long long* stats, uint32_t* frame,
+                        hipStream_t s) {
   const uint32_t blocks = n == 0 ? 1 : (n + kBlock - 1) / kBlock;  // always runs: resets counters
   hipLaunchKernelGGL(k_nodes, dim3(blocks), dim3(kBlock), 0, s, parents, local, abs_out, n,
-                     counters, stats);
+                     counters, stats, frame);
   return hipGetLastError();
 }
 

@@ -32,7 +32,8 @@
 
 namespace gbr {
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
-                        uint32_t* counters, unsigned long long* stats, hipStream_t s);
+                        uint32_t* counters, unsigned long long* stats, uint32_t* frame,
+                        hipStream_t s);
 hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s);
 int setup_blocks_per_cu(uint32_t lds_bins);
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
@@ -203,7 +204,13 @@ struct gbr_renderer {
   DevBuf<ItemCull> d_cull;
   DevBuf<uint32_t> d_list1, d_cand, d_list2, d_item_vis;
   DevBuf<float> d_hiz;
-  uint32_t frame_no = 1;
+  DevBuf<uint32_t> d_frame;  // device frame number (k_nodes advances it)
+
+  // the last frame's kernel sequence as a HIP graph, replayed while its
+  // launch parameters stay the same
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint8_t> graph_key;
   DevBuf<float> d_caminfo;
   VisModel vis;
   bool collecting = false;
@@ -493,7 +500,8 @@ gbr_status rebuild_structure(gbr_renderer* r) {
     TRY(r->d_item_vis.reserve(n));
     // visibility history refers to the old item numbering
     HIP_TRY(hipMemsetAsync(r->d_item_vis.ptr, 0, n * sizeof(uint32_t), s));
-    r->frame_no = 1;
+    TRY(r->d_frame.reserve(1));
+    HIP_TRY(hipMemsetD32Async(r->d_frame.ptr, 1u, 1, s));
   }
   TRY(upload(r->d_ritems, ritems, s));
   if (bins > r->n_bins || r->d_bin_count.ptr == nullptr) {
@@ -601,6 +609,16 @@ struct StageTimer {
   }
 };
 
+gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p);
+
+void release_graph(gbr_renderer* r) {
+  if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+  if (r->graph) (void)hipGraphDestroy(r->graph);
+  r->graph_exec = nullptr;
+  r->graph = nullptr;
+  r->graph_key.clear();
+}
+
 gbr_status do_draw(gbr_renderer* r) {
   TRY(set_device(r));
   TRY(ensure_planes(r));
@@ -653,7 +671,7 @@ gbr_status do_draw(gbr_renderer* r) {
   // visibility feedback is kept in every mode, so switching to the
   // previous-frame proposal starts from the last frame drawn
   p.item_vis = r->d_item_vis.ptr;
-  p.frame = ++r->frame_no;
+  p.frame = r->d_frame.ptr;
   p.proposal_threshold = r->tun.proposal_threshold;
   p.cam_info = r->d_caminfo.ptr;
   const bool learned = p.cull_mode == kCullLearned;
@@ -694,10 +712,48 @@ gbr_status do_draw(gbr_renderer* r) {
   p.top_down = r->cfg.top_down ? 1 : 0;
   p.no_textures = (r->cfg.flags & GBR_FLAG_NO_TEXTURES) ? 1 : 0;
 
+  // every launch below reads its per-frame inputs from device memory, so
+  // with timing and row collection off the sequence is captured once as a
+  // HIP graph and replayed while the parameters stay the same
+  const bool graphable = r->tun.graph && !r->tun.timing && !r->collecting;
+  if (!graphable) return enqueue_frame(r, p);
+  std::vector<uint8_t> key(sizeof p + 8 * sizeof(uint64_t));
+  {
+    const uint64_t extra[8] = {(uint64_t)(uintptr_t)r->d_parents.ptr, (uint64_t)(uintptr_t)r->d_local.ptr,
+                               (uint64_t)(uintptr_t)r->d_abs.ptr, (uint64_t)(uintptr_t)r->d_ritems.ptr,
+                               ((uint64_t)r->n_nodes << 32) | r->n_ritems,
+                               ((uint64_t)r->cu_count << 32) | (uint32_t)r->setup_bpc,
+                               (uint64_t)(uintptr_t)r->vis.Xi.ptr, (uint64_t)(uintptr_t)r->vis.logits_i.ptr};
+    std::memcpy(key.data(), &p, sizeof p);
+    std::memcpy(key.data() + sizeof p, extra, sizeof extra);
+  }
+  if (!r->graph_exec || key != r->graph_key) {
+    release_graph(r);
+    HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeRelaxed));
+    const gbr_status st = enqueue_frame(r, p);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(r->stream, &g);
+    if (st != GBR_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    HIP_TRY(e);
+    r->graph = g;
+    HIP_TRY(hipGraphInstantiate(&r->graph_exec, r->graph, nullptr, nullptr, 0));
+    r->graph_key = std::move(key);
+  }
+  HIP_TRY(hipGraphLaunch(r->graph_exec, r->stream));
+  return GBR_OK;
+}
+
+// The frame's kernel sequence on the renderer's stream (no host
+// synchronisation: list lengths live in device counters).
+gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
+  const bool learned = p.cull_mode == kCullLearned;
   {
     StageTimer t(r, GBR_TIMING_NODES);
     HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, p.counters,
-                         p.stats, r->stream));
+                         p.stats, p.frame, r->stream));
   }
   const uint32_t setup_grid = (uint32_t)(r->cu_count * r->setup_bpc);
   const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
@@ -861,6 +917,7 @@ gbr_status gbr_tuning_init(gbr_tuning* t) {
   t->cull_mode = GBR_CULL_NONE;
   t->queue_grain = 4;
   t->lane_pixels = 96;
+  t->graph = 1;
   return GBR_OK;
 }
 
@@ -873,6 +930,8 @@ static gbr_status validate_tuning(const gbr_tuning* t) {
     return fail(GBR_ERR_ARGUMENT, "tuning: capacity must be positive");
   if (t->raster_mode < 0 || t->raster_mode > 1)
     return fail(GBR_ERR_ARGUMENT, "tuning: raster_mode must be 0 or 1");
+  if (t->graph != 0 && t->graph != 1)
+    return fail(GBR_ERR_ARGUMENT, "tuning: graph must be 0 or 1");
   if (t->lane_pixels < 1 || t->lane_pixels > kMaxBin * kMaxBin)
     return fail(GBR_ERR_ARGUMENT, "tuning: lane_pixels must be in [1, 1024]");
   if (t->timing != 0 && t->timing != 1)
@@ -943,6 +1002,8 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   if (!r) return GBR_OK;
   (void)hipSetDevice(r->device);
   if (r->stream) (void)hipStreamSynchronize(r->stream);
+  release_graph(r);
+  r->d_frame.release();
   DevBuf<float>* fb[] = {&r->d_pos_x, &r->d_pos_y, &r->d_pos_z, &r->d_lut, &r->d_local, &r->d_abs,
                          &r->d_cameras, &r->d_unproj, &r->d_depth, &r->d_lin};
   for (auto* b : fb) b->release();

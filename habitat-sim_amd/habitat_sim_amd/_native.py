@@ -50,7 +50,8 @@ class Tuning(ctypes.Structure):
                 ("bin_capacity_factor", ctypes.c_float), ("min_bin_capacity", ctypes.c_int32),
                 ("cull_mode", ctypes.c_int32), ("queue_grain", ctypes.c_int32),
                 ("timing", ctypes.c_int32), ("raster_mode", ctypes.c_int32),
-                ("proposal_threshold", ctypes.c_float), ("lane_pixels", ctypes.c_int32)]
+                ("proposal_threshold", ctypes.c_float), ("lane_pixels", ctypes.c_int32),
+                ("graph", ctypes.c_int32)]
 
 
 VIS_FEATURES = 32

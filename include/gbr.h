@@ -120,6 +120,10 @@ typedef struct gbr_tuning {
   int32_t lane_pixels;        /* raster_mode 0: triangles covering more than this
                                  many pixels of a bin are rasterised by the whole
                                  workgroup, the rest by one lane each: 1..1024 */
+  int32_t graph;              /* 1: gbr_draw() replays its kernel sequence as a
+                                 captured HIP graph while nothing but cameras and
+                                 transforms changed (timing and row collection
+                                 draw directly) */
 } gbr_tuning;
 
 /* stages timed when gbr_tuning.timing is set (gbr_frame_timings) */
