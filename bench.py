@@ -8,8 +8,10 @@ in HBM (colour RGBA8, linear depth FP32, semantic uint32 planes).  The walk is
 run once per culling mode (plain = every triangle set up, frustum, hiz =
 two-phase depth-pyramid occlusion with the previous frame as proposal,
 learned = the same with the visibility MLP as proposal, trained first on
-other walks); the headline value is the learned mode, and the final frame of
-every mode is compared pixel by pixel with the plain one.
+other walks); each mode's launch parameters come from the library's
+autotuner (gbr_autotune) on the walk's first view; the headline value is the
+learned mode, and the final frame of every mode is compared pixel by pixel
+with the plain one.
 
 Prints ONE JSON line (rank 0).  ``roofline`` is the headline mode's frame
 (its kernels back to back on the renderer's stream): the frozen formula
@@ -151,6 +153,10 @@ def run_mode(r, mode, spec, views, proj, warmup, steps):
     from habitat_sim_amd import hip
     r.set_tuning(cull_mode=mode, timing=0)
     stream = r.stream()
+    # launch parameters for this mode from the autotuner, on the walk's
+    # first views (every candidate renders the same pixels)
+    r.update_cameras(0, proj, views[0])
+    tuned = r.autotune(warmup=2, frames=3)
     for k in range(warmup):
         r.update_cameras(0, proj, views[k])
         r.draw()
@@ -178,7 +184,11 @@ def run_mode(r, mode, spec, views, proj, warmup, steps):
     r.synchronize()
     stages = r.frame_timings()
     r.set_tuning(timing=0)
-    return {"wall": wall, "gpu_ms": gpu_ms, "stages": stages, "stats": stats, "planes": planes}
+    return {"wall": wall, "gpu_ms": gpu_ms, "stages": stages, "stats": stats, "planes": planes,
+            "tuning": {k: tuned["best"][k] for k in ("bin_size", "bin_capacity_factor",
+                                                     "lane_pixels", "queue_grain", "raster_mode")},
+            "autotune": {"candidates": tuned["candidates"], "default_ms": tuned["initial_ms"],
+                         "tuned_ms": tuned["best_ms"]}}
 
 
 def diff_pixels(a, b):
@@ -273,6 +283,8 @@ def main():
             "triangles_setup": x["stats"]["triangles_setup"],
             "bin_entries": x["stats"]["bin_entries"],
             "stage_ms": {k: round(v, 4) for k, v in x["stages"].items() if v > 0},
+            "tuning": x["tuning"],
+            "autotune": x["autotune"],
         }
 
     # roofline of the headline mode's frame (its kernels back to back on one

@@ -1567,6 +1567,88 @@ gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* t) {
   return GBR_OK;
 }
 
+gbr_status gbr_autotune_params_init(gbr_autotune_params* params) {
+  if (!params) return fail(GBR_ERR_ARGUMENT, "autotune_params_init: null");
+  params->warmup = 2;
+  params->frames = 4;
+  params->passes = 1;
+  return GBR_OK;
+}
+
+namespace {
+
+// GPU milliseconds per draw() under tuning t (warmup draws first)
+gbr_status time_tuning(gbr_renderer* r, const gbr_tuning& t, const gbr_autotune_params& ap,
+                       float* ms) {
+  TRY(gbr_set_tuning(r, &t));
+  for (int i = 0; i < ap.warmup; ++i) TRY(do_draw(r));
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  hipError_t e = hipEventCreate(&b);
+  if (e == hipSuccess) e = hipEventRecord(a, r->stream);
+  gbr_status st = GBR_OK;
+  for (int i = 0; e == hipSuccess && st == GBR_OK && i < ap.frames; ++i) st = do_draw(r);
+  if (e == hipSuccess && st == GBR_OK) e = hipEventRecord(b, r->stream);
+  if (e == hipSuccess && st == GBR_OK) e = hipEventSynchronize(b);
+  float total = 0.0f;
+  if (e == hipSuccess && st == GBR_OK) e = hipEventElapsedTime(&total, a, b);
+  (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (st != GBR_OK) return st;
+  HIP_TRY(e);
+  *ms = total / (float)ap.frames;
+  return GBR_OK;
+}
+
+}  // namespace
+
+gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
+                        gbr_autotune_report* report) {
+  if (!r || !params || !report) return fail(GBR_ERR_ARGUMENT, "autotune: null argument");
+  if (params->warmup < 1 || params->frames < 1 || params->passes < 1)
+    return fail(GBR_ERR_ARGUMENT, "autotune: warmup, frames and passes must be >= 1");
+  TRY(set_device(r));
+  std::memset(report, 0, sizeof *report);
+  gbr_tuning best = r->tun;
+  float best_ms = 0.0f;
+  TRY(time_tuning(r, best, *params, &best_ms));
+  report->initial_ms = best_ms;
+  report->candidates = 1;
+  // candidate values per parameter
+  const int bins[] = {8, 16, 32};
+  const float caps[] = {1.0f, 1.5f, 2.0f, 3.0f};
+  const int lanes[] = {32, 64, 96, 128, 192};
+  const int grains[] = {1, 2, 4, 8, 16};
+  const int modes[] = {0, 1};
+  for (int pass = 0; pass < params->passes; ++pass) {
+    for (int param = 0; param < 5; ++param) {
+      const int n = param == 0 ? 3 : param == 1 ? 4 : param == 2 ? 5 : param == 3 ? 5 : 2;
+      for (int k = 0; k < n; ++k) {
+        gbr_tuning t = best;
+        switch (param) {
+          case 0: t.bin_size = bins[k]; break;
+          case 1: t.bin_capacity_factor = caps[k]; break;
+          case 2: t.lane_pixels = lanes[k]; break;
+          case 3: t.queue_grain = grains[k]; break;
+          default: t.raster_mode = modes[k]; break;
+        }
+        if (std::memcmp(&t, &best, sizeof t) == 0) continue;
+        float ms = 0.0f;
+        TRY(time_tuning(r, t, *params, &ms));
+        ++report->candidates;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+    }
+  }
+  TRY(gbr_set_tuning(r, &best));
+  report->best_ms = best_ms;
+  report->best = best;
+  return GBR_OK;
+}
+
 gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
   if (!r || !out) return fail(GBR_ERR_ARGUMENT, "frame_stats: null argument");
   std::memset(out, 0, sizeof *out);

@@ -54,6 +54,16 @@ class Tuning(ctypes.Structure):
                 ("graph", ctypes.c_int32)]
 
 
+class AutotuneParams(ctypes.Structure):
+    _fields_ = [("warmup", ctypes.c_int32), ("frames", ctypes.c_int32),
+                ("passes", ctypes.c_int32)]
+
+
+class AutotuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int32), ("initial_ms", ctypes.c_float),
+                ("best_ms", ctypes.c_float), ("best", Tuning)]
+
+
 VIS_FEATURES = 32
 VIS_PARAMS = 20865
 
@@ -172,6 +182,8 @@ SIGNATURES = {
                                         ctypes.c_int32]),
     "gbr_semantic_image_into": (S, [R, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.c_int32]),
+    "gbr_autotune_params_init": (S, [ctypes.POINTER(AutotuneParams)]),
+    "gbr_autotune": (S, [R, ctypes.POINTER(AutotuneParams), ctypes.POINTER(AutotuneReport)]),
     "gbr_color_device_pointer": (S, [R, ctypes.POINTER(ctypes.c_void_p)]),
     "gbr_depth_device_pointer": (S, [R, ctypes.POINTER(ctypes.c_void_p)]),
     "gbr_plane_device_pointer": (S, [R, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),

@@ -446,6 +446,20 @@ class RendererStandalone:
             setattr(t, k, v)
         N.call("gbr_set_tuning", self._h, ctypes.byref(t))
 
+    def autotune(self, warmup: int = 2, frames: int = 4, passes: int = 1) -> dict:
+        """Times draw() of the current scenes and cameras under candidate
+        launch parameters and keeps the fastest (same pixels for all);
+        returns the candidates timed, ms per frame before and after, and the
+        tuning left set."""
+        ap = N.AutotuneParams()
+        N.call("gbr_autotune_params_init", ctypes.byref(ap))
+        ap.warmup, ap.frames, ap.passes = warmup, frames, passes
+        rep = N.AutotuneReport()
+        N.call("gbr_autotune", self._h, ctypes.byref(ap), ctypes.byref(rep))
+        best = {k: getattr(rep.best, k) for k, _ in rep.best._fields_}
+        return {"candidates": rep.candidates, "initial_ms": rep.initial_ms,
+                "best_ms": rep.best_ms, "best": best}
+
     # ---- learned visibility proposal ------------------------------------
     def vis_init(self, seed: int = 1) -> None:
         """Random visibility-model weights (He-uniform) from seed."""

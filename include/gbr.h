@@ -339,6 +339,27 @@ gbr_status gbr_unproject_depth(const float* unprojection2, float* depth, int64_t
 gbr_status gbr_get_tuning(const gbr_renderer* r, gbr_tuning* out);
 gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* tuning);
 gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out);
+
+/* new: autotuner.  Times gbr_draw() of the current scenes and cameras under
+   candidate launch parameters, one parameter at a time (bin size, per-bin
+   capacity factor, lane/workgroup raster split, setup claim size, raster
+   mode; each from the best values found so far), and leaves the fastest set
+   (every candidate renders the same pixels).  cull_mode, timing, graph,
+   chunk_triangles and proposal_threshold are kept. */
+typedef struct gbr_autotune_params {
+  int32_t warmup; /* untimed draws per candidate (>= 1; absorbs rebuilds) */
+  int32_t frames; /* timed draws per candidate (>= 1) */
+  int32_t passes; /* rounds over the parameters (>= 1) */
+} gbr_autotune_params;
+typedef struct gbr_autotune_report {
+  int32_t candidates;  /* candidates timed */
+  float initial_ms;    /* GPU ms per frame with the tuning on entry */
+  float best_ms;       /* ... with the tuning left set */
+  gbr_tuning best;
+} gbr_autotune_report;
+gbr_status gbr_autotune_params_init(gbr_autotune_params* params);
+gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
+                        gbr_autotune_report* report);
 /* ---- learned visibility proposal (new) -------------------------------------
    No counterpart in the reference, whose batch renderer submits every mesh
    of every scene (Renderer.cpp:1298-1351).  A 32 -> 128 -> 128 -> 1 MLP
