@@ -56,7 +56,8 @@ struct alignas(16) SceneDesc {
   uint32_t prim_count;
   unsigned long long entry_base;  // first bin entry of this scene (bins x bin_cap)
   unsigned long long rec_base;    // first primitive record of this scene
-  unsigned long long pad3;
+  int32_t stride;                 // row stride of the scene's planes (the tile width)
+  uint32_t planes;                // GBR_PLANE_* written for this scene
 };
 
 // One chunk of <= kChunkTris consecutive triangles of a draw; everything a
@@ -195,9 +196,12 @@ struct alignas(16) RasterItem {
   int16_t x0, y0, x1, y1;   // the bin's pixels (inclusive)
   uint32_t bin_cap;         // entries per bin of the scene
   uint32_t plane_offset;    // first pixel of the scene's planes
-  int32_t width, height;    // scene size in pixels
+  int32_t width, height;    // the scene's viewport in pixels
   unsigned long long entries;   // the bin's first entry in bin_entries
   unsigned long long rec_base;  // the scene's first primitive record
+  int32_t stride;           // row stride of the planes (the tile width)
+  uint32_t planes;          // GBR_PLANE_* written for the scene
+  uint32_t pad[2];
 };
 
 struct alignas(16) MaterialDesc {
@@ -287,6 +291,7 @@ struct FrameParams {
 
 struct GatherArgs {
   const uint8_t* plane;
+  const SceneDesc* scenes;  // per-scene viewports
   uint8_t* dst;
   long long dst_stride;
   int x0, y0, w, h;       // rectangle in framebuffer pixels

@@ -24,6 +24,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include "gbr.h"
 #include "gbr_device.h"
 
 #pragma clang fp contract(off)
@@ -1423,7 +1424,8 @@ __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int ph
 struct BinInfo {
   PixRect bin;
   uint32_t gbin, scene, bin_cap, plane_offset;
-  int width, height;
+  int width, height, stride;  // viewport; row stride of the planes
+  uint32_t planes;
   unsigned long long entries, rec_base;
 };
 
@@ -1441,6 +1443,8 @@ __device__ __forceinline__ BinInfo bin_info(const RasterItem& it) {
   b.height = it.height;
   b.entries = it.entries;
   b.rec_base = it.rec_base;
+  b.stride = it.stride;
+  b.planes = it.planes;
   return b;
 }
 
@@ -1475,7 +1479,7 @@ __device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b
   const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
   for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
     const int lx = l % bw, ly = l / bw;
-    p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.width + b.bin.x0 + lx] =
+    p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.stride + b.bin.x0 + lx] =
         s_vis[ly * B + lx];
   }
 }
@@ -1620,7 +1624,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
       const int l = threadIdx.x + q * kBlock;
       const int ly = l / bw, lx = l - ly * bw;
       keys[q] = l < bw * bh
-                    ? p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.width + b.bin.x0 + lx]
+                    ? p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.stride + b.bin.x0 + lx]
                     : 0ull;
     }
 #pragma unroll
@@ -1686,7 +1690,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
     for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
       const int lx = l % bw, ly = l / bw;
       s_vis[ly * B + lx] =
-          p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.width + b.bin.x0 + lx];
+          p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.stride + b.bin.x0 + lx];
     }
   } else {
     for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
@@ -1861,12 +1865,17 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
   const int lane = threadIdx.x & 63;
   const float hw = (float)b.width * 0.5f, hh = (float)b.height * 0.5f;
   const float ihw = 1.0f / hw, ihh = 1.0f / hh;
+  // the scene's sensor set
+  const bool want_color = p.out_color && (b.planes & GBR_PLANE_COLOR);
+  const bool want_depth = p.out_depth && (b.planes & GBR_PLANE_DEPTH);
+  const bool want_lin = p.out_lindepth && (b.planes & GBR_PLANE_LINEAR_DEPTH);
+  const bool want_sem = p.out_semantic && (b.planes & GBR_PLANE_SEMANTIC);
   for (int base = 0; base < npx; base += kBlock) {
     const int l = base + (int)threadIdx.x;
     const bool valid = l < npx;
     const int px = b.bin.x0 + l % bw, py = b.bin.y0 + l / bw;
     const unsigned long long key =
-        valid ? p.vis[(size_t)b.plane_offset + (size_t)py * b.width + px] : kClearKey;
+        valid ? p.vis[(size_t)b.plane_offset + (size_t)py * b.stride + px] : kClearKey;
     const bool drawn = key != kClearKey;
     const uint32_t prim = (uint32_t)key;
     ResolveRec rs;
@@ -1912,23 +1921,23 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     if (!valid) continue;
     const int row = p.top_down ? (b.height - 1 - py) : py;
-    const size_t o = (size_t)b.plane_offset + (size_t)row * b.width + px;
+    const size_t o = (size_t)b.plane_offset + (size_t)row * b.stride + px;
     if (!drawn) {
-      if (p.out_color) p.out_color[o] = kClearColor;
-      if (p.out_depth) p.out_depth[o] = 1.0f;
-      if (p.out_lindepth) p.out_lindepth[o] = 0.0f;
-      if (p.out_semantic) p.out_semantic[o] = 0u;
+      if (want_color) p.out_color[o] = kClearColor;
+      if (want_depth) p.out_depth[o] = 1.0f;
+      if (want_lin) p.out_lindepth[o] = 0.0f;
+      if (want_sem) p.out_semantic[o] = 0u;
       continue;
     }
     const float z = __uint_as_float((uint32_t)(key >> 32));
-    if (p.out_color)
+    if (want_color)
       p.out_color[o] = shade_pixel(p, s_lut, hw, hh, A, bc, s, t, mat, textured, q, d, prim, px, py);
-    if (p.out_depth) p.out_depth[o] = z;
-    if (p.out_lindepth) {
+    if (want_depth) p.out_depth[o] = z;
+    if (want_lin) {
       const float lin = un1 / (z + un0);
       p.out_lindepth[o] = lin == far_depth ? 0.0f : lin;
     }
-    if (p.out_semantic) p.out_semantic[o] = d.semantic;
+    if (want_sem) p.out_semantic[o] = d.semantic;
   }
 }
 
@@ -2248,7 +2257,7 @@ __global__ __launch_bounds__(kBlock) void k_hiz(FrameParams p, uint32_t regions_
     const int ly = (threadIdx.x >> 5) + 8 * q;
     const int py = ry * 32 + ly;
     if (px < sc.width && py < sc.height) {
-      const unsigned long long key = p.vis[(size_t)sc.plane_offset + (size_t)py * sc.width + px];
+      const unsigned long long key = p.vis[(size_t)sc.plane_offset + (size_t)py * sc.stride + px];
       atomicMax(&s_max[(ly >> 3) * 4 + (lx >> 3)], (uint32_t)(key >> 32));  // depth >= 0
     }
   }
@@ -2299,11 +2308,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g) {
   const int tx = fx / g.tile_w, ty = fy / g.tile_h;
   const int scene = ty * g.tiles_x + tx;
   const int px = fx - tx * g.tile_w, py = fy - ty * g.tile_h;
-  const int row = g.top_down ? (g.tile_h - 1 - py) : py;
   uint32_t v = (uint32_t)g.fill;
-  if (scene < g.scene_count)
-    v = *reinterpret_cast<const uint32_t*>(
-        g.plane + (((long long)scene * g.tile_h + row) * g.tile_w + px) * 4);
+  if (scene < g.scene_count) {
+    // the scene's viewport is the bottom-left of its tile; the rest is clear
+    const int vw = g.scenes[scene].width, vh = g.scenes[scene].height;
+    if (px < vw && py < vh) {
+      const int row = g.top_down ? (vh - 1 - py) : py;
+      v = *reinterpret_cast<const uint32_t*>(
+          g.plane + (((long long)scene * g.tile_h + row) * g.tile_w + px) * 4);
+    }
+  }
   uint8_t* d = g.dst + (long long)ly * g.dst_stride + (long long)lx * g.dst_bytes;
   if (g.dst_bytes == 4) {
     *reinterpret_cast<uint32_t*>(d) = v;

@@ -146,6 +146,8 @@ struct Scene {
   float unproj[2] = {0.0f, 0.0f};
   float cam_info[8] = {0, 0, 0, 0, 0, -1, 0, 0};  // camera position, forward (world)
   bool xform_dirty = true;
+  int32_t vp_w = 0, vp_h = 0;  // viewport (0: the whole tile)
+  uint32_t planes = 0;         // sensor set (0: every configured plane)
 };
 
 float unorm(int i) { return (float)i / 255.0f; }
@@ -375,11 +377,17 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   const uint32_t nsc = r->scene_count();
   const int W = r->cfg.tile_size[0], H = r->cfg.tile_size[1];
   const int B = r->tun.bin_size;
-  const int bins_x = (W + B - 1) / B, bins_y = (H + B - 1) / B;
   const uint32_t chunk = (uint32_t)r->tun.chunk_triangles;
-  // every scene has the same bin grid; larger grids than the LDS counters
-  // hold append each pair straight to the global bins
-  r->lds_bins = bins_x * bins_y <= kMaxSceneBins ? (uint32_t)(bins_x * bins_y) : 0u;
+  // bin grids follow the scenes' viewports; setup counts a scene's pairs in
+  // LDS when its grid fits the largest grid that does (kMaxSceneBins), and
+  // appends them straight to the global bins otherwise
+  r->lds_bins = 0;
+  for (uint32_t si = 0; si < nsc; ++si) {
+    const Scene& sc = r->scenes[si];
+    const int vw = sc.vp_w ? sc.vp_w : W, vh = sc.vp_h ? sc.vp_h : H;
+    const uint32_t nb = (uint32_t)(((vw + B - 1) / B) * ((vh + B - 1) / B));
+    if (nb <= (uint32_t)kMaxSceneBins) r->lds_bins = std::max(r->lds_bins, nb);
+  }
   r->setup_bpc = setup_blocks_per_cu(r->lds_bins);
 
   std::vector<int32_t> parents;
@@ -407,8 +415,12 @@ gbr_status rebuild_structure(gbr_renderer* r) {
       for (uint32_t i = 0; i < sc.draws.size(); ++i)
         if (sc.draws[i].batch == b) order.push_back(i);
     SceneDesc& sd = r->scene_desc[si];
-    sd.width = W;
-    sd.height = H;
+    const int vw = sc.vp_w ? sc.vp_w : W, vh = sc.vp_h ? sc.vp_h : H;
+    const int bins_x = (vw + B - 1) / B, bins_y = (vh + B - 1) / B;
+    sd.width = vw;
+    sd.height = vh;
+    sd.stride = W;
+    sd.planes = sc.planes ? (sc.planes & r->cfg.planes) : r->cfg.planes;
     sd.bins_x = bins_x;
     sd.bins_y = bins_y;
     sd.draw_begin = (uint32_t)draws.size();
@@ -469,14 +481,16 @@ gbr_status rebuild_structure(gbr_renderer* r) {
       const int bx = (int)(b % (uint32_t)bins_x), by = (int)(b / (uint32_t)bins_x);
       ri.x0 = (int16_t)(bx * B);
       ri.y0 = (int16_t)(by * B);
-      ri.x1 = (int16_t)(std::min(bx * B + B, W) - 1);
-      ri.y1 = (int16_t)(std::min(by * B + B, H) - 1);
+      ri.x1 = (int16_t)(std::min(bx * B + B, vw) - 1);
+      ri.y1 = (int16_t)(std::min(by * B + B, vh) - 1);
       ri.bin_cap = cap;
       ri.plane_offset = sd.plane_offset;
-      ri.width = W;
-      ri.height = H;
+      ri.width = vw;
+      ri.height = vh;
       ri.entries = sd.entry_base + (uint64_t)b * cap;
       ri.rec_base = sd.rec_base;
+      ri.stride = W;
+      ri.planes = sd.planes;
       ritems.push_back(ri);
     }
   }
@@ -840,6 +854,7 @@ gbr_status gather_plane(gbr_renderer* r, uint32_t plane, const int32_t* rect, vo
   if (!src) return fail(GBR_ERR_STATE, "image_into: nothing drawn yet");
   GatherArgs g{};
   g.plane = src;
+  g.scenes = r->d_scenes.ptr;
   g.x0 = rect[0];
   g.y0 = rect[1];
   g.w = w;
@@ -1378,6 +1393,40 @@ gbr_status gbr_clear(gbr_renderer* r, uint32_t scene) {
   sc.draws.clear();
   sc.batches.clear();
   r->structure_dirty = true;
+  return GBR_OK;
+}
+
+gbr_status gbr_set_scene_viewport(gbr_renderer* r, uint32_t scene, int32_t width, int32_t height) {
+  TRY(check_scene(r, scene, "set_scene_viewport"));
+  if (width < 1 || height < 1 || width > r->cfg.tile_size[0] || height > r->cfg.tile_size[1])
+    return fail(GBR_ERR_SIZE, "set_scene_viewport: " + std::to_string(width) + "x" +
+                                  std::to_string(height) + " doesn't fit a tile of " +
+                                  std::to_string(r->cfg.tile_size[0]) + "x" +
+                                  std::to_string(r->cfg.tile_size[1]));
+  Scene& sc = r->scenes[scene];
+  const int32_t w = width == r->cfg.tile_size[0] && height == r->cfg.tile_size[1] ? 0 : width;
+  const int32_t h = w ? height : 0;
+  if (sc.vp_w != w || sc.vp_h != h) r->structure_dirty = true;
+  sc.vp_w = w;
+  sc.vp_h = h;
+  return GBR_OK;
+}
+
+gbr_status gbr_scene_viewport(const gbr_renderer* r, uint32_t scene, int32_t* out2) {
+  TRY(check_scene(r, scene, "scene_viewport"));
+  if (!out2) return fail(GBR_ERR_ARGUMENT, "scene_viewport: null output");
+  const Scene& sc = r->scenes[scene];
+  out2[0] = sc.vp_w ? sc.vp_w : r->cfg.tile_size[0];
+  out2[1] = sc.vp_h ? sc.vp_h : r->cfg.tile_size[1];
+  return GBR_OK;
+}
+
+gbr_status gbr_set_scene_planes(gbr_renderer* r, uint32_t scene, uint32_t planes) {
+  TRY(check_scene(r, scene, "set_scene_planes"));
+  if (planes & ~r->cfg.planes)
+    return fail(GBR_ERR_ARGUMENT, "set_scene_planes: planes outside gbr_config.planes");
+  if (r->scenes[scene].planes != planes) r->structure_dirty = true;
+  r->scenes[scene].planes = planes;
   return GBR_OK;
 }
 

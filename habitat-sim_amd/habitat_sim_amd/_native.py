@@ -161,6 +161,9 @@ SIGNATURES = {
     "gbr_add_light": (S, [R, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
                           ctypes.POINTER(ctypes.c_uint64)]),
     "gbr_clear": (S, [R, ctypes.c_uint32]),
+    "gbr_set_scene_viewport": (S, [R, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32]),
+    "gbr_scene_viewport": (S, [R, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]),
+    "gbr_set_scene_planes": (S, [R, ctypes.c_uint32, ctypes.c_uint32]),
     "gbr_clear_lights": (S, [R, ctypes.c_uint32]),
     "gbr_camera": (S, [R, ctypes.c_uint32, ctypes.c_void_p]),
     "gbr_camera_depth_unprojection": (S, [R, ctypes.c_uint32, ctypes.c_void_p]),

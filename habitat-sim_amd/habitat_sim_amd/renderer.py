@@ -307,6 +307,21 @@ class RendererStandalone:
     def clear(self, scene: int) -> None:
         N.call("gbr_clear", self._h, scene)
 
+    def set_scene_viewport(self, scene: int, size) -> None:
+        """Render the scene into the bottom-left size = (width, height)
+        pixels of its tile (new: per-scene resolutions in one batch)."""
+        N.call("gbr_set_scene_viewport", self._h, scene, int(size[0]), int(size[1]))
+
+    def scene_viewport(self, scene: int) -> tuple:
+        out = (ctypes.c_int32 * 2)()
+        N.call("gbr_scene_viewport", self._h, scene, out)
+        return int(out[0]), int(out[1])
+
+    def set_scene_planes(self, scene: int, planes: int) -> None:
+        """Sensor set of the scene: the PLANE_* bits draw() writes for it
+        (0 = every configured plane)."""
+        N.call("gbr_set_scene_planes", self._h, scene, planes)
+
     def clear_lights(self, scene: int) -> None:
         N.call("gbr_clear_lights", self._h, scene)
 

@@ -61,6 +61,8 @@ class EnvSpec:
     instances: List[Instance]
     camera_pose: np.ndarray      # camera-to-world
     projection: np.ndarray
+    viewport: Optional[Tuple[int, int]] = None  # (w, h) inside the tile; None = the tile
+    planes: int = 0              # sensor set (PLANE_* bits; 0 = every plane)
 
 
 @dataclass
@@ -91,6 +93,10 @@ def populate(renderer, spec: BatchSpec, first_scene: int = 0, add_file: bool = T
         for inst in env.instances:
             renderer.add_node_hierarchy(s, inst.template, inst.bake, inst.semantic)
         renderer.update_camera(s, env.projection, m3.inverted_rigid(env.camera_pose))
+        if env.viewport is not None:
+            renderer.set_scene_viewport(s, env.viewport)
+        if env.planes and hasattr(renderer, "set_scene_planes"):
+            renderer.set_scene_planes(s, env.planes)
 
 
 # ---------------------------------------------------------------------------
@@ -414,3 +420,51 @@ def apartments(n_envs: int = 512, tile: int = 128, instances: int = 300,
         hi = ((room[0] + 1) * ROOM - 0.3, (room[1] + 1) * ROOM - 0.3)
         envs.append(EnvSpec(insts, random_pose_in_box(rng, lo, hi), P))
     return BatchSpec("apartments", (tile, tile), fd, envs, tris)
+
+
+# sensor planes (the library's PLANE_* bits)
+PLANE_COLOR, PLANE_DEPTH, PLANE_LINEAR_DEPTH, PLANE_SEMANTIC = 0x1, 0x2, 0x4, 0x8
+
+
+def heterogeneous(n_envs: int = 1024, tri_range=(1e4, 1e6), resolutions=(64, 128, 256),
+                  res_p=(0.3, 0.5, 0.2), config_index: int = 2, template_file=None) -> BatchSpec:
+    """Uneven batch: per env a triangle budget drawn log-uniformly from
+    tri_range (apartment shell plus as many templates as fit), a square
+    resolution from `resolutions` with probabilities res_p (the env's
+    viewport in a tile of the largest resolution), and a sensor set of 1-3
+    of colour, depth and semantic (linear depth comes with depth).  Cameras
+    as in apartments()."""
+    fd, tris, weights = template_file or apartment_templates(1)
+    tile = max(resolutions)
+    shell = tris["apartment"]
+    n_t = len(weights)
+    mean_t = float(sum(w * tris[f"obj{i}"] for i, w in enumerate(weights)))
+    sensor_sets = [PLANE_COLOR, PLANE_DEPTH | PLANE_LINEAR_DEPTH, PLANE_SEMANTIC]
+    envs = []
+    for e in range(n_envs):
+        rng = env_rng(config_index, e)
+        budget = math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))
+        count = max(0, int(round((budget - shell) / mean_t)))
+        res = int(rng.choice(resolutions, p=res_p))
+        k = int(rng.integers(1, 4))
+        chosen = rng.choice(3, size=k, replace=False)
+        planes = 0
+        for c in chosen:
+            planes |= sensor_sets[int(c)]
+        P = m3.pinhole_projection((res, res), math.radians(90.0), 0.01, 1000.0)
+        insts = [Instance("apartment", m3.identity(), 1)]
+        picks = rng.choice(n_t, size=count, p=weights)
+        for i in range(count):
+            s = rng.uniform(0.3, 1.2)
+            x = rng.uniform(0.5, ROOM * GRID - 0.5)
+            z = rng.uniform(0.5, ROOM * GRID - 0.5)
+            yaw = rng.uniform(0, 2 * math.pi)
+            bake = m3.mul(m3.mul(m3.translation((x, 0.5 * s, z)), m3.rotation_y(yaw)),
+                          m3.scaling(s))
+            insts.append(Instance(f"obj{int(picks[i])}", bake, 2 + i))
+        room = rng.integers(0, GRID, 2)
+        lo = (room[0] * ROOM + 0.3, room[1] * ROOM + 0.3)
+        hi = ((room[0] + 1) * ROOM - 0.3, (room[1] + 1) * ROOM - 0.3)
+        vp = None if res == tile else (res, res)
+        envs.append(EnvSpec(insts, random_pose_in_box(rng, lo, hi), P, vp, planes))
+    return BatchSpec("heterogeneous", (tile, tile), fd, envs, tris)

@@ -272,6 +272,16 @@ gbr_status gbr_add_light(gbr_renderer* r, uint32_t scene, uint64_t node, int32_t
 /* clear() Renderer.h:631, clearLights() Renderer.h:640 */
 gbr_status gbr_clear(gbr_renderer* r, uint32_t scene);
 gbr_status gbr_clear_lights(gbr_renderer* r, uint32_t scene);
+/* new: heterogeneous batches.  A scene renders into the bottom-left
+   width x height pixels of its tile (1 <= width <= tile width, same for
+   height; the projection maps onto that viewport) and the rest of the tile
+   reads back as clear; by default the viewport is the whole tile.  A scene's
+   sensor set (a subset of gbr_config.planes; 0 = all of them) limits the
+   planes draw() writes for it; its other planes hold unspecified values.
+   The reference's tiles are uniform (Renderer.h:51-113). */
+gbr_status gbr_set_scene_viewport(gbr_renderer* r, uint32_t scene, int32_t width, int32_t height);
+gbr_status gbr_scene_viewport(const gbr_renderer* r, uint32_t scene, int32_t* out2);
+gbr_status gbr_set_scene_planes(gbr_renderer* r, uint32_t scene, uint32_t planes);
 
 /* ---- cameras and transformations ----------------------------------------- */
 /* camera() Renderer.h:647: combined projection*view */

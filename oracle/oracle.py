@@ -156,6 +156,7 @@ class _Scene:
     camera: np.ndarray = field(default_factory=identity)
     projection: np.ndarray = field(default_factory=identity)
     unprojection: np.ndarray = field(default_factory=lambda: np.zeros(2, np.float32))
+    viewport: Optional[Tuple[int, int]] = None  # (w, h) in the tile's bottom-left
 
 
 class OracleRenderer:
@@ -332,7 +333,16 @@ class OracleRenderer:
     def clear(self, scene_id: int) -> None:
         self.scenes[scene_id] = _Scene(camera=self.scenes[scene_id].camera,
                                        projection=self.scenes[scene_id].projection,
-                                       unprojection=self.scenes[scene_id].unprojection)
+                                       unprojection=self.scenes[scene_id].unprojection,
+                                       viewport=self.scenes[scene_id].viewport)
+
+    def set_scene_viewport(self, scene_id: int, size) -> None:
+        """The scene renders into the bottom-left size = (w, h) pixels of its
+        tile (the viewport transform uses w, h); the rest stays clear."""
+        w, h = int(size[0]), int(size[1])
+        tw, th = self.tile_size
+        assert 1 <= w <= tw and 1 <= h <= th
+        self.scenes[scene_id].viewport = None if (w, h) == (tw, th) else (w, h)
 
     def update_camera(self, scene_id: int, projection, view) -> None:
         sc = self.scenes[scene_id]
@@ -396,6 +406,20 @@ class OracleRenderer:
             out["draw"] = np.empty((n, th, tw), np.int32)
             out["tri"] = np.empty((n, th, tw), np.int32)
         run_job(job, out, nthreads)
+        # scenes with a viewport: their rows are packed w wide at the start
+        # of their slot; spread them into the tile, clear elsewhere
+        clear = dict(color=np.array([0x1f, 0x1f, 0x1f, 0xff], np.uint8), depth=1.0, lindepth=0.0,
+                     semantic=0, count=0, draw=-1, tri=-1)
+        for i, s in enumerate(ids):
+            vp = self.scenes[s].viewport
+            if vp is None:
+                continue
+            w, h = vp
+            for k, a in out.items():
+                flat = a[i].reshape((th * tw,) + a.shape[3:])
+                packed = flat[:w * h].reshape((h, w) + a.shape[3:]).copy()
+                a[i][...] = clear[k]
+                a[i][:h, :w] = packed
         return out
 
     def framebuffer(self, planes: Dict[str, np.ndarray], key: str = "color") -> np.ndarray:
@@ -489,7 +513,8 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
             df.append(f)
         npar.extend(sc.parents)
         nloc.extend(np.asarray(t, np.float32).reshape(16) for t in sc.transforms)
-        ei.append([tw, th, node_start, len(sc.parents), draw_start, len(sd),
+        vw, vh = sc.viewport if sc.viewport is not None else (tw, th)
+        ei.append([vw, vh, node_start, len(sc.parents), draw_start, len(sd),
                    NO_TEXTURES if (r.flags & NO_TEXTURES) else 0, pix])
         e = np.zeros(32, np.float32)
         e[:16] = sc.camera.reshape(-1)

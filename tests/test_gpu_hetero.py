@@ -1,0 +1,126 @@
+"""Uneven batches in one draw(): per-env triangle counts, viewports (64, 128
+and 256 pixels in 256-pixel tiles) and sensor sets.  Every plane a scene
+produces matches the oracle bit for bit, the tile area outside a viewport
+reads back clear, and the culling modes (including the learned proposal on
+1024-triangle chunks) change no pixel."""
+import os
+
+import numpy as np
+import pytest
+
+from habitat_sim_amd import RendererConfiguration, RendererStandalone, scenes, visibility
+from habitat_sim_amd import _native as N
+from oracle.oracle import OracleRenderer
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+PLANE_OF = {"color": N.PLANE_COLOR, "depth": N.PLANE_DEPTH,
+            "linear_depth": N.PLANE_LINEAR_DEPTH, "semantic": N.PLANE_SEMANTIC}
+CLEAR = {"color": (0x1f, 0x1f, 0x1f, 0xff), "depth": 1.0, "linear_depth": 0.0, "semantic": 0}
+
+
+def _spec(n=10):
+    return scenes.heterogeneous(n_envs=n, tri_range=(2e3, 3e4))
+
+
+def _renderer(spec, mode=N.CULL_NONE):
+    gx, gy = spec.grid
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy)))
+    r.set_tuning(cull_mode=mode)
+    scenes.populate(r, spec)
+    return r
+
+
+def _same_sets(a, b, spec, tag=""):
+    """Planes equal on every scene that produces them (a plane outside a
+    scene's sensor set holds unspecified values)."""
+    for name, bit in PLANE_OF.items():
+        x = getattr(a, f"{name}_image")()
+        y = getattr(b, f"{name}_image")()
+        for s, env in enumerate(spec.envs):
+            if env.planes and not env.planes & bit:
+                continue
+            d = _tile(x, spec, s) != _tile(y, spec, s)
+            if d.ndim == 3:
+                d = d.any(-1)
+            assert not d.any(), f"{tag} scene {s} {name}: {int(d.sum())} pixels differ"
+
+
+def _tile(img, spec, s):
+    gx, _ = spec.grid
+    tw, th = spec.tile
+    x, y = s % gx, s // gx
+    return img[y * th:(y + 1) * th, x * tw:(x + 1) * tw]
+
+
+def test_uneven_batch_matches_the_oracle():
+    spec = _spec()
+    assert len({e.viewport for e in spec.envs}) > 1 and len({e.planes for e in spec.envs}) > 2
+    r = _renderer(spec)
+    r.draw()
+    gx, gy = spec.grid
+    o = OracleRenderer(spec.tile, (gx, gy))
+    scenes.populate(o, spec)
+    o.draw(NTHREADS)
+    for name, bit in PLANE_OF.items():
+        a = getattr(r, f"{name}_image")()
+        b = getattr(o, f"{name}_image")()
+        for s, env in enumerate(spec.envs):
+            ta, tb = _tile(a, spec, s), _tile(b, spec, s)
+            w, h = env.viewport or spec.tile
+            if not env.planes & bit:
+                continue  # not in this scene's sensor set
+            diff = ta != tb
+            if diff.ndim == 3:
+                diff = diff.any(-1)
+            assert not diff.any(), f"scene {s} {name}: {int(diff.sum())} pixels differ"
+            # outside the viewport: clear
+            clear = np.asarray(CLEAR[name], ta.dtype)
+            assert (ta[h:] == clear).all() and (ta[:, w:] == clear).all(), (s, name)
+    assert len(np.unique(r.semantic_image())) > 10
+    r.close()
+
+
+def test_viewport_and_sensor_set_validation():
+    spec = scenes.small_rooms(n_envs=1)
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count(spec.tile, (1, 1)))
+    with pytest.raises(Exception):
+        r.set_scene_viewport(0, (spec.tile[0] + 1, 8))
+    with pytest.raises(Exception):
+        r.set_scene_viewport(0, (0, 8))
+    r.set_scene_viewport(0, (32, 16))
+    assert r.scene_viewport(0) == (32, 16)
+    r.set_scene_viewport(0, spec.tile)
+    assert r.scene_viewport(0) == tuple(spec.tile)
+    r.close()
+
+
+@pytest.mark.parametrize("mode", [N.CULL_FRUSTUM, N.CULL_HIZ])
+def test_uneven_batch_culling_changes_no_pixel(mode):
+    spec = _spec(16)
+    plain = _renderer(spec)
+    culled = _renderer(spec, mode)
+    for k in range(3):
+        plain.draw()
+        culled.draw()
+        _same_sets(plain, culled, spec, tag=f"frame {k}")
+    plain.close()
+    culled.close()
+
+
+def test_uneven_batch_learned_cull_on_1024_triangle_chunks():
+    spec = _spec(16)
+    plain = _renderer(spec)
+    learned = _renderer(spec)
+    assert learned.tuning().chunk_triangles == 1024
+    visibility.train(learned, spec, frames=8, steps=60, seed=5, walk_length=4, batch=2048)
+    learned.set_tuning(cull_mode=N.CULL_LEARNED)
+    for k in range(3):
+        plain.draw()
+        learned.draw()
+        _same_sets(plain, learned, spec, tag=f"frame {k}")
+    st = learned.frame_stats()
+    assert st["items_phase1"] > 0
+    plain.close()
+    learned.close()
