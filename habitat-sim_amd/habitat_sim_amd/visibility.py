@@ -84,3 +84,27 @@ def train(renderer, spec: BatchSpec, frames: int = 64, steps: int = 600, seed: i
         renderer.update_camera(e, env.projection, m3.inverted_rigid(env.camera_pose))
     renderer.set_tuning(cull_mode=mode)
     return report
+
+
+def online_walk(renderer, spec: BatchSpec, walk: list, finetune_every: int = 50,
+                fit_steps: int = 20, batch: int = 4096, learning_rate: float = 1e-3,
+                seed: int = 1, on_frame=None) -> list:
+    """Renders a camera walk (per step one view matrix per env) in the
+    renderer's current cull mode while fine-tuning the visibility model
+    online: every frame also appends its training rows, and every
+    `finetune_every` frames the model takes `fit_steps` Adam steps on the
+    rows gathered since the previous fine-tune (then they are dropped).
+    on_frame(step) is called after each frame; returns the fit reports."""
+    reports = []
+    projections = np.stack([e.projection for e in spec.envs])
+    renderer.vis_clear_rows()
+    for k, views in enumerate(walk):
+        renderer.update_cameras(0, projections, np.stack(views))
+        renderer.vis_collect()
+        if on_frame is not None:
+            on_frame(k)
+        if (k + 1) % finetune_every == 0:
+            reports.append(renderer.vis_fit(steps=fit_steps, batch=batch,
+                                            learning_rate=learning_rate, seed=seed + k))
+            renderer.vis_clear_rows()
+    return reports

@@ -156,3 +156,38 @@ def test_learned_proposal_without_history(trained):
     for k in range(len(frames)):
         np.testing.assert_array_equal(res[N.CULL_LEARNED][1][k], res[N.CULL_FRUSTUM][1][k])
     assert res[N.CULL_LEARNED][0] < res[N.CULL_FRUSTUM][0]
+
+
+def test_online_fine_tuning_walk_stays_exact():
+    # a walk in learned mode with rows collected every frame and a short
+    # fine-tune every 5 frames (the 4096-env x 500-step configuration at a
+    # small scale): every frame equals the unculled one
+    spec = scenes.apartments(n_envs=24, tile=64, instances=60)
+    gx, gy = spec.grid
+
+    def make():
+        r = RendererStandalone(RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy)))
+        scenes.populate(r, spec)
+        return r
+
+    plain, learned = make(), make()
+    visibility.train(learned, spec, frames=8, steps=60, seed=2, walk_length=4, batch=2048)
+    learned.set_tuning(cull_mode=N.CULL_LEARNED)
+    walk = visibility.camera_walk(spec, 15, seed=9)
+    P = np.stack([e.projection for e in spec.envs])
+    w0 = learned.vis_weights().copy()
+
+    def check(k):
+        plain.update_cameras(0, P, np.stack(walk[k]))
+        plain.draw()
+        for name in ("color", "depth", "semantic"):
+            a = getattr(plain, f"{name}_image")()
+            b = getattr(learned, f"{name}_image")()
+            assert np.array_equal(a, b), (k, name)
+
+    reports = visibility.online_walk(learned, spec, walk, finetune_every=5, fit_steps=10,
+                                     batch=1024, on_frame=check)
+    assert len(reports) == 3 and all(np.isfinite(r["loss_last"]) for r in reports)
+    assert not np.array_equal(learned.vis_weights(), w0)  # the model moved
+    plain.close()
+    learned.close()
