@@ -395,6 +395,38 @@ __device__ __forceinline__ void vertex_uv(const FrameParams& p, uint32_t flags, 
   t = b;
 }
 
+// a vertex's texture coordinates as stored (half pair, or two floats' bits)
+__device__ __forceinline__ void load_uv_raw(const FrameParams& p, uint32_t flags, uint32_t v,
+                                            uint32_t& a, uint32_t& b) {
+  if (flags & kDrawUvHalf) {
+    a = p.uv16[v];
+    b = 0u;
+  } else {
+    const float2 t2 = p.uv32[v];
+    a = __float_as_uint(t2.x);
+    b = __float_as_uint(t2.y);
+  }
+}
+
+// vertex_uv() on coordinates already loaded
+__device__ __forceinline__ void uv_from_raw(uint32_t flags, uint32_t a, uint32_t b, const float* tm,
+                                            float& s, float& t) {
+  float u, vv;
+  if (flags & kDrawUvHalf) {
+    u = __half2float(__ushort_as_half((unsigned short)(a & 0xffff)));
+    vv = __half2float(__ushort_as_half((unsigned short)(a >> 16)));
+  } else {
+    u = __uint_as_float(a);
+    vv = __uint_as_float(b);
+  }
+  float x = tm[0] * u; float x1 = tm[3] * vv; x = x + x1; x = x + tm[6];
+  float y = tm[1] * u; float y1 = tm[4] * vv; y = y + y1; y = y + tm[7];
+  s = x;
+  t = y;
+}
+
+constexpr int kVtxPerThread = (kVertexCache + kBlock - 1) / kBlock;  // vertex-cache slots per thread
+
 // wave-aggregated append of one PrimRef per flagged lane (ballot + prefix
 // popcount, one atomic per wave)
 // Depth-pyramid test: true iff every fragment with depth >= zmin inside the
@@ -534,10 +566,13 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
     }
     ++grab;
     have_pre = false;
-    // the next item of the chunk: its index now, its descriptor after the
-    // triangle loop, each load overlapping this item's work
+    // the next item of the chunk: its descriptor now (scalar loads, done by
+    // the end of this item's vertex stage)
     const bool prefetch = grab < grab_end;
-    if (prefetch) pre_item = list ? uniform_load(list, grab) : grab;
+    if (prefetch) {
+      pre_item = list ? uniform_load(list, grab) : grab;
+      pre_wi = uniform_load(p.items, pre_item);
+    }
     tri_acc += wi.tri_count;
     vtx_acc += wi.vcount;
     const SceneDesc sc = uniform_load(p.scenes, wi.scene);
@@ -561,9 +596,26 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
 
     for (int b = threadIdx.x; b < nlb; b += kBlock) s_bcount[b] = 0u;
     if (threadIdx.x == 0) s_npairs = nlb ? 0u : (uint32_t)kPairCap;
+    // the item's triangle indices, loaded while its vertex stage runs
+    uint32_t ix[kChunkTris / kBlock][3];
+#pragma unroll
+    for (int k = 0; k < kChunkTris / kBlock; ++k) {
+      const uint32_t j = (uint32_t)k * kBlock + threadIdx.x;
+      if (cached && j < wi.tri_count) {
+        const uint32_t* ip = p.indices + wi.first_index + 3 * j;
+        ix[k][0] = ip[0];
+        ix[k][1] = ip[1];
+        ix[k][2] = ip[2];
+      }
+    }
     if (cached) {
-      for (uint32_t v = threadIdx.x; v < wi.vcount; v += kBlock) {
+#pragma unroll
+      for (int q = 0; q < kVtxPerThread; ++q) {
+        const uint32_t v = threadIdx.x + (uint32_t)q * kBlock;
+        if (v >= wi.vcount) break;
         const uint32_t gv = wi.vmin + v;
+        uint32_t ua, ub;
+        load_uv_raw(p, d.flags, gv, ua, ub);
         const float4 c = vertex_clip(M, PV, p.pos_x[gv], p.pos_y[gv], p.pos_z[gv]);
         const Snap s = snap_vertex(c, hw, hh, gx, gy);
         s_x[v] = s.X;
@@ -574,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
         s_cy[v] = c.y;
         s_cw[v] = c.w;
         float ss, tt;
-        vertex_uv(p, d.flags, gv, tm, ss, tt);
+        uv_from_raw(d.flags, ua, ub, tm, ss, tt);
         s_s[v] = ss;
         s_t[v] = tt;
       }
@@ -583,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
 
     const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
     const int sh = p.bin_shift;
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < kChunkTris / kBlock; ++k) {
       const uint32_t j = (uint32_t)k * kBlock + threadIdx.x;
       bool clip_me = false;
@@ -591,8 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
       if (j < wi.tri_count && !cached) {
         clip_me = true;  // vertex range beyond the LDS cache: k_setup_clip takes it whole
       } else if (j < wi.tri_count) {
-        const uint32_t* ip = p.indices + wi.first_index + 3 * j;
-        const uint32_t vi[3] = {ip[0], ip[1], ip[2]};
+        const uint32_t vi[3] = {ix[k][0], ix[k][1], ix[k][2]};
         int32_t X[3], Y[3];
         float Z[3];
         uint32_t f[3];
@@ -659,10 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
       append_ref(p, kCtrClipped * kCtrStride, clip_me, PrimRef{wi.draw, prim, item, 0u}, p.clipped,
                  p.clipped_cap, lane, lt_mask);
     }
-    if (prefetch) {
-      pre_wi = uniform_load(p.items, pre_item);
-      have_pre = true;
-    }
+    have_pre = prefetch;
     __syncthreads();
     // one global reservation per touched bin
     for (int b = threadIdx.x; b < nlb; b += kBlock) {
