@@ -63,6 +63,7 @@ class MeshData:
     colors: Optional[np.ndarray] = None   # (V, 4) float32 or uint8
     texcoord_half: bool = False           # store texture coordinates as fp16
     index_size: int = 4                   # byte size of the source index type
+    normals: Optional[np.ndarray] = None  # (V, 3) float32 (shaded materials)
 
 
 @dataclass
@@ -494,6 +495,9 @@ def load_gltf(path: str) -> FileData:
             if "TEXCOORD_0" in at:
                 uv = g.accessor(at["TEXCOORD_0"], normalized_float=True).astype(np.float32)
                 uv[:, 1] = np.float32(1.0) - uv[:, 1]
+            nrm = None
+            if "NORMAL" in at:
+                nrm = np.ascontiguousarray(g.accessor(at["NORMAL"]).astype(np.float32))
             col = None
             if "COLOR_0" in at:
                 c = g.accessor(at["COLOR_0"], normalized_float=True).astype(np.float32)
@@ -502,7 +506,7 @@ def load_gltf(path: str) -> FileData:
                 col = c
             ids.append((len(fd.meshes), prim.get("material", -1)))
             fd.meshes.append(MeshData(positions=np.ascontiguousarray(pos), indices=idx,
-                                      texcoords=uv, colors=col, index_size=isz))
+                                      texcoords=uv, colors=col, index_size=isz, normals=nrm))
         prim_ids.append(ids)
 
     # scene: nodes in index order, parents from children lists

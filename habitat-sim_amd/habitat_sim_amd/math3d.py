@@ -143,3 +143,18 @@ def camera_pose(position, yaw: float, pitch: float) -> np.ndarray:
     m = mul(rotation_y(yaw), rotation_x(pitch))
     m[3, :3] = np.asarray(position, np.float32)
     return m
+
+
+def look_at(eye, target, up) -> np.ndarray:
+    """Matrix4::lookAt(eye, target, up): camera-style transformation whose
+    -Z looks from eye at target (columns right, up, backward, eye)."""
+    eye = np.asarray(eye, np.float32)
+    backward = eye - np.asarray(target, np.float32)
+    backward = (backward / np.float32(np.sqrt(np.dot(backward, backward)))).astype(np.float32)
+    right = np.cross(np.asarray(up, np.float32), backward).astype(np.float32)
+    right = (right / np.float32(np.sqrt(np.dot(right, right)))).astype(np.float32)
+    real_up = np.cross(backward, right).astype(np.float32)
+    m = np.zeros((4, 4), np.float32)
+    m[0, :3], m[1, :3], m[2, :3], m[3, :3] = right, real_up, backward, eye
+    m[3, 3] = 1.0
+    return m

@@ -52,6 +52,9 @@ class _Job(ctypes.Structure):
         ("out_lindepth", ctypes.c_void_p), ("out_semantic", ctypes.c_void_p),
         ("out_count", ctypes.c_void_p), ("out_draw", ctypes.c_void_p),
         ("out_tri", ctypes.c_void_p),
+        ("nrm", ctypes.c_void_p), ("draw_diffuse", ctypes.c_void_p),
+        ("env_light", ctypes.c_void_p), ("light_i", ctypes.c_void_p),
+        ("light_f", ctypes.c_void_p),
     ]
 
 
@@ -157,6 +160,9 @@ class _Scene:
     projection: np.ndarray = field(default_factory=identity)
     unprojection: np.ndarray = field(default_factory=lambda: np.zeros(2, np.float32))
     viewport: Optional[Tuple[int, int]] = None  # (w, h) in the tile's bottom-left
+    lights: List[Tuple[int, int]] = field(default_factory=list)  # (node, type)
+    light_colors: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    light_ranges: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
 
 
 class OracleRenderer:
@@ -177,9 +183,11 @@ class OracleRenderer:
                      wrap_s=GL_REPEAT, wrap_t=GL_REPEAT)
         self.textures: List[dict] = [white]
         self.meshes: List[dict] = []
-        # material: ambient rgba, (texture id, layer, matrix3)
-        self.materials: List[Tuple[np.ndarray, Tuple[int, int, np.ndarray]]] = [
-            (np.ones(4, np.float32), (0, 0, np.eye(3, dtype=np.float32)))]
+        # material: ambient rgba, (texture id, layer, matrix3), diffuse rgba
+        # (zero for flat materials: lights add nothing to them)
+        self.materials: List[Tuple[np.ndarray, Tuple[int, int, np.ndarray], np.ndarray]] = [
+            (np.ones(4, np.float32), (0, 0, np.eye(3, dtype=np.float32)),
+             np.zeros(4, np.float32))]
         self.mesh_views: List[Tuple[int, int, int, int, np.ndarray]] = []
         self.names: Dict[str, Tuple[int, int]] = {}
         self._files_added = False
@@ -210,13 +218,15 @@ class OracleRenderer:
             base = np.asarray(mat.base_color, np.float32)
             if mat.flat or not self.max_light_count:
                 ambient = base
+                diffuse = np.zeros(4, np.float32)
             else:
                 ambient = (base * self.ambient_factor).astype(np.float32)
+                diffuse = base
             if mat.texture < 0:
                 tt = (0, 0, np.eye(3, dtype=np.float32))
             else:
                 tt = (mat.texture + tex_off, mat.layer, np.asarray(mat.texture_matrix, np.float32))
-            self.materials.append((ambient, tt))
+            self.materials.append((ambient, tt, diffuse))
         self._files_added = True
         # addFile() re-creates the per-scene camera matrices (identity),
         # Renderer.cpp:630-631; the depth unprojection is kept.
@@ -336,6 +346,31 @@ class OracleRenderer:
                                        unprojection=self.scenes[scene_id].unprojection,
                                        viewport=self.scenes[scene_id].viewport)
 
+    # -- addLight(), lightColors(), lightRanges(), clearLights() ----------
+    def add_light(self, scene_id: int, node: int, light_type: int) -> int:
+        """Renderer::addLight(), Renderer.cpp:990-1015: colour white, range
+        infinite; type 0 directional (-backward of the node), 1 point
+        (translation of the node)."""
+        assert self.max_light_count, "Renderer::addLight(): max light count is zero"
+        sc = self.scenes[scene_id]
+        assert 0 <= node < len(sc.parents)
+        sc.lights.append((int(node), int(light_type)))
+        sc.light_colors = np.concatenate([sc.light_colors, np.ones((1, 3), np.float32)])
+        sc.light_ranges = np.concatenate([sc.light_ranges, np.full(1, np.inf, np.float32)])
+        return len(sc.lights) - 1
+
+    def light_colors(self, scene_id: int) -> np.ndarray:
+        return self.scenes[scene_id].light_colors
+
+    def light_ranges(self, scene_id: int) -> np.ndarray:
+        return self.scenes[scene_id].light_ranges
+
+    def clear_lights(self, scene_id: int) -> None:
+        sc = self.scenes[scene_id]
+        sc.lights = []
+        sc.light_colors = np.zeros((0, 3), np.float32)
+        sc.light_ranges = np.zeros(0, np.float32)
+
     def set_scene_viewport(self, scene_id: int, size) -> None:
         """The scene renders into the bottom-left size = (w, h) pixels of its
         tile (the viewport transform uses w, h); the rest stays clear."""
@@ -447,7 +482,7 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
     tw, th = r.tile_size
     b = _Built()
     # geometry: every mesh, triangles with absolute vertex indices
-    pos, uv, col, tris = [], [], [], []
+    pos, uv, col, tris, nrm = [], [], [], [], []
     vbase, tbase = [], []
     nv = nt = 0
     for m in r.meshes:
@@ -469,6 +504,10 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
         else:
             col.append(np.ones((v, 4), np.float32))
         tris.append(np.asarray(d.indices, np.uint32) + np.uint32(nv))
+        if getattr(d, "normals", None) is not None:
+            nrm.append(np.asarray(d.normals, np.float32))
+        else:
+            nrm.append(np.tile(np.array([0, 0, 1], np.float32), (v, 1)))
         vbase.append(nv)
         tbase.append(nt)
         nv += v
@@ -477,6 +516,7 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
     b.uv = np.ascontiguousarray(np.concatenate(uv) if uv else np.zeros((1, 2), np.float32))
     b.col = np.ascontiguousarray(np.concatenate(col) if col else np.ones((1, 4), np.float32))
     b.tris = np.ascontiguousarray(np.concatenate(tris) if tris else np.zeros(3, np.uint32))
+    b.nrm = np.ascontiguousarray(np.concatenate(nrm) if nrm else np.zeros((1, 3), np.float32))
 
     # textures
     desc, offs, blobs = [], [], []
@@ -496,6 +536,7 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
 
     # per scene nodes and sorted draws
     di, df, npar, nloc, ei, ef = [], [], [], [], [], []
+    diffuse, env_light, light_i, light_f = [], [], [], []
     node_start = draw_start = 0
     pix = 0
     for s in scene_ids:
@@ -503,7 +544,8 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
         sd = r.sorted_draws(s)
         for d in sd:
             mesh = r.meshes[d.mesh]
-            amb, (tex, layer, tm) = r.materials[d.material]
+            amb, (tex, layer, tm), dif = r.materials[d.material]
+            diffuse.append(dif)
             assert d.index_offset % 3 == 0 and d.index_count % 3 == 0
             di.append([tbase[d.mesh] + d.index_offset // 3, d.index_count // 3, d.node,
                        tex, layer, d.semantic, 1 if mesh["vcolor"] else 0, 0])
@@ -513,6 +555,13 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
             df.append(f)
         npar.extend(sc.parents)
         nloc.extend(np.asarray(t, np.float32).reshape(16) for t in sc.transforms)
+        # lights beyond the maximum are not shaded (the reference's shaders
+        # are compiled for maxLightCount lights)
+        nl = min(len(sc.lights), r.max_light_count)
+        env_light.append([len(light_i), nl])
+        for k in range(nl):
+            light_i.append(list(sc.lights[k]))
+            light_f.append(list(sc.light_colors[k]) + [float(sc.light_ranges[k])])
         vw, vh = sc.viewport if sc.viewport is not None else (tw, th)
         ei.append([vw, vh, node_start, len(sc.parents), draw_start, len(sd),
                    NO_TEXTURES if (r.flags & NO_TEXTURES) else 0, pix])
@@ -528,6 +577,10 @@ def build_job(r: OracleRenderer, scene_ids: List[int]) -> _Built:
     b.node_parent = np.ascontiguousarray(np.array(npar, np.int32) if npar else np.zeros(1, np.int32))
     b.node_local = np.ascontiguousarray(np.array(nloc, np.float32).reshape(-1, 16) if nloc else np.zeros((1, 16), np.float32))
     b.env_i = np.ascontiguousarray(np.array(ei, np.int32).reshape(-1, 8))
+    b.draw_diffuse = np.ascontiguousarray(np.array(diffuse, np.float32).reshape(-1, 4) if diffuse else np.zeros((1, 4), np.float32))
+    b.env_light = np.ascontiguousarray(np.array(env_light, np.int32).reshape(-1, 2))
+    b.light_i = np.ascontiguousarray(np.array(light_i, np.int32).reshape(-1, 2) if light_i else np.zeros((1, 2), np.int32))
+    b.light_f = np.ascontiguousarray(np.array(light_f, np.float32).reshape(-1, 4) if light_f else np.zeros((1, 4), np.float32))
     b.env_f = np.ascontiguousarray(np.array(ef, np.float32).reshape(-1, 32))
     b.nenv = len(scene_ids)
     return b
@@ -548,4 +601,7 @@ def run_job(b: _Built, out: Dict[str, np.ndarray], nthreads: int = 1) -> None:
     j.out_count = _p(out.get("count"))
     j.out_draw = _p(out.get("draw"))
     j.out_tri = _p(out.get("tri"))
+    j.nrm = _p(b.nrm)
+    j.draw_diffuse = _p(b.draw_diffuse)
+    j.env_light, j.light_i, j.light_f = _p(b.env_light), _p(b.light_i), _p(b.light_f)
     lib().orc_render(ctypes.byref(j), int(nthreads))

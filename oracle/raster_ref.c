@@ -103,6 +103,13 @@ typedef struct {
   uint32_t* out_count;     /* optional: covered-fragment count (no depth test) */
   int32_t* out_draw;       /* optional: winning draw index (global), -1 = none */
   int32_t* out_tri;        /* optional: winning triangle index within draw */
+  /* lights (Renderer.cpp:990-1015, 1263-1289); may be NULL when no scene
+     has lights */
+  const float* nrm;          /* [nv][3] vertex normals */
+  const float* draw_diffuse; /* [nd][4] diffuse colour; all zero: flat material */
+  const int32_t* env_light;  /* [ne][2] first light, light count (capped at the maximum) */
+  const int32_t* light_i;    /* [nl][2] node (env-local), type (0 directional, 1 point) */
+  const float* light_f;      /* [nl][4] colour rgb, range */
 } orc_job;
 
 /* ------------------------------------------------------------------ */
@@ -396,6 +403,8 @@ typedef struct {
   float s[3], t[3];
   float col[3][4];
   int has_col;
+  /* shaded materials: world-space positions and normals of the vertices */
+  float wp[3][3], wn[3][3];
   /* homogeneous barycentric rows */
   float e[3][3];
 } tri_attr;
@@ -411,6 +420,11 @@ typedef struct {
   int32_t* odraw;
   int32_t* otri;
   int no_textures;
+  /* the env's lights in world space: position (point) or direction towards
+     the light (directional), colour, range, type */
+  int nlights;
+  float lpos[8][3], lcol[8][3], lrange[8];
+  int ltype[8];
 } env_ctx;
 
 static void bary_rows(tri_attr* A) {
@@ -474,6 +488,7 @@ static void shade(const env_ctx* C, const tri_attr* A, int draw, int px,
   float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   bary_at(C, A, fx, fy, b);
   float c[4] = {df[0], df[1], df[2], df[3]};
+  float texc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   int tex = di[3];
   if (!C->no_textures && tex >= 0) {
     const int32_t* td = J->tex_desc + (int64_t)tex * 8;
@@ -505,13 +520,69 @@ static void shade(const env_ctx* C, const tri_attr* A, int draw, int px,
     lambda = lambda * 0.5f;
     float tc[4];
     sample_tex(&tv, di[4], s, t, lambda, tc);
-    for (int k = 0; k < 4; ++k) c[k] = c[k] * tc[k];
+    for (int k = 0; k < 4; ++k) {
+      texc[k] = tc[k];
+      c[k] = c[k] * tc[k];
+    }
   }
+  float vc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   if (A->has_col) {
     for (int k = 0; k < 4; ++k) {
       float v = interp3(b, A->col[0][k], A->col[1][k], A->col[2][k]);
+      vc[k] = v;
       c[k] = c[k] * v;
     }
+  }
+  /* shaded material in a renderer with lights enabled: Phong diffuse term
+     per light (no specular), in world space, attenuation
+     clamp(1 - (d/R)^4, 0, 1)^2 / (1 + d^2) for point lights; the diffuse
+     alpha is added to the ambient alpha also when the scene has no lights
+     (the shader is compiled for the maximum light count) */
+  const float* dd = J->draw_diffuse ? J->draw_diffuse + (int64_t)draw * 4 : NULL;
+  if (dd && (dd[0] != 0.0f || dd[1] != 0.0f || dd[2] != 0.0f || dd[3] != 0.0f)) {
+    float d[4] = {dd[0], dd[1], dd[2], dd[3]};
+    for (int k = 0; k < 4; ++k) d[k] = d[k] * texc[k];
+    for (int k = 0; k < 4; ++k) d[k] = d[k] * vc[k];
+    float n[3], pw[3];
+    for (int k = 0; k < 3; ++k) {
+      n[k] = interp3(b, A->wn[0][k], A->wn[1][k], A->wn[2][k]);
+      pw[k] = interp3(b, A->wp[0][k], A->wp[1][k], A->wp[2][k]);
+    }
+    float nn = n[0] * n[0]; float n1 = n[1] * n[1]; nn = nn + n1; float n2 = n[2] * n[2]; nn = nn + n2;
+    float nl = sqrtf(nn);
+    for (int k = 0; k < 3; ++k) n[k] = n[k] / nl;
+    for (int l = 0; l < C->nlights; ++l) {
+      float L[3], att = 1.0f;
+      if (C->ltype[l] == 1) {
+        for (int k = 0; k < 3; ++k) L[k] = C->lpos[l][k] - pw[k];
+      } else {
+        for (int k = 0; k < 3; ++k) L[k] = C->lpos[l][k];
+      }
+      float ll = L[0] * L[0]; float l1 = L[1] * L[1]; ll = ll + l1; float l2 = L[2] * L[2]; ll = ll + l2;
+      float dist = sqrtf(ll);
+      for (int k = 0; k < 3; ++k) L[k] = L[k] / dist;
+      if (C->ltype[l] == 1) {
+        float q = dist / C->lrange[l];
+        q = q * q;
+        q = q * q;
+        float wdw = 1.0f - q;
+        if (!(wdw > 0.0f)) wdw = 0.0f;
+        if (wdw > 1.0f) wdw = 1.0f;
+        att = wdw * wdw;
+        float den = dist * dist;
+        den = 1.0f + den;
+        att = att / den;
+      }
+      float ndl = n[0] * L[0]; float t1 = n[1] * L[1]; ndl = ndl + t1; float t2 = n[2] * L[2]; ndl = ndl + t2;
+      if (!(ndl > 0.0f)) ndl = 0.0f;
+      float inten = ndl * att;
+      for (int k = 0; k < 3; ++k) {
+        float v = d[k] * C->lcol[l][k];
+        v = v * inten;
+        c[k] = c[k] + v;
+      }
+    }
+    c[3] = c[3] + d[3];
   }
   for (int k = 0; k < 4; ++k) out[k] = to_unorm8(c[k]);
 }
@@ -697,12 +768,52 @@ static void render_env(const orc_job* J, int e) {
                  abs_t + 16 * (i + 1));
   }
 
+  /* lights from their nodes' absolute transforms: point = translation,
+     directional = -backward (the transformation's third column negated) */
+  if (J->env_light) {
+    const int32_t* el = J->env_light + (int64_t)e * 2;
+    C.nlights = el[1] < 8 ? el[1] : 8;
+    for (int l = 0; l < C.nlights; ++l) {
+      const int32_t* li = J->light_i + (int64_t)(el[0] + l) * 2;
+      const float* lf = J->light_f + (int64_t)(el[0] + l) * 4;
+      const float* T = abs_t + 16 * (li[0] + 1);
+      C.ltype[l] = li[1];
+      for (int k = 0; k < 3; ++k) {
+        C.lpos[l][k] = li[1] == 1 ? T[12 + k] : -T[8 + k];
+        C.lcol[l][k] = lf[k];
+      }
+      C.lrange[l] = lf[3];
+    }
+  }
+
   const float* PV = ef;
   for (int d = 0; d < ei[5]; ++d) {
     int dg = ei[4] + d;
     const int32_t* di = J->draw_i + (int64_t)dg * 8;
     const float* df = J->draw_f + (int64_t)dg * 16;
     const float* M = abs_t + 16 * (di[2] + 1);
+    /* normal matrix: inverse transpose of M's upper 3x3 (cofactors over the
+       determinant), [col][row] */
+    float NM[9];
+    {
+      const float a00 = M[0], a01 = M[4], a02 = M[8];   /* row 0 */
+      const float a10 = M[1], a11 = M[5], a12 = M[9];   /* row 1 */
+      const float a20 = M[2], a21 = M[6], a22 = M[10];  /* row 2 */
+      float c00 = a11 * a22; float t0 = a12 * a21; c00 = c00 - t0;
+      float c01 = a12 * a20; float t1 = a10 * a22; c01 = c01 - t1;
+      float c02 = a10 * a21; float t2 = a11 * a20; c02 = c02 - t2;
+      float c10 = a02 * a21; float t3 = a01 * a22; c10 = c10 - t3;
+      float c11 = a00 * a22; float t4 = a02 * a20; c11 = c11 - t4;
+      float c12 = a01 * a20; float t5 = a00 * a21; c12 = c12 - t5;
+      float c20 = a01 * a12; float t6 = a02 * a11; c20 = c20 - t6;
+      float c21 = a02 * a10; float t7 = a00 * a12; c21 = c21 - t7;
+      float c22 = a00 * a11; float t8 = a01 * a10; c22 = c22 - t8;
+      float det = a00 * c00; float t9 = a01 * c01; det = det + t9; float t10 = a02 * c02; det = det + t10;
+      /* inverse transpose = cofactor matrix / det; NM[col*3 + row] */
+      NM[0] = c00 / det; NM[3] = c01 / det; NM[6] = c02 / det;
+      NM[1] = c10 / det; NM[4] = c11 / det; NM[7] = c12 / det;
+      NM[2] = c20 / det; NM[5] = c21 / det; NM[8] = c22 / det;
+    }
     for (int t = 0; t < di[1]; ++t) {
       const uint32_t* tri = J->tris + 3 * ((int64_t)di[0] + t);
       tri_attr A;
@@ -715,6 +826,16 @@ static void render_env(const orc_job* J, int e) {
         mat4_vec(M, p4, q);
         mat4_vec(PV, q, c);
         A.cx[k] = c[0]; A.cy[k] = c[1]; cz[k] = c[2]; A.cw[k] = c[3];
+        A.wp[k][0] = q[0]; A.wp[k][1] = q[1]; A.wp[k][2] = q[2];
+        if (J->nrm) {
+          const float* nv = J->nrm + 3 * (int64_t)v;
+          for (int r = 0; r < 3; ++r) {
+            float x = NM[r] * nv[0]; float y = NM[3 + r] * nv[1]; x = x + y;
+            float z = NM[6 + r] * nv[2]; A.wn[k][r] = x + z;
+          }
+        } else {
+          A.wn[k][0] = 0.0f; A.wn[k][1] = 0.0f; A.wn[k][2] = 1.0f;
+        }
         float u = J->uv[2 * v], vv = J->uv[2 * v + 1];
         /* texture matrix applied in the vertex stage: ((c0*u + c1*v) + c2) */
         float s = df[4] * u; float s1 = df[7] * vv; s = s + s1; s = s + df[10];

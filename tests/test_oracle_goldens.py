@@ -7,8 +7,8 @@ import scenarios as S
 from oracle.oracle import OracleRenderer, unproject_depth
 
 
-def make(tile_size, tile_count, flags=0):
-    return OracleRenderer(tile_size, tile_count, flags)
+def make(tile_size, tile_count, flags=0, max_lights=0):
+    return OracleRenderer(tile_size, tile_count, flags, max_light_count=max_lights)
 
 
 @pytest.mark.parametrize("case", list(S.FILE_DATA))
@@ -60,6 +60,19 @@ def test_clear_scene(case):
 
 def test_lights_disabled():
     checks, images = S.lights_disabled(make)
+    S.assert_checks(checks)
+    S.assert_images(images)
+
+
+@pytest.mark.parametrize("case", list(S.LIGHT_DATA))
+def test_lights(case):
+    checks, images = S.lights(make, *S.LIGHT_DATA[case], (0.75, 0.005))
+    S.assert_checks(checks)
+    S.assert_images(images)
+
+
+def test_clear_lights():
+    checks, images = S.clear_lights(make)
     S.assert_checks(checks)
     S.assert_images(images)
 
