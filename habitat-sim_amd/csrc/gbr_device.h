@@ -208,9 +208,22 @@ struct alignas(16) MaterialDesc {
   float ambient[4];
   int32_t texture;      // -1 = untextured
   uint32_t layer;
-  uint32_t pad[2];
+  uint32_t lit;         // diffuse term of the scene's lights applies (diffuse != 0)
+  uint32_t pad;
   float texmat[9];      // column-major
   float pad2[3];
+  float diffuse[4];     // base colour of a shaded material with lights enabled, else 0
+};
+
+// A light of a scene: the node whose absolute transform places it (global
+// node index), its type (GBR_LIGHT_*), colour and range.
+struct alignas(16) LightDesc {
+  uint32_t node;
+  int32_t type;
+  float range;
+  uint32_t pad;
+  float color[3];
+  float pad2;
 };
 
 struct alignas(16) TextureDesc {
@@ -240,6 +253,11 @@ struct FrameParams {
   const float* cameras;        // [scene][16] projection*view
   const float* unprojections;  // [scene][2]
   const float* abs_nodes;      // [node][16]
+  // lights (null when the renderer has max_light_count 0)
+  const float* normals;        // [vertex][3] mesh-local vertex normals
+  const float* normal_mats;    // [node][12] inverse transpose of the upper 3x3, [col][row]
+  const LightDesc* lights;
+  const uint32_t* scene_lights;  // [scene][2] first light, light count
   // binning
   uint32_t* bin_count;
   uint32_t* bin_entries;

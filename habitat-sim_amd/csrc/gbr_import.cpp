@@ -950,6 +950,10 @@ bool load_gltf(const std::string& path, File& out, std::string& err) {
           if (comps != 2) return (err = "TEXCOORD_0 is not VEC2"), false;
           for (size_t i = 0; i < nv; ++i) mesh.texcoords[2 * i + 1] = 1.0f - mesh.texcoords[2 * i + 1];
         }
+        if (at->has("NORMAL")) {
+          if (!g.accessor_f(inum(at->get("NORMAL"), -1), mesh.normals, comps, false)) return (err = g.err), false;
+          if (comps != 3) return (err = "NORMAL is not VEC3"), false;
+        }
         if (at->has("COLOR_0")) {
           std::vector<float> c;
           if (!g.accessor_f(inum(at->get("COLOR_0"), -1), c, comps, true)) return (err = g.err), false;
@@ -1031,6 +1035,7 @@ gbr_file_desc File::desc() {
     d.index_count = (uint32_t)m.indices.size();
     d.indices = m.indices.data();
     d.source_index_size = m.index_size;
+    d.normals = m.normals.empty() ? nullptr : m.normals.data();
   }
   texel_blobs.assign(textures.size(), {});
   texture_desc.assign(textures.size(), gbr_texture_desc{});

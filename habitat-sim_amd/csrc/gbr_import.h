@@ -22,6 +22,7 @@ struct Mesh {
   std::vector<float> positions;   // [V][3]
   std::vector<float> texcoords;   // [V][2] or empty
   std::vector<float> colors;      // [V][4] or empty
+  std::vector<float> normals;     // [V][3] or empty
   std::vector<uint32_t> indices;  // triangle list
   uint32_t index_size = 4;        // bytes per index in the file
 };

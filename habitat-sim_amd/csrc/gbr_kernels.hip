@@ -270,7 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
                                                   float* __restrict__ abs_out, uint32_t n,
                                                   uint32_t* __restrict__ counters,
                                                   unsigned long long* __restrict__ stats,
-                                                  uint32_t* __restrict__ frame) {
+                                                  uint32_t* __restrict__ frame,
+                                                  float* __restrict__ nrm_out) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   // first kernel of every frame: reset the frame's counters, advance the
   // frame number (kept on the device, so a captured frame replays as is)
@@ -293,6 +294,28 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) abs_out[(size_t)i * 16 + k] = acc[k];
+  if (nrm_out) {
+    // normal matrix: inverse transpose of the upper 3x3 (cofactors over the
+    // determinant), [col][row]
+    const float a00 = acc[0], a01 = acc[4], a02 = acc[8];
+    const float a10 = acc[1], a11 = acc[5], a12 = acc[9];
+    const float a20 = acc[2], a21 = acc[6], a22 = acc[10];
+    float c00 = a11 * a22; float t0 = a12 * a21; c00 = c00 - t0;
+    float c01 = a12 * a20; float t1 = a10 * a22; c01 = c01 - t1;
+    float c02 = a10 * a21; float t2 = a11 * a20; c02 = c02 - t2;
+    float c10 = a02 * a21; float t3 = a01 * a22; c10 = c10 - t3;
+    float c11 = a00 * a22; float t4 = a02 * a20; c11 = c11 - t4;
+    float c12 = a01 * a20; float t5 = a00 * a21; c12 = c12 - t5;
+    float c20 = a01 * a12; float t6 = a02 * a11; c20 = c20 - t6;
+    float c21 = a02 * a10; float t7 = a00 * a12; c21 = c21 - t7;
+    float c22 = a00 * a11; float t8 = a01 * a10; c22 = c22 - t8;
+    float det = a00 * c00; float t9 = a01 * c01; det = det + t9; float t10 = a02 * c02;
+    det = det + t10;
+    float* nm = nrm_out + (size_t)i * 12;
+    nm[0] = c00 / det; nm[3] = c01 / det; nm[6] = c02 / det;
+    nm[1] = c10 / det; nm[4] = c11 / det; nm[7] = c12 / det;
+    nm[2] = c20 / det; nm[5] = c21 / det; nm[8] = c22 / det;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -1237,12 +1260,102 @@ struct QuadST {
   bool h00, h10, h01;
 };
 
+// Phong diffuse term of the scene's lights (no specular) for a shaded
+// material, in world space: per light, intensity = max(0, N.L) * attenuation
+// with attenuation clamp(1 - (d/R)^4, 0, 1)^2 / (1 + d^2) for point lights
+// (1 for directional ones), colour += diffuse * light colour * intensity; the
+// diffuse alpha is added to the ambient alpha also with no lights.  diffuse =
+// material diffuse * texture colour * vertex colour.  Vertex positions and
+// normals come from the mesh through the draw's absolute transform and normal
+// matrix, interpolated with the pixel's barycentrics b.
+__device__ void add_lights(const FrameParams& p, const MaterialDesc& mat, const DrawDesc& d,
+                           uint32_t prim, uint32_t scene, const float* b, const float* texc,
+                           const float* vc, float* c) {
+  float dd[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dd[k] = mat.diffuse[k] * texc[k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dd[k] = dd[k] * vc[k];
+  const uint32_t* ip = p.indices + d.index_base + 3 * (prim - d.prim_base);
+  const float* M = p.abs_nodes + (size_t)d.node * 16;
+  const float* NM = p.normal_mats + (size_t)d.node * 12;
+  // world position and normal interpolated vertex by vertex: the sums
+  // b0*a0 + b1*a1 + b2*a2 of interp3, in its order
+  float n[3], pw[3];
+#pragma unroll 1
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t v = ip[k];
+    const float bk = k == 0 ? b[0] : (k == 1 ? b[1] : b[2]);
+    float q[4];
+    mat4_vec(M, p.pos_x[v], p.pos_y[v], p.pos_z[v], 1.0f, q);
+    const float n0 = p.normals[3 * (size_t)v], n1 = p.normals[3 * (size_t)v + 1],
+                n2 = p.normals[3 * (size_t)v + 2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      float x = NM[r] * n0; float y = NM[3 + r] * n1; x = x + y;
+      float z = NM[6 + r] * n2; x = x + z;
+      const float tn = bk * x, tp = bk * q[r];
+      n[r] = k == 0 ? tn : n[r] + tn;
+      pw[r] = k == 0 ? tp : pw[r] + tp;
+    }
+  }
+  float nn = n[0] * n[0]; float n1 = n[1] * n[1]; nn = nn + n1; float n2 = n[2] * n[2];
+  nn = nn + n2;
+  const float nl = sqrtf(nn);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) n[k] = n[k] / nl;
+  const uint32_t first = p.scene_lights[2 * scene], count = p.scene_lights[2 * scene + 1];
+  for (uint32_t l = first; l < first + count; ++l) {
+    const LightDesc ld = p.lights[l];
+    const float* T = p.abs_nodes + (size_t)ld.node * 16;
+    float L[3], att = 1.0f;
+    if (ld.type == GBR_LIGHT_POINT) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) L[k] = T[12 + k] - pw[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) L[k] = -T[8 + k];
+    }
+    float ll = L[0] * L[0]; float l1 = L[1] * L[1]; ll = ll + l1; float l2 = L[2] * L[2];
+    ll = ll + l2;
+    const float dist = sqrtf(ll);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L[k] = L[k] / dist;
+    if (ld.type == GBR_LIGHT_POINT) {
+      float qd = dist / ld.range;
+      qd = qd * qd;
+      qd = qd * qd;
+      float wdw = 1.0f - qd;
+      if (!(wdw > 0.0f)) wdw = 0.0f;
+      if (wdw > 1.0f) wdw = 1.0f;
+      att = wdw * wdw;
+      float den = dist * dist;
+      den = 1.0f + den;
+      att = att / den;
+    }
+    float ndl = n[0] * L[0]; float t1 = n[1] * L[1]; ndl = ndl + t1; float t2 = n[2] * L[2];
+    ndl = ndl + t2;
+    if (!(ndl > 0.0f)) ndl = 0.0f;
+    const float inten = ndl * att;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float v = dd[k] * ld.color[k];
+      v = v * inten;
+      c[k] = c[k] + v;
+    }
+  }
+  c[3] = c[3] + dd[3];
+}
+
+template <bool kLights>
 __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw, float hh,
                                const TriAttr& A, const float* b, float s, float t,
                                const MaterialDesc& mat, bool textured, const QuadST& q,
-                               const DrawDesc& d, uint32_t prim, int px, int py) {
+                               const DrawDesc& d, uint32_t prim, uint32_t scene, int px, int py) {
   const float ihw = 1.0f / hw, ihh = 1.0f / hh;
   float c[4] = {mat.ambient[0], mat.ambient[1], mat.ambient[2], mat.ambient[3]};
+  float texc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+  float vc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   if (textured) {
     const TextureDesc& td = p.textures[mat.texture];
     const int qx = px & ~1, qy = py & ~1;
@@ -1276,7 +1389,10 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
     float tc[4];
     sample_tex(p, lut, td, (int)mat.layer, s, t, lambda, tc);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) c[k] = c[k] * tc[k];
+    for (int k = 0; k < 4; ++k) {
+      texc[k] = tc[k];
+      c[k] = c[k] * tc[k];
+    }
   }
   const uint32_t cf = (d.flags >> kDrawColorShift) & 3u;
   if (cf != kColorNone) {
@@ -1299,9 +1415,11 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const float v = interp3(b, col[0][k], col[1][k], col[2][k]);
+      vc[k] = v;
       c[k] = c[k] * v;
     }
   }
+  if (kLights && mat.lit) add_lights(p, mat, d, prim, scene, b, texc, vc, c);
   return to_unorm8(c[0]) | (to_unorm8(c[1]) << 8) | (to_unorm8(c[2]) << 16) |
          (to_unorm8(c[3]) << 24);
 }
@@ -1894,6 +2012,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
 // linear depth, semantic id; every output written once
 // --------------------------------------------------------------------------
 
+// kLights: the renderer has lights enabled (a separate instance, so that
+// renderers without lights keep the smaller register footprint)
+template <bool kLights>
 __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
                                                     const RasterItem* __restrict__ items) {
   __shared__ float s_lut[256];
@@ -1979,7 +2100,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     const float z = __uint_as_float((uint32_t)(key >> 32));
     if (want_color)
-      p.out_color[o] = shade_pixel(p, s_lut, hw, hh, A, bc, s, t, mat, textured, q, d, prim, px, py);
+      p.out_color[o] = shade_pixel<kLights>(p, s_lut, hw, hh, A, bc, s, t, mat, textured, q, d, prim, it.scene,
+                                    px, py);
     if (want_depth) p.out_depth[o] = z;
     if (want_lin) {
       const float lin = un1 / (z + un0);
@@ -2382,10 +2504,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g) {
 
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
                         uint32_t* counters, unsigned long long* stats, uint32_t* frame,
-                        hipStream_t s) {
+                        float* nrm_out, hipStream_t s) {
   const uint32_t blocks = n == 0 ? 1 : (n + kBlock - 1) / kBlock;  // always runs: resets counters
   hipLaunchKernelGGL(k_nodes, dim3(blocks), dim3(kBlock), 0, s, parents, local, abs_out, n,
-                     counters, stats, frame);
+                     counters, stats, frame, nrm_out);
   return hipGetLastError();
 }
 
@@ -2430,7 +2552,10 @@ hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t
 hipError_t launch_resolve(const FrameParams& p, const RasterItem* items, uint32_t n,
                           hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_resolve, dim3(n), dim3(kBlock), 0, s, p, items);
+  if (p.lights)
+    hipLaunchKernelGGL(k_resolve<true>, dim3(n), dim3(kBlock), 0, s, p, items);
+  else
+    hipLaunchKernelGGL(k_resolve<false>, dim3(n), dim3(kBlock), 0, s, p, items);
   return hipGetLastError();
 }
 

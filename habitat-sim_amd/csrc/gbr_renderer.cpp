@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -33,7 +34,7 @@
 namespace gbr {
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
                         uint32_t* counters, unsigned long long* stats, uint32_t* frame,
-                        hipStream_t s);
+                        float* nrm_out, hipStream_t s);
 hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s);
 int setup_blocks_per_cu(uint32_t lds_bins);
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
@@ -169,6 +170,7 @@ struct gbr_renderer {
   std::vector<uint32_t> uv16;
   std::vector<uint32_t> col8;
   std::vector<float4> col32;
+  std::vector<float> nrm;  // [vertex][3], kept only with lights enabled
   std::vector<uint32_t> indices;
   std::vector<uint8_t> texels;
   std::vector<TextureDesc> textures;
@@ -206,6 +208,9 @@ struct gbr_renderer {
   DevBuf<ItemCull> d_cull;
   DevBuf<uint32_t> d_list1, d_cand, d_list2, d_item_vis;
   DevBuf<float> d_hiz;
+  DevBuf<float> d_nrm, d_nmat;       // vertex normals, per-node normal matrices
+  DevBuf<LightDesc> d_lights;
+  DevBuf<uint32_t> d_scene_lights;   // [scene][2] first light, count
   DevBuf<uint32_t> d_frame;  // device frame number (k_nodes advances it)
 
   // the last frame's kernel sequence as a HIP graph, replayed while its
@@ -236,6 +241,8 @@ struct gbr_renderer {
   // pinned per-frame staging for cameras
   float* h_cam = nullptr;
   size_t h_cam_cap = 0;
+  uint8_t* h_light = nullptr;  // pinned per-frame staging for lights
+  size_t h_light_cap = 0;
   hipEvent_t cam_event = nullptr;
   // optional per-stage timing of gbr_draw (tuning.timing): one event set per
   // frame since the last gbr_frame_timings(), up to kTimingFrames frames
@@ -360,6 +367,7 @@ gbr_status upload_assets(gbr_renderer* r) {
   TRY(upload(r->d_uv16, r->uv16, s));
   TRY(upload(r->d_col8, r->col8, s));
   TRY(upload(r->d_col32, r->col32, s));
+  if (r->cfg.max_light_count) TRY(upload(r->d_nrm, r->nrm, s));
   TRY(upload(r->d_indices, r->indices, s));
   TRY(upload(r->d_texels, r->texels, s));
   TRY(upload(r->d_textures, r->textures, s));
@@ -501,6 +509,7 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   TRY(upload(r->d_parents, parents, s));
   TRY(upload(r->d_local, local, s));
   TRY(r->d_abs.reserve(std::max<size_t>(local.size(), 16)));
+  if (r->cfg.max_light_count) TRY(r->d_nmat.reserve(std::max<size_t>(parents.size(), 1) * 12));
   TRY(upload(r->d_draws, draws, s));
   TRY(upload(r->d_scenes, r->scene_desc, s));
   TRY(upload(r->d_items, items, s));
@@ -577,6 +586,47 @@ gbr_status upload_cameras(gbr_renderer* r) {
   HIP_TRY(hipMemcpyAsync(r->d_unproj.ptr, r->h_cam + n * 16, n * 8, hipMemcpyHostToDevice, r->stream));
   HIP_TRY(hipMemcpyAsync(r->d_caminfo.ptr, r->h_cam + n * 18, n * 32, hipMemcpyHostToDevice,
                          r->stream));
+  if (r->cfg.max_light_count) {
+    // per scene its first light and light count (at most max_light_count,
+    // the count the reference's shaders are compiled for), then the lights
+    // with their nodes as global indices
+    size_t nl = 0;
+    for (size_t i = 0; i < n; ++i)
+      nl += std::min<size_t>(r->scenes[i].lights.size(), r->cfg.max_light_count);
+    const size_t head = (n * 2 * sizeof(uint32_t) + 15) & ~size_t(15);
+    const size_t bytes = head + std::max<size_t>(nl, 1) * sizeof(LightDesc);
+    if (r->h_light_cap < bytes) {
+      if (r->h_light) (void)hipHostFree(r->h_light);
+      r->h_light = nullptr;
+      HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&r->h_light), bytes));
+      r->h_light_cap = bytes;
+    }
+    uint32_t* ranges = reinterpret_cast<uint32_t*>(r->h_light);
+    LightDesc* lights = reinterpret_cast<LightDesc*>(r->h_light + head);
+    uint32_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const Scene& sc = r->scenes[i];
+      const uint32_t cnt = (uint32_t)std::min<size_t>(sc.lights.size(), r->cfg.max_light_count);
+      ranges[2 * i] = k;
+      ranges[2 * i + 1] = cnt;
+      for (uint32_t l = 0; l < cnt; ++l, ++k) {
+        const Light& L = sc.lights[l];
+        LightDesc& d = lights[k];
+        d = LightDesc{};
+        d.node = r->scene_node_base[i] + (uint32_t)L.node;
+        d.type = L.type;
+        d.range = L.range;
+        std::memcpy(d.color, L.color, sizeof d.color);
+      }
+    }
+    TRY(r->d_scene_lights.reserve(n * 2 + 2));
+    TRY(r->d_lights.reserve(std::max<size_t>(nl, 1)));
+    HIP_TRY(hipMemcpyAsync(r->d_scene_lights.ptr, ranges, n * 2 * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, r->stream));
+    if (nl)
+      HIP_TRY(hipMemcpyAsync(r->d_lights.ptr, lights, nl * sizeof(LightDesc),
+                             hipMemcpyHostToDevice, r->stream));
+  }
   HIP_TRY(hipEventRecord(r->cam_event, r->stream));
   r->cam_event_pending = true;
   return GBR_OK;
@@ -663,6 +713,12 @@ gbr_status do_draw(gbr_renderer* r) {
   p.cameras = r->d_cameras.ptr;
   p.unprojections = r->d_unproj.ptr;
   p.abs_nodes = r->d_abs.ptr;
+  if (r->cfg.max_light_count) {
+    p.normals = r->d_nrm.ptr;
+    p.normal_mats = r->d_nmat.ptr;
+    p.lights = r->d_lights.ptr;
+    p.scene_lights = r->d_scene_lights.ptr;
+  }
   p.bin_count = r->d_bin_count.ptr;
   p.bin_entries = r->d_bin_entries.ptr;
   p.rrecs = r->d_rrecs.ptr;
@@ -767,7 +823,7 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
   {
     StageTimer t(r, GBR_TIMING_NODES);
     HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, p.counters,
-                         p.stats, p.frame, r->stream));
+                         p.stats, p.frame, const_cast<float*>(p.normal_mats), r->stream));
   }
   const uint32_t setup_grid = (uint32_t)(r->cu_count * r->setup_bpc);
   const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
@@ -968,8 +1024,9 @@ gbr_status gbr_create(const gbr_config* config, gbr_renderer** out) {
     return fail(GBR_ERR_ARGUMENT, "gbr_create: tile size and count must be positive");
   if (config->tile_size[0] > 8192 || config->tile_size[1] > 8192)
     return fail(GBR_ERR_ARGUMENT, "gbr_create: tile size above 8192 pixels (guard band)");
-  if (config->max_light_count != 0)
-    return fail(GBR_ERR_UNSUPPORTED, "gbr_create: lights (max_light_count > 0) are not supported");
+  if (config->max_light_count > GBR_MAX_LIGHTS)
+    return fail(GBR_ERR_UNSUPPORTED, "gbr_create: max_light_count above " +
+                                         std::to_string(GBR_MAX_LIGHTS));
   if (config->planes & ~GBR_PLANE_ALL) return fail(GBR_ERR_ARGUMENT, "gbr_create: unknown plane bits");
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
@@ -1048,7 +1105,13 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   r->d_fb_color.release();
   r->d_fb_depth.release();
   r->d_stage.release();
+  r->d_hiz.release();
+  r->d_nrm.release();
+  r->d_nmat.release();
+  r->d_lights.release();
+  r->d_scene_lights.release();
   if (r->h_cam) (void)hipHostFree(r->h_cam);
+  if (r->h_light) (void)hipHostFree(r->h_light);
   if (r->cam_event) (void)hipEventDestroy(r->cam_event);
   for (auto& set : r->t_pool)
     for (hipEvent_t e : set)
@@ -1159,6 +1222,10 @@ gbr_status gbr_add_file_data(gbr_renderer* r, const gbr_file_desc* f, uint32_t f
       r->pos_x.push_back(m.positions[3 * v]);
       r->pos_y.push_back(m.positions[3 * v + 1]);
       r->pos_z.push_back(m.positions[3 * v + 2]);
+      if (r->cfg.max_light_count) {  // meshes without normals shade with +Z
+        for (int k = 0; k < 3; ++k)
+          r->nrm.push_back(m.normals ? m.normals[3 * v + k] : (k == 2 ? 1.0f : 0.0f));
+      }
       float u = m.texcoords ? m.texcoords[2 * v] : 0.0f;
       float vv = m.texcoords ? m.texcoords[2 * v + 1] : 0.0f;
       r->uv32.push_back(make_float2(u, vv));
@@ -1188,8 +1255,12 @@ gbr_status gbr_add_file_data(gbr_renderer* r, const gbr_file_desc* f, uint32_t f
     MaterialDesc m{};
     for (int c = 0; c < 4; ++c) {
       float b = md.base_color[c];
-      m.ambient[c] = (md.flat || !r->cfg.max_light_count) ? b : b * r->cfg.ambient_factor;
+      const bool shaded = !md.flat && r->cfg.max_light_count;
+      m.ambient[c] = shaded ? b * r->cfg.ambient_factor : b;
+      m.diffuse[c] = shaded ? b : 0.0f;
     }
+    m.lit = (m.diffuse[0] != 0.0f || m.diffuse[1] != 0.0f || m.diffuse[2] != 0.0f ||
+             m.diffuse[3] != 0.0f) ? 1u : 0u;
     if (md.texture < 0) {
       m.texture = 0;
       m.layer = 0;
@@ -1377,10 +1448,45 @@ gbr_status gbr_add_light(gbr_renderer* r, uint32_t scene, uint64_t node, int32_t
   if (!r) return fail(GBR_ERR_ARGUMENT, "addLight(): null renderer");
   if (!r->cfg.max_light_count) return fail(GBR_ERR_STATE, "Renderer::addLight(): max light count is zero");
   TRY(check_scene(r, scene, "addLight()"));
-  (void)node;
-  (void)type;
-  (void)out;
-  return fail(GBR_ERR_UNSUPPORTED, "addLight(): lights are not supported");
+  Scene& sc = r->scenes[scene];
+  if (node >= sc.parents.size())
+    return fail(GBR_ERR_RANGE, "Renderer::addLight(): index " + std::to_string(node) +
+                                   " out of range for " + std::to_string(sc.parents.size()) +
+                                   " nodes in scene " + std::to_string(scene));
+  if (type != GBR_LIGHT_DIRECTIONAL && type != GBR_LIGHT_POINT)
+    return fail(GBR_ERR_ARGUMENT, "Renderer::addLight(): unknown light type " + std::to_string(type));
+  Light L{};
+  L.node = node;
+  L.type = type;
+  L.color[0] = L.color[1] = L.color[2] = 1.0f;  // 0xffffff
+  L.range = std::numeric_limits<float>::infinity();
+  sc.lights.push_back(L);
+  if (out) *out = sc.lights.size() - 1;
+  return GBR_OK;
+}
+
+// lightColors() / lightRanges(), Renderer.cpp:1101-1121: strided views into
+// the scene's light records
+gbr_status gbr_light_colors(gbr_renderer* r, uint32_t scene, float** out_data, uint64_t* out_count,
+                            uint64_t* out_stride) {
+  TRY(check_scene(r, scene, "Renderer::lightColors()"));
+  if (!out_data || !out_count || !out_stride) return fail(GBR_ERR_ARGUMENT, "lightColors(): null output");
+  Scene& sc = r->scenes[scene];
+  *out_data = sc.lights.empty() ? nullptr : sc.lights.data()->color;
+  *out_count = sc.lights.size();
+  *out_stride = sizeof(Light);
+  return GBR_OK;
+}
+
+gbr_status gbr_light_ranges(gbr_renderer* r, uint32_t scene, float** out_data, uint64_t* out_count,
+                            uint64_t* out_stride) {
+  TRY(check_scene(r, scene, "Renderer::lightRanges()"));
+  if (!out_data || !out_count || !out_stride) return fail(GBR_ERR_ARGUMENT, "lightRanges(): null output");
+  Scene& sc = r->scenes[scene];
+  *out_data = sc.lights.empty() ? nullptr : &sc.lights.data()->range;
+  *out_count = sc.lights.size();
+  *out_stride = sizeof(Light);
+  return GBR_OK;
 }
 
 // Renderer::clear(), Renderer.cpp:1017-1040 (camera kept)

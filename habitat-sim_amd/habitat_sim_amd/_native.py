@@ -24,6 +24,7 @@ PLANE_COLOR, PLANE_DEPTH, PLANE_LINEAR_DEPTH, PLANE_SEMANTIC, PLANE_ALL = 0x1, 0
 FORMAT_RGBA8, FORMAT_RGB8 = 0, 1
 MEMORY_HOST, MEMORY_DEVICE = 0, 1
 CULL_NONE, CULL_FRUSTUM, CULL_HIZ, CULL_LEARNED = 0, 1, 2, 3
+LIGHT_DIRECTIONAL, LIGHT_POINT, MAX_LIGHTS = 0, 1, 64
 TIMING_STAGES = ("nodes", "cull", "setup1", "clip1", "raster1", "hiz", "setup2", "clip2",
                  "raster2", "resolve")
 
@@ -97,7 +98,7 @@ class MeshDesc(ctypes.Structure):
                 ("texcoords", ctypes.c_void_p), ("colors", ctypes.c_void_p),
                 ("colors_u8", ctypes.c_uint32), ("texcoords_half", ctypes.c_uint32),
                 ("index_count", ctypes.c_uint32), ("indices", ctypes.c_void_p),
-                ("source_index_size", ctypes.c_uint32)]
+                ("source_index_size", ctypes.c_uint32), ("normals", ctypes.c_void_p)]
 
 
 class TextureDesc(ctypes.Structure):
@@ -165,6 +166,10 @@ SIGNATURES = {
     "gbr_scene_viewport": (S, [R, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]),
     "gbr_set_scene_planes": (S, [R, ctypes.c_uint32, ctypes.c_uint32]),
     "gbr_clear_lights": (S, [R, ctypes.c_uint32]),
+    "gbr_light_colors": (S, [R, ctypes.c_uint32, ctypes.POINTER(c_float_p),
+                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
+    "gbr_light_ranges": (S, [R, ctypes.c_uint32, ctypes.POINTER(c_float_p),
+                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "gbr_camera": (S, [R, ctypes.c_uint32, ctypes.c_void_p]),
     "gbr_camera_depth_unprojection": (S, [R, ctypes.c_uint32, ctypes.c_void_p]),
     "gbr_update_camera": (S, [R, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),

@@ -221,6 +221,8 @@ class RendererStandalone:
             d.index_count = len(idx)
             d.indices = ptr(idx)
             d.source_index_size = m.index_size
+            if getattr(m, "normals", None) is not None:
+                d.normals = ptr(np.ascontiguousarray(m.normals, np.float32))
         textures = (N.TextureDesc * max(len(fd.textures), 1))()
         for i, t in enumerate(fd.textures):
             d = textures[i]
@@ -324,6 +326,29 @@ class RendererStandalone:
 
     def clear_lights(self, scene: int) -> None:
         N.call("gbr_clear_lights", self._h, scene)
+
+    def _light_view(self, fn: str, scene: int, cols: int) -> np.ndarray:
+        data = N.c_float_p()
+        count, stride = ctypes.c_uint64(), ctypes.c_uint64()
+        N.call(fn, self._h, scene, ctypes.byref(data), ctypes.byref(count), ctypes.byref(stride))
+        n, st = count.value, stride.value
+        shape = (n, cols) if cols > 1 else (n,)
+        if n == 0:
+            return np.zeros(shape, np.float32)
+        buf = (ctypes.c_char * ((n - 1) * st + 4 * cols)).from_address(
+            ctypes.cast(data, ctypes.c_void_p).value)
+        strides = (st, 4) if cols > 1 else (st,)
+        return np.ndarray(shape, np.float32, buffer=buf, strides=strides)
+
+    def light_colors(self, scene: int) -> np.ndarray:
+        """Renderer::lightColors(): writable (lights, 3) view of the scene's
+        light colours, valid until the next add_light(), clear_lights() or
+        clear() of the scene."""
+        return self._light_view("gbr_light_colors", scene, 3)
+
+    def light_ranges(self, scene: int) -> np.ndarray:
+        """Renderer::lightRanges(): writable (lights,) view of the ranges."""
+        return self._light_view("gbr_light_ranges", scene, 1)
 
     def camera(self, scene: int) -> np.ndarray:
         out = np.empty(16, np.float32)

@@ -56,6 +56,9 @@ enum {
 /* RendererLightType, Renderer.h:170-173 */
 #define GBR_LIGHT_DIRECTIONAL 0
 #define GBR_LIGHT_POINT 1
+/* largest max_light_count (the reference compiles its shaders for any count,
+   Renderer.cpp:835-860) */
+#define GBR_MAX_LIGHTS 64
 
 /* Output planes produced by gbr_draw() (new; the reference has colour and
    window depth only, RendererStandalone.cpp:123-124) */
@@ -88,7 +91,7 @@ typedef struct gbr_config {
   int32_t tile_size[2];   /* default {128, 128} */
   int32_t tile_count[2];  /* default {1, 1}; scene count = product */
   uint32_t flags;         /* GBR_FLAG_* */
-  uint32_t max_light_count; /* only 0 (flat shading) is supported */
+  uint32_t max_light_count; /* 0: every material flat; at most GBR_MAX_LIGHTS */
   float ambient_factor;   /* default 0.1 */
   int32_t device;         /* HIP device ordinal; -1 = current device */
   uint32_t standalone_flags; /* GBR_STANDALONE_* */
@@ -170,6 +173,7 @@ typedef struct gbr_mesh_desc {
   uint32_t index_count;     /* triangle-list indices */
   const uint32_t* indices;
   uint32_t source_index_size; /* bytes per index in the source file */
+  const float* normals;     /* [vertex_count*3] or NULL (+Z); used with lights */
 } gbr_mesh_desc;
 
 typedef struct gbr_texture_desc {
@@ -266,9 +270,22 @@ gbr_status gbr_set_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, uint
                                            uint32_t semantic_id);
 /* addEmptyNode() Renderer.h:598 */
 gbr_status gbr_add_empty_node(gbr_renderer* r, uint32_t scene, uint64_t* out_node);
-/* addLight() Renderer.h:619-621: needs max_light_count > 0 (not supported) */
+/* addLight() Renderer.h:619-621, Renderer.cpp:990-1015: a light of
+   GBR_LIGHT_* type placed by the scene node's absolute transform (point:
+   its translation, directional: towards -backward), colour white, range
+   infinite.  GBR_ERR_STATE when max_light_count is 0, GBR_ERR_RANGE for a
+   node out of range.  A scene's lights beyond max_light_count don't shade. */
 gbr_status gbr_add_light(gbr_renderer* r, uint32_t scene, uint64_t node, int32_t type,
                          uint64_t* out_light);
+/* lightColors() Renderer.h:697-698, lightRanges() Renderer.h:713-714:
+   mutable host views of the scene's lights (3 floats of colour, resp. one
+   float of range, per light, `*out_stride` bytes apart), valid until the
+   next addLight(), clearLights() or clear() of the scene; read at every
+   gbr_draw() */
+gbr_status gbr_light_colors(gbr_renderer* r, uint32_t scene, float** out_data, uint64_t* out_count,
+                            uint64_t* out_stride);
+gbr_status gbr_light_ranges(gbr_renderer* r, uint32_t scene, float** out_data, uint64_t* out_count,
+                            uint64_t* out_stride);
 /* clear() Renderer.h:631, clearLights() Renderer.h:640 */
 gbr_status gbr_clear(gbr_renderer* r, uint32_t scene);
 gbr_status gbr_clear_lights(gbr_renderer* r, uint32_t scene);

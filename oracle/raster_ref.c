@@ -397,6 +397,9 @@ static int clip_poly(cvert* poly, int n, float gx, float gy) {
 /* Per-env rendering                                                   */
 /* ------------------------------------------------------------------ */
 
+/* largest light count per env (the renderer's maximum) */
+#define ORC_MAX_LIGHTS 64
+
 typedef struct {
   /* unclipped triangle in clip space + per-vertex attributes */
   float cx[3], cy[3], cw[3];
@@ -423,8 +426,8 @@ typedef struct {
   /* the env's lights in world space: position (point) or direction towards
      the light (directional), colour, range, type */
   int nlights;
-  float lpos[8][3], lcol[8][3], lrange[8];
-  int ltype[8];
+  float lpos[ORC_MAX_LIGHTS][3], lcol[ORC_MAX_LIGHTS][3], lrange[ORC_MAX_LIGHTS];
+  int ltype[ORC_MAX_LIGHTS];
 } env_ctx;
 
 static void bary_rows(tri_attr* A) {
@@ -772,7 +775,7 @@ static void render_env(const orc_job* J, int e) {
      directional = -backward (the transformation's third column negated) */
   if (J->env_light) {
     const int32_t* el = J->env_light + (int64_t)e * 2;
-    C.nlights = el[1] < 8 ? el[1] : 8;
+    C.nlights = el[1] < ORC_MAX_LIGHTS ? el[1] : ORC_MAX_LIGHTS;
     for (int l = 0; l < C.nlights; ++l) {
       const int32_t* li = J->light_i + (int64_t)(el[0] + l) * 2;
       const float* lf = J->light_f + (int64_t)(el[0] + l) * 4;

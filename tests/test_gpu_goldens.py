@@ -13,21 +13,22 @@ from oracle.oracle import OracleRenderer
 pytestmark = pytest.mark.gpu
 
 
-def make(tile_size, tile_count, flags=0):
+def make(tile_size, tile_count, flags=0, max_lights=0):
     from habitat_sim_amd import RendererConfiguration, RendererStandalone
-    cfg = RendererConfiguration().set_tile_size_count(tile_size, tile_count).set_flags(flags)
+    cfg = (RendererConfiguration().set_tile_size_count(tile_size, tile_count).set_flags(flags)
+           .set_max_light_count(max_lights))
     return RendererStandalone(cfg)
 
 
-def make_oracle(tile_size, tile_count, flags=0):
-    return OracleRenderer(tile_size, tile_count, flags)
+def make_oracle(tile_size, tile_count, flags=0, max_lights=0):
+    return OracleRenderer(tile_size, tile_count, flags, max_light_count=max_lights)
 
 
 def _same_as_oracle(fn, *args):
     checks, images = fn(make, *args)
     _, oimages = fn(make_oracle, *args)
-    for (a, name, _), (b, _, _) in zip(images, oimages):
-        np.testing.assert_array_equal(a, b, err_msg=f"{name}: GPU vs oracle")
+    for a, b in zip(images, oimages):
+        np.testing.assert_array_equal(a[0], b[0], err_msg=f"{a[1]}: GPU vs oracle")
     return checks, images
 
 
@@ -79,6 +80,19 @@ def test_clear_scene(case):
 
 def test_lights_disabled():
     checks, images = _same_as_oracle(S.lights_disabled)
+    S.assert_checks(checks)
+    S.assert_images(images)
+
+
+@pytest.mark.parametrize("case", list(S.LIGHT_DATA))
+def test_lights(case):
+    checks, images = _same_as_oracle(S.lights, *S.LIGHT_DATA[case], (0.75, 0.005))
+    S.assert_checks(checks)
+    S.assert_images(images)
+
+
+def test_clear_lights():
+    checks, images = _same_as_oracle(S.clear_lights)
     S.assert_checks(checks)
     S.assert_images(images)
 

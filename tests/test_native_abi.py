@@ -99,5 +99,5 @@ def test_invalid_config_rejected_before_device_use():
     h = ctypes.c_void_p()
     assert lib.gbr_create(ctypes.byref(c), ctypes.byref(h)) == N.ERR_ARGUMENT
     lib.gbr_config_init(ctypes.byref(c))
-    c.max_light_count = 2
+    c.max_light_count = N.MAX_LIGHTS + 1
     assert lib.gbr_create(ctypes.byref(c), ctypes.byref(h)) == N.ERR_UNSUPPORTED
