@@ -124,19 +124,6 @@ def pinhole_projection(resolution_hw, hfov_rad: float = math.pi / 2, near: float
     return perspective_projection_size(near_plane / scale, near, far)
 
 
-def look_at(eye, target, up) -> np.ndarray:
-    """Magnum Matrix4::lookAt(): camera transform looking from eye to target."""
-    eye = np.asarray(eye, np.float64)
-    back = eye - np.asarray(target, np.float64)
-    back /= np.linalg.norm(back)
-    right = np.cross(np.asarray(up, np.float64), back)
-    right /= np.linalg.norm(right)
-    realup = np.cross(back, right)
-    m = identity()
-    m[0, :3], m[1, :3], m[2, :3], m[3, :3] = right, realup, back, eye
-    return m
-
-
 def camera_pose(position, yaw: float, pitch: float) -> np.ndarray:
     """Camera-to-world transform: yaw about +Y, then pitch about +X, camera
     looking down -Z (habitat convention, Y up)."""

@@ -71,7 +71,12 @@ class ReplayRendererConfiguration:
 
 
 class BatchReplayRenderer:
-    def __init__(self, cfg: ReplayRendererConfiguration):
+    def __init__(self, cfg: ReplayRendererConfiguration,
+                 batch_renderer_configuration: Optional[RendererConfiguration] = None):
+        """BatchReplayRenderer(cfg, gfx_batch::RendererConfiguration&&)
+        (BatchReplayRenderer.cpp:20-56): the batch configuration (flags, max
+        light count, ambient factor) gets the tile size and count from the
+        sensors and the environment count."""
         if not cfg.sensor_specifications:
             raise ValueError("at least one sensor specification is needed")
         res = {tuple(int(v) for v in s.resolution) for s in cfg.sensor_specifications}
@@ -94,7 +99,8 @@ class BatchReplayRenderer:
         planes = 0
         for k in kinds:
             planes |= _PLANES[k]
-        rc = RendererConfiguration().set_tile_size_count((w, h), self.grid)
+        rc = (batch_renderer_configuration or RendererConfiguration()).set_tile_size_count(
+            (w, h), self.grid)
         sc = RendererStandaloneConfiguration()
         if cfg.gpu_device_id >= 0:
             sc.set_cuda_device(cfg.gpu_device_id)

@@ -18,7 +18,10 @@ semantics of the reference's ``gfx::replay::Player``
   instances with their latest transformations (the batch renderer cannot
   remove single hierarchies), so node ids stay dense;
 * going back to an earlier keyframe clears the scene and replays from the
-  first one.
+  first one;
+* ``lightsChanged`` replaces the scene's lights (``changeLightSetup``); a
+  deletion's clear removes them until the next light change, as in the
+  reference.
 
 Difference from the reference: instance metadata ``semanticId`` is forwarded
 to the renderer's semantic plane (the reference's batch player drops it).
@@ -27,6 +30,7 @@ from __future__ import annotations
 
 import json
 import math
+import warnings
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple, Union
 
@@ -36,6 +40,7 @@ from . import math3d as m3
 
 FILE_WHOLE = 1
 FILE_GENERATE_MIPMAP = 2
+LIGHT_DIRECTIONAL, LIGHT_POINT = 0, 1  # RendererLightType
 
 _PRIMITIVE_PREFIXES = ("capsule3DSolid", "capsule3DWireframe", "coneSolid", "coneWireframe",
                        "cubeSolid", "cubeWireframe", "cylinderSolid", "cylinderWireframe",
@@ -251,6 +256,8 @@ class BatchPlayer:
         for key, (t, r) in k.state_updates:
             if key in self._created:
                 self._set_transform(self._created[key], transform_matrix(t, r))
+        if k.lights_changed:
+            self._change_light_setup(k.lights)
 
     def _deletions(self, k: Keyframe) -> None:
         """Player::hackProcessDeletions, batch branch (Player.cpp:291-325)."""
@@ -286,6 +293,35 @@ class BatchPlayer:
         bake = m3.mul(s, rot)
         sem = metadata.get("semanticId", -1)
         return r.add_node_hierarchy(self.scene, path, bake, max(int(sem), 0))
+
+    def _change_light_setup(self, lights: List[dict]) -> None:
+        """BatchPlayerImplementation::changeLightSetup (:127-170): the
+        scene's lights are replaced; each light gets a new empty node, a
+        point light (vector w != 0) translated to vector.xyz, a directional
+        one looking along vector.xyz (lookAt with +Y up; only its Z axis
+        matters, NaNs in the others are harmless).  Range stays infinite."""
+        r = self.renderer
+        if not r.max_light_count:
+            warnings.warn(f"Attempted to change {len(lights)} lights for scene {self.scene} "
+                          "but the renderer is configured without lights")
+            return
+        r.clear_lights(self.scene)
+        for light in lights:
+            model = str(light.get("model", "global")).lower()
+            if model != "global":
+                raise RuntimeError(f"light position model {model!r} is not global")
+            v = [float(x) for x in light.get("vector", (0.0, 0.0, 1.0, 0.0))]
+            node = r.add_empty_node(self.scene)
+            if v[3]:
+                xf, kind = m3.translation(tuple(v[:3])), LIGHT_POINT
+            else:
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    xf = m3.look_at((0.0, 0.0, 0.0), tuple(v[:3]), (0.0, 1.0, 0.0))
+                kind = LIGHT_DIRECTIONAL
+            r.transformations(self.scene)[node] = xf
+            lid = r.add_light(self.scene, node, kind)
+            r.light_colors(self.scene)[lid] = np.asarray(light.get("color", (1.0, 1.0, 1.0)),
+                                                         np.float32)[:3]
 
     def _set_metadata(self, node: int, md: dict) -> None:
         sem = int(md.get("semanticId", -1))

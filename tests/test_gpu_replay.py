@@ -2,6 +2,8 @@
 depth_images) fills caller arrays with GL rows like the reference's
 AbstractReplayRenderer::render, get_observations() gives habitat's top-down
 observation dict, and both equal the oracle's planes."""
+import math
+
 import numpy as np
 import pytest
 
@@ -78,4 +80,46 @@ def test_keyframe_player_on_the_gpu_matches_oracle():
         o.draw()
         np.testing.assert_array_equal(r.color_image(), o.color_image(), err_msg=f"keyframe {idx}")
         np.testing.assert_array_equal(r.depth_image(), o.depth_image(), err_msg=f"keyframe {idx}")
+    r.close()
+
+
+def test_keyframe_lights_on_the_gpu_match_oracle():
+    """Keyframes that change the light setup (changeLightSetup) over two
+    shaded spheres of batch.gltf: every keyframe's pixels identical to the
+    oracle's, and each light change visible."""
+    import json
+    import os
+    from habitat_sim_amd import RendererConfiguration, RendererStandalone
+    from habitat_sim_amd.replay_player import BatchPlayer
+    asset = os.path.join(os.path.dirname(__file__), "golden", "scenes", "batch.gltf")
+    l1 = [{"vector": [-1.0, 1.5, 1.0, 1.0], "color": [0.6, 0.6, 3.0], "model": "global"},
+          {"vector": [3.0, 1.0, 3.0, 0.0], "color": [0.8, 1.0, 1.0], "model": "global"}]
+    l2 = [{"vector": [0.0, 0.3, 1.0, 0.0], "color": [1.0, 0.5, 0.25], "model": "global"}]
+    frames = [json.dumps({"lightsChanged": True, "lights": ls}) for ls in (l1, l2, [])]
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count((96, 96), (1, 1))
+                           .set_max_light_count(2))
+    o = OracleRenderer((96, 96), (1, 1), max_light_count=2)
+    P = m3.perspective_projection(math.radians(60.0), 1.0, 0.1, 20.0)
+    V = m3.inverted(m3.translation((0.0, 0.0, 5.0)))
+    players = []
+    for x in (r, o):
+        assert x.add_file(asset)
+        a = x.add_node_hierarchy(0, "shaded yellow sphere")
+        b = x.add_node_hierarchy(0, "shaded checkerboard sphere")
+        x.transformations(0)[a] = m3.translation((-1.1, 0.0, 0.0))
+        x.transformations(0)[b] = m3.translation((1.1, 0.0, 0.0))
+        x.update_camera(0, P, V)
+        p = BatchPlayer(x, 0)
+        for k in frames:
+            p.append_keyframe(k)
+        players.append(p)
+    images = []
+    for idx in range(len(frames)):
+        for p in players:
+            p.set_keyframe_index(idx)
+        r.draw()
+        o.draw()
+        np.testing.assert_array_equal(r.color_image(), o.color_image(), err_msg=f"keyframe {idx}")
+        images.append(r.color_image())
+    assert not np.array_equal(images[0], images[1]) and not np.array_equal(images[1], images[2])
     r.close()

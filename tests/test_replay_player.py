@@ -103,3 +103,48 @@ def test_state_update_rotation_and_user_transforms():
     R = quaternion_matrix(q).astype(np.float64)
     np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-6)
     assert abs(np.linalg.det(R) - 1) < 1e-6
+
+
+def _light_keyframes():
+    load = {"type": 0, "filepath": ASSET,
+            "frame": {"up": [0, 1, 0], "front": [0, 0, -1], "origin": [0, 0, 0]}}
+    lights = [{"vector": [1.0, 2.0, 3.0, 1.0], "color": [0.5, 0.25, 1.0], "model": "global"},
+              {"vector": [0.0, -1.0, -1.0, 0.0], "color": [2.0, 2.0, 2.0], "model": "global"}]
+    k0 = {"loads": [load], "creations": [_creation(0)], "lightsChanged": True, "lights": lights}
+    k1 = {"stateUpdates": [_update(0, [0, 0, 1])]}
+    k2 = {"lightsChanged": True, "lights": lights[1:]}
+    return [json.dumps(k) for k in (k0, k1, k2)]
+
+
+def test_light_setup_changes():
+    """BatchPlayerImplementation::changeLightSetup (:127-170): each light on
+    a new empty node, point = translation, directional = lookAt along the
+    vector with +Y up; colours from the keyframe, ranges infinite."""
+    r = OracleRenderer((16, 16), (1, 1), max_light_count=4)
+    p = BatchPlayer(r, 0)
+    for k in _light_keyframes():
+        p.append_keyframe(k)
+    p.set_keyframe_index(0)
+    t = r.transformations(0)
+    assert len(t) == 5 + 2 and len(r.light_colors(0)) == 2
+    np.testing.assert_array_equal(t[5], m3.translation((1.0, 2.0, 3.0)))
+    back = np.asarray(t[6])[2, :3]  # backward = -vector, normalised
+    np.testing.assert_allclose(back, -np.array([0.0, -1.0, -1.0]) / np.sqrt(2.0), rtol=1e-6)
+    assert r.scenes[0].lights == [(5, 1), (6, 0)]
+    np.testing.assert_array_equal(r.light_colors(0), np.array([[0.5, 0.25, 1.0], [2, 2, 2]], np.float32))
+    assert np.isinf(r.light_ranges(0)).all()
+    p.set_keyframe_index(1)  # no light change: kept
+    assert len(r.light_colors(0)) == 2
+    p.set_keyframe_index(2)  # replaced by one light on a third new node
+    assert len(r.light_colors(0)) == 1 and len(r.transformations(0)) == 8
+    assert r.scenes[0].lights == [(7, 0)]
+
+
+def test_light_setup_without_lights_warns():
+    r = OracleRenderer((16, 16), (1, 1))
+    p = BatchPlayer(r, 0)
+    for k in _light_keyframes()[:1]:
+        p.append_keyframe(k)
+    with pytest.warns(UserWarning, match="configured without lights"):
+        p.set_keyframe_index(0)
+    assert len(r.transformations(0)) == 5
