@@ -71,6 +71,14 @@ __device__ __forceinline__ float4 vertex_clip(const float* M, const float* PV, f
   return make_float4(c[0], c[1], c[2], c[3]);
 }
 
+// l / n for 0 <= l < 4096 and 1 <= n <= 64 from a float reciprocal
+// inv = 1/n: (l + 0.5) / n lies at least 0.5 / n >= 2^-7 away from an
+// integer and the float product errs by less than 2^-10 there, so the
+// truncation is exact (an integer division costs ~25 VALU operations).
+__device__ __forceinline__ int div_small(int l, float inv) {
+  return (int)(((float)l + 0.5f) * inv);
+}
+
 struct Snap {
   int32_t X, Y;
   float z;
@@ -1349,10 +1357,9 @@ __device__ void add_lights(const FrameParams& p, const MaterialDesc& mat, const 
 
 template <bool kLights>
 __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw, float hh,
-                               const TriAttr& A, const float* b, float s, float t,
-                               const MaterialDesc& mat, bool textured, const QuadST& q,
+                               float ihw, float ihh, const TriAttr& A, const float* b, float s,
+                               float t, const MaterialDesc& mat, bool textured, const QuadST& q,
                                const DrawDesc& d, uint32_t prim, uint32_t scene, int px, int py) {
-  const float ihw = 1.0f / hw, ihh = 1.0f / hh;
   float c[4] = {mat.ambient[0], mat.ambient[1], mat.ambient[2], mat.ambient[3]};
   float texc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   float vc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
@@ -1643,8 +1650,9 @@ __device__ __forceinline__ uint32_t load_raster_rec(const FrameParams& p, unsign
 __device__ __forceinline__ void store_vis(const FrameParams& p, const BinInfo& b,
                                           const unsigned long long* s_vis, int B) {
   const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
+  const float inv_bw = 1.0f / (float)bw;
   for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
-    const int lx = l % bw, ly = l / bw;
+    const int ly = div_small(l, inv_bw), lx = l - ly * bw;
     p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.stride + b.bin.x0 + lx] =
         s_vis[ly * B + lx];
   }
@@ -1785,10 +1793,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
   if (phase == 2) {
     constexpr int kPer = kMaxBin * kMaxBin / kBlock;
     unsigned long long keys[kPer];
+    const float inv_bw = 1.0f / (float)bw;
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int l = threadIdx.x + q * kBlock;
-      const int ly = l / bw, lx = l - ly * bw;
+      const int ly = div_small(l, inv_bw), lx = l - ly * bw;
       keys[q] = l < bw * bh
                     ? p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.stride + b.bin.x0 + lx]
                     : 0ull;
@@ -1796,7 +1805,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int l = threadIdx.x + q * kBlock;
-      const int ly = l / bw, lx = l - ly * bw;
+      const int ly = div_small(l, inv_bw), lx = l - ly * bw;
       if (l < bw * bh) s_vis[ly * B + lx] = keys[q];
     }
   } else {
@@ -1853,8 +1862,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
   const int B = p.bin_size;
   if (phase == 2) {
     const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
+    const float inv_bw = 1.0f / (float)bw;
     for (int l = threadIdx.x; l < bw * bh; l += kBlock) {
-      const int lx = l % bw, ly = l / bw;
+      const int ly = div_small(l, inv_bw), lx = l - ly * bw;
       s_vis[ly * B + lx] =
           p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.stride + b.bin.x0 + lx];
     }
@@ -2034,6 +2044,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
   const int lane = threadIdx.x & 63;
   const float hw = (float)b.width * 0.5f, hh = (float)b.height * 0.5f;
   const float ihw = 1.0f / hw, ihh = 1.0f / hh;
+  const float inv_bw = 1.0f / (float)bw;
   // the scene's sensor set
   const bool want_color = p.out_color && (b.planes & GBR_PLANE_COLOR);
   const bool want_depth = p.out_depth && (b.planes & GBR_PLANE_DEPTH);
@@ -2042,7 +2053,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
   for (int base = 0; base < npx; base += kBlock) {
     const int l = base + (int)threadIdx.x;
     const bool valid = l < npx;
-    const int px = b.bin.x0 + l % bw, py = b.bin.y0 + l / bw;
+    const int ly = div_small(l, inv_bw);
+    const int px = b.bin.x0 + (l - ly * bw), py = b.bin.y0 + ly;
     const unsigned long long key =
         valid ? p.vis[(size_t)b.plane_offset + (size_t)py * b.stride + px] : kClearKey;
     const bool drawn = key != kClearKey;
@@ -2100,7 +2112,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     const float z = __uint_as_float((uint32_t)(key >> 32));
     if (want_color)
-      p.out_color[o] = shade_pixel<kLights>(p, s_lut, hw, hh, A, bc, s, t, mat, textured, q, d, prim, it.scene,
+      p.out_color[o] = shade_pixel<kLights>(p, s_lut, hw, hh, ihw, ihh, A, bc, s, t, mat, textured, q, d, prim, it.scene,
                                     px, py);
     if (want_depth) p.out_depth[o] = z;
     if (want_lin) {
