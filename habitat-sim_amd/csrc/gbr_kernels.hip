@@ -1087,27 +1087,40 @@ __device__ __forceinline__ int floor_to_int(float u) {
   return (int)floorf(u);
 }
 
-__device__ __forceinline__ void fetch(const FrameParams& p, const float* lut, const TextureDesc& t,
-                                      int level, int layer, int i, int j, float* out) {
-  int w = max(t.width >> level, 1), h = max(t.height >> level, 1);
-  const uint32_t texel = *reinterpret_cast<const uint32_t*>(
-      p.texels + t.level_offset[level] + (((long long)layer * h + j) * w + i) * 4);
+// A texture's header held in registers (one 32-byte load), and the byte
+// offset of the level being sampled: every field is read once per pixel
+// instead of once per use.
+struct TexHeader {
+  int32_t width, height, layers, levels;
+  int32_t min_filter, mag_filter, wrap_s, wrap_t;
+};
+
+__device__ __forceinline__ TexHeader load_tex_header(const TextureDesc* t) {
+  const int4 a = *reinterpret_cast<const int4*>(t);
+  const int4 b = *(reinterpret_cast<const int4*>(t) + 1);
+  return TexHeader{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
+// texel (i, j) of `layer` at a level of size w x h starting at byte `base`
+__device__ __forceinline__ void fetch(const float* lut, const uint8_t* base, int w, int h,
+                                      int layer, int i, int j, float* out) {
+  const uint32_t idx = ((uint32_t)layer * (uint32_t)h + (uint32_t)j) * (uint32_t)w + (uint32_t)i;
+  const uint32_t texel = *reinterpret_cast<const uint32_t*>(base + (size_t)idx * 4);
   out[0] = lut[texel & 0xff];
   out[1] = lut[(texel >> 8) & 0xff];
   out[2] = lut[(texel >> 16) & 0xff];
   out[3] = lut[texel >> 24];
 }
 
-__device__ void sample_level(const FrameParams& p, const float* lut, const TextureDesc& t, int level,
-                             int layer,
-                             float s, float tc, bool linear, float* out) {
+__device__ void sample_level(const float* lut, const TexHeader& t, const uint8_t* base, int level,
+                             int layer, float s, float tc, bool linear, float* out) {
   int w = max(t.width >> level, 1), h = max(t.height >> level, 1);
   float u = s * (float)w;
   float v = tc * (float)h;
   if (!linear) {
     int i = wrap_coord(floor_to_int(u), w, t.wrap_s);
     int j = wrap_coord(floor_to_int(v), h, t.wrap_t);
-    fetch(p, lut, t, level, layer, i, j, out);
+    fetch(lut, base, w, h, layer, i, j, out);
     return;
   }
   float uu = u - 0.5f;
@@ -1121,10 +1134,10 @@ __device__ void sample_level(const FrameParams& p, const float* lut, const Textu
   i0 = wrap_coord(i0, w, t.wrap_s);
   j0 = wrap_coord(j0, h, t.wrap_t);
   float t00[4], t10[4], t01[4], t11[4];
-  fetch(p, lut, t, level, layer, i0, j0, t00);
-  fetch(p, lut, t, level, layer, i1, j0, t10);
-  fetch(p, lut, t, level, layer, i0, j1, t01);
-  fetch(p, lut, t, level, layer, i1, j1, t11);
+  fetch(lut, base, w, h, layer, i0, j0, t00);
+  fetch(lut, base, w, h, layer, i1, j0, t10);
+  fetch(lut, base, w, h, layer, i0, j1, t01);
+  fetch(lut, base, w, h, layer, i1, j1, t11);
   float ia = 1.0f - a;
   float ib = 1.0f - b;
   float w00 = ia * ib, w10 = a * ib, w01 = ia * b, w11 = a * b;
@@ -1141,9 +1154,14 @@ __device__ void sample_level(const FrameParams& p, const float* lut, const Textu
   }
 }
 
-__device__ void sample_tex(const FrameParams& p, const float* lut, const TextureDesc& t, int layer,
-                           float s,
-                           float tc, float lambda, float* out) {
+__device__ __forceinline__ const uint8_t* level_base(const FrameParams& p, const TextureDesc* td,
+                                                     int level) {
+  return p.texels + td->level_offset[level];
+}
+
+__device__ void sample_tex(const FrameParams& p, const float* lut, const TextureDesc* td,
+                           const TexHeader& t, int layer, float s, float tc, float lambda,
+                           float* out) {
   if (layer < 0) layer = 0;
   if (layer >= t.layers) layer = t.layers - 1;
   float c = 0.0f;
@@ -1151,14 +1169,14 @@ __device__ void sample_tex(const FrameParams& p, const float* lut, const Texture
       (t.min_filter == GL_NEAREST_MIPMAP_NEAREST || t.min_filter == GL_NEAREST_MIPMAP_LINEAR))
     c = 0.5f;
   if (lambda <= c) {
-    sample_level(p, lut, t, 0, layer, s, tc, t.mag_filter == GL_LINEAR, out);
+    sample_level(lut, t, level_base(p, td, 0), 0, layer, s, tc, t.mag_filter == GL_LINEAR, out);
     return;
   }
   const int q = t.levels - 1;
   switch (t.min_filter) {
     case GL_NEAREST:
     case GL_LINEAR:
-      sample_level(p, lut, t, 0, layer, s, tc, t.min_filter == GL_LINEAR, out);
+      sample_level(lut, t, level_base(p, td, 0), 0, layer, s, tc, t.min_filter == GL_LINEAR, out);
       return;
     case GL_NEAREST_MIPMAP_NEAREST:
     case GL_LINEAR_MIPMAP_NEAREST: {
@@ -1169,21 +1187,24 @@ __device__ void sample_tex(const FrameParams& p, const float* lut, const Texture
         float f = ceilf(lambda + 0.5f) - 1.0f;
         dlev = f > (float)q ? q : (int)f;
       }
-      sample_level(p, lut, t, dlev, layer, s, tc, t.min_filter == GL_LINEAR_MIPMAP_NEAREST, out);
+      sample_level(lut, t, level_base(p, td, dlev), dlev, layer, s, tc,
+                   t.min_filter == GL_LINEAR_MIPMAP_NEAREST, out);
       return;
     }
     default: {
       bool lin = t.min_filter == GL_LINEAR_MIPMAP_LINEAR;
       if (lambda >= (float)q) {
-        sample_level(p, lut, t, q, layer, s, tc, lin, out);
+        sample_level(lut, t, level_base(p, td, q), q, layer, s, tc, lin, out);
         return;
       }
       float fl = floorf(lambda);
       int d1 = (int)fl;
       float f = lambda - fl;
+      const uint8_t* b1 = level_base(p, td, d1);
+      const uint8_t* b2 = level_base(p, td, d1 + 1);
       float t1[4], t2[4];
-      sample_level(p, lut, t, d1, layer, s, tc, lin, t1);
-      sample_level(p, lut, t, d1 + 1, layer, s, tc, lin, t2);
+      sample_level(lut, t, b1, d1, layer, s, tc, lin, t1);
+      sample_level(lut, t, b2, d1 + 1, layer, s, tc, lin, t2);
       float g = 1.0f - f;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -1364,7 +1385,8 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
   float texc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   float vc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   if (textured) {
-    const TextureDesc& td = p.textures[mat.texture];
+    const TextureDesc* tdp = p.textures + mat.texture;
+    const TexHeader td = load_tex_header(tdp);
     const int qx = px & ~1, qy = py & ~1;
     float s00 = q.s00, t00 = q.t00, s10 = q.s10, t10 = q.t10, s01 = q.s01, t01 = q.t01;
     float bq[3];
@@ -1394,7 +1416,7 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
     float lambda = gbr_log2(r2);
     lambda = lambda * 0.5f;
     float tc[4];
-    sample_tex(p, lut, td, (int)mat.layer, s, t, lambda, tc);
+    sample_tex(p, lut, tdp, td, (int)mat.layer, s, t, lambda, tc);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       texc[k] = tc[k];

@@ -1177,6 +1177,10 @@ gbr_status gbr_add_file_data(gbr_renderer* r, const gbr_file_desc* f, uint32_t f
       const gbr_texture_desc& t = f->textures[i];
       if (!t.texels || t.width < 1 || t.height < 1 || t.layers < 1 || t.levels < 1 || t.levels > 16)
         return fail(GBR_ERR_IO, "addFile(): can't import texture " + std::to_string(i) + " of " + fname);
+      // the kernels index a level's texels with 32 bits
+      if ((uint64_t)t.width * t.height * t.layers >= (1ull << 32))
+        return fail(GBR_ERR_SIZE, "addFile(): texture " + std::to_string(i) + " of " + fname +
+                                      " has 2^32 or more texels per level");
       TextureDesc td{};
       td.width = (int)t.width;
       td.height = (int)t.height;
