@@ -43,6 +43,13 @@ struct alignas(16) DrawDesc {
   uint32_t flags;       // color format | uv format
   uint32_t prim_base;   // scene-local primitive order of the first triangle
   uint32_t scene;
+  // the material's shading fields, copied so that the resolve pass reaches
+  // them with one dependent load less
+  float ambient[4];
+  int32_t texture;      // 0 = untextured (white)
+  uint32_t layer;
+  uint32_t lit;         // MaterialDesc::lit
+  uint32_t pad;
 };
 
 struct alignas(16) SceneDesc {
@@ -273,6 +280,7 @@ struct FrameParams {
   uint32_t n_items;             // work items of the frame
   // culling
   const WorkItem* items;
+  const DrawDesc* item_draws;   // per work item, its draw's descriptor
   const ItemBounds* bounds;     // per work item
   ItemCull* cull;               // per work item, this frame
   uint32_t* list1;              // work items of phase 1

@@ -1297,9 +1297,9 @@ struct QuadST {
 // material diffuse * texture colour * vertex colour.  Vertex positions and
 // normals come from the mesh through the draw's absolute transform and normal
 // matrix, interpolated with the pixel's barycentrics b.
-__device__ void add_lights(const FrameParams& p, const MaterialDesc& mat, const DrawDesc& d,
-                           uint32_t prim, uint32_t scene, const float* b, const float* texc,
-                           const float* vc, float* c) {
+__device__ void add_lights(const FrameParams& p, const DrawDesc& d, uint32_t prim, uint32_t scene,
+                           const float* b, const float* texc, const float* vc, float* c) {
+  const MaterialDesc& mat = p.materials[d.material];
   float dd[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) dd[k] = mat.diffuse[k] * texc[k];
@@ -1379,13 +1379,13 @@ __device__ void add_lights(const FrameParams& p, const MaterialDesc& mat, const 
 template <bool kLights>
 __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw, float hh,
                                float ihw, float ihh, const TriAttr& A, const float* b, float s,
-                               float t, const MaterialDesc& mat, bool textured, const QuadST& q,
-                               const DrawDesc& d, uint32_t prim, uint32_t scene, int px, int py) {
-  float c[4] = {mat.ambient[0], mat.ambient[1], mat.ambient[2], mat.ambient[3]};
+                               float t, bool textured, const QuadST& q, const DrawDesc& d,
+                               uint32_t prim, uint32_t scene, int px, int py) {
+  float c[4] = {d.ambient[0], d.ambient[1], d.ambient[2], d.ambient[3]};
   float texc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   float vc[4] = {1.0f, 1.0f, 1.0f, 1.0f};
   if (textured) {
-    const TextureDesc* tdp = p.textures + mat.texture;
+    const TextureDesc* tdp = p.textures + d.texture;
     const TexHeader td = load_tex_header(tdp);
     const int qx = px & ~1, qy = py & ~1;
     float s00 = q.s00, t00 = q.t00, s10 = q.s10, t10 = q.t10, s01 = q.s01, t01 = q.t01;
@@ -1416,7 +1416,7 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
     float lambda = gbr_log2(r2);
     lambda = lambda * 0.5f;
     float tc[4];
-    sample_tex(p, lut, tdp, td, (int)mat.layer, s, t, lambda, tc);
+    sample_tex(p, lut, tdp, td, (int)d.layer, s, t, lambda, tc);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       texc[k] = tc[k];
@@ -1448,7 +1448,7 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
       c[k] = c[k] * v;
     }
   }
-  if (kLights && mat.lit) add_lights(p, mat, d, prim, scene, b, texc, vc, c);
+  if (kLights && d.lit) add_lights(p, d, prim, scene, b, texc, vc, c);
   return to_unorm8(c[0]) | (to_unorm8(c[1]) << 8) | (to_unorm8(c[2]) << 16) |
          (to_unorm8(c[3]) << 24);
 }
@@ -2083,7 +2083,6 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     const uint32_t prim = (uint32_t)key;
     ResolveRec rs;
     DrawDesc d;
-    MaterialDesc mat;
     TriAttr A;
     float bc[3] = {1.0f, 0.0f, 0.0f};
     float s = 0.0f, t = 0.0f;
@@ -2093,8 +2092,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
       // every drawn primitive's record was written this frame; the clamp
       // only keeps the lookups in bounds
       const uint32_t item = min(rs.item, p.n_items - 1u);
-      d = p.draws[p.items[item].draw];
-      mat = p.materials[d.material];
+      d = p.item_draws[item];
       if (p.item_vis) p.item_vis[item] = frame;  // visibility feedback for the next frame
       A.cx[0] = rs.cx0; A.cy[0] = rs.cy0; A.cw[0] = rs.cw0;
       A.cx[1] = rs.cx1; A.cy[1] = rs.cy1; A.cw[1] = rs.cw1;
@@ -2104,7 +2102,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
       A.s[2] = rs.s2; A.t[2] = rs.t2;
       bary_rows(A);
       bary_at(A, hw, hh, ihw, ihh, (float)px + 0.5f, (float)py + 0.5f, bc);
-      textured = !p.no_textures && mat.texture > 0;
+      textured = !p.no_textures && d.texture > 0;
       if (textured) {
         s = interp3(bc, A.s[0], A.s[1], A.s[2]);
         t = interp3(bc, A.t[0], A.t[1], A.t[2]);
@@ -2134,7 +2132,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     const float z = __uint_as_float((uint32_t)(key >> 32));
     if (want_color)
-      p.out_color[o] = shade_pixel<kLights>(p, s_lut, hw, hh, ihw, ihh, A, bc, s, t, mat, textured, q, d, prim, it.scene,
+      p.out_color[o] = shade_pixel<kLights>(p, s_lut, hw, hh, ihw, ihh, A, bc, s, t, textured, q, d, prim, it.scene,
                                     px, py);
     if (want_depth) p.out_depth[o] = z;
     if (want_lin) {

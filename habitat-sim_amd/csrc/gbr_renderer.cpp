@@ -204,6 +204,7 @@ struct gbr_renderer {
   DevBuf<DrawDesc> d_draws;
   DevBuf<SceneDesc> d_scenes;
   DevBuf<WorkItem> d_items;
+  DevBuf<DrawDesc> d_item_draws;  // each work item's draw, indexed by item
   DevBuf<ItemBounds> d_bounds;
   DevBuf<ItemCull> d_cull;
   DevBuf<uint32_t> d_list1, d_cand, d_list2, d_item_vis;
@@ -448,6 +449,13 @@ gbr_status rebuild_structure(gbr_renderer* r) {
       dd.flags = (m.color_fmt << kDrawColorShift) | (m.uv_half ? kDrawUvHalf : 0u);
       dd.prim_base = prim;
       dd.scene = si;
+      {
+        const MaterialDesc& md = r->materials[dr.material];
+        for (int c = 0; c < 4; ++c) dd.ambient[c] = md.ambient[c];
+        dd.texture = md.texture;
+        dd.layer = md.layer;
+        dd.lit = md.lit;
+      }
       const uint32_t gdraw = (uint32_t)draws.size();
       draws.push_back(dd);
       for (const ChunkInfo& ci : view_chunks(r, v, chunk)) {
@@ -513,6 +521,11 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   TRY(upload(r->d_draws, draws, s));
   TRY(upload(r->d_scenes, r->scene_desc, s));
   TRY(upload(r->d_items, items, s));
+  // (kept alive until the stream synchronisation below, like the other
+  // host arrays of this upload)
+  std::vector<DrawDesc> item_draws(items.size());
+  for (size_t i = 0; i < items.size(); ++i) item_draws[i] = draws[items[i].draw];
+  TRY(upload(r->d_item_draws, item_draws, s));
   TRY(upload(r->d_bounds, bounds, s));
   {
     const size_t n = std::max<size_t>(items.size(), 1);
@@ -732,6 +745,7 @@ gbr_status do_draw(gbr_renderer* r) {
   p.queue_grain = (uint32_t)r->tun.queue_grain;
   p.n_items = r->n_items;
   p.items = r->d_items.ptr;
+  p.item_draws = r->d_item_draws.ptr;
   p.bounds = r->d_bounds.ptr;
   p.cull = r->d_cull.ptr;
   p.list1 = r->d_list1.ptr;
@@ -1097,6 +1111,7 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   r->d_vis.release();
   r->d_scenes.release();
   r->d_items.release();
+  r->d_item_draws.release();
   r->d_ritems.release();
   r->d_parents.release();
   r->d_stats.release();
