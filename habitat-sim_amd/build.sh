@@ -14,8 +14,10 @@ FLAGS=(--offload-arch=gfx950 -O3 -fPIC -std=c++17 -ffp-contract=off
 "$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_renderer.cpp" -o "$OUT/gbr_renderer.o"
 "$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_vis.cpp" -o "$OUT/gbr_vis.o"
 "$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_import.cpp" -o "$OUT/gbr_import.o"
+"$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/gbr_noise.hip" -o "$OUT/gbr_noise.o"
 "$HIPCC" --offload-arch=gfx950 -shared -fPIC -o "$OUT/libgbr.so.tmp" "$OUT/gbr_kernels.o" \
-  "$OUT/gbr_mlp.o" "$OUT/gbr_renderer.o" "$OUT/gbr_vis.o" "$OUT/gbr_import.o" -lz
+  "$OUT/gbr_mlp.o" "$OUT/gbr_renderer.o" "$OUT/gbr_vis.o" "$OUT/gbr_import.o" "$OUT/gbr_noise.o" -lz
 mv "$OUT/libgbr.so.tmp" "$OUT/libgbr.so"
-rm -f "$OUT/gbr_kernels.o" "$OUT/gbr_mlp.o" "$OUT/gbr_renderer.o" "$OUT/gbr_vis.o" "$OUT/gbr_import.o"
+rm -f "$OUT/gbr_kernels.o" "$OUT/gbr_mlp.o" "$OUT/gbr_renderer.o" "$OUT/gbr_vis.o" "$OUT/gbr_import.o" \
+  "$OUT/gbr_noise.o"
 echo "built $OUT/libgbr.so"

@@ -25,6 +25,7 @@ FORMAT_RGBA8, FORMAT_RGB8 = 0, 1
 MEMORY_HOST, MEMORY_DEVICE = 0, 1
 CULL_NONE, CULL_FRUSTUM, CULL_HIZ, CULL_LEARNED = 0, 1, 2, 3
 LIGHT_DIRECTIONAL, LIGHT_POINT, MAX_LIGHTS = 0, 1, 64
+REDWOOD_MODEL_FLOATS = 80 * 80 * 5
 TIMING_STAGES = ("nodes", "cull", "setup1", "clip1", "raster1", "hiz", "setup2", "clip2",
                  "raster2", "resolve")
 
@@ -214,6 +215,20 @@ SIGNATURES = {
     "gbr_vis_loss_grad": (S, [R, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_float,
                               ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
     "gbr_vis_predict": (S, [R, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    # Redwood depth noise
+    "gbr_redwood_create": (S, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64,
+                               ctypes.POINTER(ctypes.c_void_p)]),
+    "gbr_redwood_destroy": (S, [ctypes.c_void_p]),
+    "gbr_redwood_simulate": (S, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                 ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "gbr_redwood_simulate_host": (S, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_void_p]),
+    "gbr_redwood_simulate_normals": (S, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "gbr_redwood_calls": (S, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "gbr_redwood_random_bits": (S, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                    ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 _lib = None

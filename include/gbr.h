@@ -441,6 +441,45 @@ gbr_status gbr_vis_predict(gbr_renderer* r, const float* x, int32_t rows, float*
    counts 0); needs gbr_tuning.timing = 1; synchronises the stream */
 gbr_status gbr_frame_timings(gbr_renderer* r, float* ms);
 
+/* ---- Redwood depth noise: RedwoodNoiseModelGPUImpl (RedwoodNoiseModel.h:
+   50-97, RedwoodNoiseModel.cu; Python RedwoodDepthNoiseModel,
+   redwood_depth_noise_model.py:107-148) ------------------------------------ */
+typedef struct gbr_redwood gbr_redwood;
+/* the undistortion table (redwood-depth-dist-model.npy), [80][80][5] floats */
+#define GBR_REDWOOD_MODEL_FLOATS (80 * 80 * 5)
+/* RedwoodNoiseModelGPUImpl(model, gpuDeviceId, noiseMultiplier)
+   (RedwoodNoiseModel.h:58-62): copies the table to `device` (-1: current).
+   The reference seeds its per-thread curand states from rand(); here the
+   deviates come from a Philox4x32-10 counter generator keyed by `seed`, so
+   call k of a model reproduces from (seed, k). */
+gbr_status gbr_redwood_create(const float* model, int32_t device, float noise_multiplier,
+                              uint64_t seed, gbr_redwood** out);
+gbr_status gbr_redwood_destroy(gbr_redwood* m);
+/* simulateFromGPU(devDepth, rows, cols, devNoisyDepth) (RedwoodNoiseModel.h:
+   78-81) for `images` consecutive rows x cols depth images in device memory
+   (rows, cols >= 2), asynchronous on `stream` (a hipStream_t; NULL = the
+   null stream); every call draws fresh deviates.  Depths >= 10 m give 0, and
+   so do negative and NaN ones (the reference reads its table out of bounds
+   for those). */
+gbr_status gbr_redwood_simulate(gbr_redwood* m, const float* depth, int32_t images, int32_t rows,
+                                int32_t cols, float* out, void* stream);
+/* simulateFromCPU(depth, rows, cols) (RedwoodNoiseModel.h:70-72): host
+   image in, host image out, synchronous */
+gbr_status gbr_redwood_simulate_host(gbr_redwood* m, const float* depth, int32_t rows, int32_t cols,
+                                     float* out);
+/* new: gbr_redwood_simulate with caller-given standard normal deviates, 3
+   per pixel (y shift, x shift, disparity), in device memory */
+gbr_status gbr_redwood_simulate_normals(gbr_redwood* m, const float* depth, int32_t images,
+                                        int32_t rows, int32_t cols, const float* normals,
+                                        float* out, void* stream);
+/* new: generator calls so far (the next call's index) */
+gbr_status gbr_redwood_calls(const gbr_redwood* m, uint64_t* out);
+/* new: the generator's raw blocks: for pixels first .. first+n-1 of call
+   `call`, Philox4x32-10 of counter (pixel lo, pixel hi, call lo, call hi)
+   and key (seed lo, seed hi), 4 words each, into device memory */
+gbr_status gbr_redwood_random_bits(gbr_redwood* m, uint64_t call, uint64_t first, uint64_t n,
+                                   uint32_t* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

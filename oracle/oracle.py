@@ -69,6 +69,10 @@ def lib():
         _LIB.orc_mat4_mul.argtypes = [ctypes.c_void_p] * 3
         _LIB.orc_depth_unprojection.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _LIB.orc_unproject_depth.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        _LIB.orc_redwood.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+        _LIB.orc_philox4x32.argtypes = [ctypes.c_void_p] * 3
         _LIB.orc_log2_export.argtypes = [ctypes.c_float]
         _LIB.orc_log2_export.restype = ctypes.c_float
     return _LIB
@@ -605,3 +609,32 @@ def run_job(b: _Built, out: Dict[str, np.ndarray], nthreads: int = 1) -> None:
     j.draw_diffuse = _p(b.draw_diffuse)
     j.env_light, j.light_i, j.light_f = _p(b.env_light), _p(b.light_i), _p(b.light_f)
     lib().orc_render(ctypes.byref(j), int(nthreads))
+
+
+# ---------------------------------------------------------------------------
+# Redwood depth noise (oracle/redwood_ref.c)
+# ---------------------------------------------------------------------------
+
+def redwood_noise(depth: np.ndarray, model: np.ndarray, noise_multiplier: float,
+                  normals: np.ndarray) -> np.ndarray:
+    """RedwoodNoiseModel.cu:17-101 with given deviates: depth (H, W) or
+    (N, H, W) float32, normals (..., 3) per pixel."""
+    d = np.ascontiguousarray(depth, np.float32)
+    img = d.reshape((-1,) + d.shape[-2:])
+    m = np.ascontiguousarray(model, np.float32).reshape(-1)
+    assert m.size == 80 * 80 * 5 and img.shape[1] >= 2 and img.shape[2] >= 2
+    nrm = np.ascontiguousarray(normals, np.float32).reshape(-1)
+    assert nrm.size == 3 * img.size
+    out = np.empty_like(img)
+    lib().orc_redwood(img.ctypes.data, img.shape[0], img.shape[1], img.shape[2], m.ctypes.data,
+                       float(noise_multiplier), nrm.ctypes.data, out.ctypes.data)
+    return out.reshape(d.shape)
+
+
+def philox4x32(counter, key) -> np.ndarray:
+    """Philox4x32-10 of a 4-word counter and a 2-word key."""
+    c = np.ascontiguousarray(counter, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
