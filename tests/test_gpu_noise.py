@@ -107,3 +107,34 @@ def test_errors():
         m.simulate(np.zeros((1, 8), np.float32))
     assert e.value.status == N.ERR_ARGUMENT
     m.close()
+
+
+def test_on_the_renderers_linear_depth_plane():
+    """The batch renderer's linear-depth plane (one top-down image per env,
+    consecutive in device memory) fed straight to the kernel, against the
+    oracle's noise over the same observations."""
+    torch = _torch()
+    from habitat_sim_amd import RendererConfiguration, RendererStandalone, scenes
+    spec = scenes.small_rooms(n_envs=4, tile=64)
+    gx, gy = spec.grid
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy)),
+                           top_down=True)
+    scenes.populate(r, spec)
+    r.draw()
+    r.synchronize()
+    n_scenes = gx * gy
+    lin = r.plane_device_pointer(N.PLANE_LINEAR_DEPTH)
+    rng = np.random.default_rng(8)
+    normals = rng.standard_normal((n_scenes, 64, 64, 3)).astype(np.float32)
+    nd = torch.from_numpy(normals).cuda()
+    out = torch.empty((n_scenes, 64, 64), dtype=torch.float32, device="cuda")
+    m = RedwoodDepthNoiseModel(load_model(MODEL), noise_multiplier=1.0)
+    m.simulate_device_normals(lin, n_scenes, 64, 64, nd.data_ptr(), out.data_ptr())
+    torch.cuda.synchronize()
+    planes = np.empty((n_scenes, 64, 64), np.float32)
+    from habitat_sim_amd import hip
+    hip.memcpy_d2h(planes, lin)
+    exp = redwood_noise(planes, load_model(MODEL), 1.0, normals)
+    np.testing.assert_array_equal(out.cpu().numpy(), exp)
+    assert (exp > 0).mean() > 0.5  # the rooms are within 10 m
+    r.close()
