@@ -75,6 +75,20 @@ def memcpy_d2h(out, src: int) -> None:
            "hipMemcpy D2H")
 
 
+def memcpy_h2d(dst: int, src) -> None:
+    """Copies the host array src (src.nbytes) to device address dst."""
+    _check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src.ctypes.data), src.nbytes, H2D),
+           "hipMemcpy H2D")
+
+
+def to_device(a) -> DeviceBuffer:
+    """A device copy of the contiguous host array a."""
+    b = DeviceBuffer(max(a.nbytes, 1))
+    if a.nbytes:
+        memcpy_h2d(b.value, a)
+    return b
+
+
 def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
     _check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, D2D), "hipMemcpy D2D")
 

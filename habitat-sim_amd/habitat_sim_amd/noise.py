@@ -9,6 +9,11 @@ tensor on the same device, like the reference.  ``simulate_device`` works on
 raw device pointers, e.g. the batch renderer's linear-depth plane (one
 image per environment, consecutive).
 
+With torch tensors, initialise torch's GPU before this library loads
+(``torch.cuda.init()``): torch brings its own HIP runtime and libgbr then
+binds to the one already in the process; the other order leaves torch
+without a device.
+
 The undistortion table is the reference's ``redwood-depth-dist-model.npy``
 (80 x 80 x 5 floats), which the caller passes (``load_model(path)`` reads
 it).  Deviates come from a Philox4x32-10 counter generator keyed by

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from habitat_sim_amd import _native as N
+from habitat_sim_amd import hip
 from habitat_sim_amd.noise import RedwoodDepthNoiseModel, load_model
 from oracle.oracle import philox4x32, redwood_noise
 
@@ -17,41 +18,36 @@ pytestmark = pytest.mark.gpu
 MODEL = os.path.join(os.path.dirname(__file__), "golden", "noise", "redwood-depth-dist-model.npy")
 
 
-def _torch():
-    import torch
-    assert torch.cuda.is_available()
-    return torch
+def _host(buf, shape, dtype=np.float32):
+    out = np.empty(shape, dtype)
+    hip.hip().hipDeviceSynchronize()
+    hip.memcpy_d2h(out, buf.value)
+    return out
 
 
 @pytest.mark.parametrize("shape,mult", [((3, 64, 48), 1.0), ((1, 37, 53), 2.5), ((2, 2, 2), 1.0),
                                         ((4, 128, 128), 0.5)])
 def test_given_deviates_bit_exact(shape, mult):
-    torch = _torch()
     rng = np.random.default_rng(sum(shape))
     depth = rng.uniform(-0.5, 12.0, shape).astype(np.float32)
     depth.reshape(-1)[::97] = np.nan
     normals = rng.standard_normal(shape + (3,)).astype(np.float32) * 2.0
     m = RedwoodDepthNoiseModel(load_model(MODEL), noise_multiplier=mult, seed=1)
-    d = torch.from_numpy(depth).cuda()
-    n = torch.from_numpy(normals).cuda()
-    out = torch.empty_like(d)
-    m.simulate_device_normals(d.data_ptr(), shape[0], shape[1], shape[2], n.data_ptr(),
-                              out.data_ptr())
-    torch.cuda.synchronize()
+    d, n = hip.to_device(depth), hip.to_device(normals)
+    out = hip.DeviceBuffer(depth.nbytes)
+    m.simulate_device_normals(d.value, shape[0], shape[1], shape[2], n.value, out.value)
     exp = redwood_noise(depth, load_model(MODEL), mult, normals)
-    np.testing.assert_array_equal(out.cpu().numpy(), exp)
+    np.testing.assert_array_equal(_host(out, shape), exp)
     assert m.calls == 0  # caller deviates do not advance the generator
 
 
 def test_generator_blocks_bit_exact():
-    torch = _torch()
     seed = 0x0123456789ABCDEF
     m = RedwoodDepthNoiseModel(load_model(MODEL), seed=seed)
+    out = hip.DeviceBuffer(64 * 16)
     for call, first in ((0, 0), (7, 1000), (1 << 33, (1 << 32) - 5)):
-        out = torch.zeros(64 * 4, dtype=torch.int32, device="cuda")
-        m.random_bits_device(call, first, 64, out.data_ptr())
-        torch.cuda.synchronize()
-        got = out.cpu().numpy().view(np.uint32).reshape(64, 4)
+        m.random_bits_device(call, first, 64, out.value)
+        got = _host(out, (64, 4), np.uint32)
         for k in (0, 1, 31, 63):
             g = first + k
             exp = philox4x32([g & 0xFFFFFFFF, g >> 32, call & 0xFFFFFFFF, call >> 32],
@@ -60,7 +56,6 @@ def test_generator_blocks_bit_exact():
 
 
 def test_zero_noise_is_exact_and_host_equals_device():
-    torch = _torch()
     depth = np.linspace(0, 20, num=256 * 256, dtype=np.float32).reshape(256, 256)
     m0 = RedwoodDepthNoiseModel(load_model(MODEL), noise_multiplier=0.0)
     exp = redwood_noise(depth, load_model(MODEL), 0.0, np.zeros((256, 256, 3), np.float32))
@@ -69,16 +64,19 @@ def test_zero_noise_is_exact_and_host_equals_device():
     a = RedwoodDepthNoiseModel(load_model(MODEL), noise_multiplier=1.0, seed=9)
     b = RedwoodDepthNoiseModel(load_model(MODEL), noise_multiplier=1.0, seed=9)
     host = a.simulate(depth)
-    dev = b.simulate(torch.from_numpy(depth).cuda()).cpu().numpy()
-    np.testing.assert_array_equal(host, dev)
+    d = hip.to_device(depth)
+    out = hip.DeviceBuffer(depth.nbytes)
+    b.simulate_device(d.value, 1, 256, 256, out.value)
+    np.testing.assert_array_equal(_host(out, depth.shape), host)
     assert a.calls == 1 and b.calls == 1
     # the next call draws new deviates; a batch equals its images one by one
     assert not np.array_equal(a.simulate(depth), host)
-    batch = torch.from_numpy(np.stack([depth, depth[::-1].copy()])).cuda()
+    pair = np.ascontiguousarray(np.stack([depth, depth[::-1]]))
     c = RedwoodDepthNoiseModel(load_model(MODEL), seed=9)
-    out = c.simulate(batch)
-    assert out.shape == batch.shape
-    np.testing.assert_array_equal(out[0].cpu().numpy(), host)
+    dp = hip.to_device(pair)
+    op = hip.DeviceBuffer(pair.nbytes)
+    c.simulate_device(dp.value, 2, 256, 256, op.value)
+    np.testing.assert_array_equal(_host(op, pair.shape)[0], host)
 
 
 @pytest.mark.parametrize("noise_multiplier,tolerance", [(0.0, 1e-5), (1.0, 1e-2)])
@@ -113,7 +111,6 @@ def test_on_the_renderers_linear_depth_plane():
     """The batch renderer's linear-depth plane (one top-down image per env,
     consecutive in device memory) fed straight to the kernel, against the
     oracle's noise over the same observations."""
-    torch = _torch()
     from habitat_sim_amd import RendererConfiguration, RendererStandalone, scenes
     spec = scenes.small_rooms(n_envs=4, tile=64)
     gx, gy = spec.grid
@@ -126,15 +123,13 @@ def test_on_the_renderers_linear_depth_plane():
     lin = r.plane_device_pointer(N.PLANE_LINEAR_DEPTH)
     rng = np.random.default_rng(8)
     normals = rng.standard_normal((n_scenes, 64, 64, 3)).astype(np.float32)
-    nd = torch.from_numpy(normals).cuda()
-    out = torch.empty((n_scenes, 64, 64), dtype=torch.float32, device="cuda")
+    nd = hip.to_device(normals)
+    out = hip.DeviceBuffer(n_scenes * 64 * 64 * 4)
     m = RedwoodDepthNoiseModel(load_model(MODEL), noise_multiplier=1.0)
-    m.simulate_device_normals(lin, n_scenes, 64, 64, nd.data_ptr(), out.data_ptr())
-    torch.cuda.synchronize()
+    m.simulate_device_normals(lin, n_scenes, 64, 64, nd.value, out.value)
     planes = np.empty((n_scenes, 64, 64), np.float32)
-    from habitat_sim_amd import hip
     hip.memcpy_d2h(planes, lin)
     exp = redwood_noise(planes, load_model(MODEL), 1.0, normals)
-    np.testing.assert_array_equal(out.cpu().numpy(), exp)
+    np.testing.assert_array_equal(_host(out, planes.shape), exp)
     assert (exp > 0).mean() > 0.5  # the rooms are within 10 m
     r.close()
