@@ -1139,6 +1139,7 @@ gbr_status gbr_destroy(gbr_renderer* r) {
 gbr_status gbr_get_config(const gbr_renderer* r, gbr_config* out) {
   if (!r || !out) return fail(GBR_ERR_ARGUMENT, "gbr_get_config: null argument");
   *out = r->cfg;
+  out->device = r->device;  // the ordinal in use (resolved when -1 was given)
   return GBR_OK;
 }
 

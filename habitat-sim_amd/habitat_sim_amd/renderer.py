@@ -474,6 +474,28 @@ class RendererStandalone:
         N.call("gbr_plane_device_pointer", self._h, plane, ctypes.byref(out))
         return out.value
 
+    @property
+    def device_id(self) -> int:
+        """The HIP device ordinal the renderer runs on."""
+        cfg = N.Config()
+        N.call("gbr_get_config", self._h, ctypes.byref(cfg))
+        return cfg.device
+
+    def plane_dlpack(self, plane: int):
+        """Zero-copy DLPack view of a whole output plane, one image per
+        scene: colour (scenes, H, W, 4) uint8, depth and linear depth
+        (scenes, H, W) float32, semantic (scenes, H, W) int32; valid until
+        the next draw() (``torch.from_dlpack(r.plane_dlpack(PLANE_COLOR))``)."""
+        from .dlpack import DeviceArray
+        w, h = self.tile_size
+        n = self.scene_count
+        shape, dtype = {N.PLANE_COLOR: ((n, h, w, 4), np.uint8),
+                        N.PLANE_DEPTH: ((n, h, w), np.float32),
+                        N.PLANE_LINEAR_DEPTH: ((n, h, w), np.float32),
+                        N.PLANE_SEMANTIC: ((n, h, w), np.int32)}[plane]
+        return DeviceArray(self.plane_device_pointer(plane), shape, dtype, self.device_id, self,
+                           sync=self.synchronize)
+
     # -- tuning / counters ---------------------------------------------------------
     def tuning(self) -> N.Tuning:
         t = N.Tuning()

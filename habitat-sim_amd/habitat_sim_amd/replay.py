@@ -211,6 +211,31 @@ class BatchReplayRenderer:
             out[uuid] = full[:n] if env is None else full[env]
         return out
 
+    def observations_dlpack(self) -> Dict[str, object]:
+        """Zero-copy DLPack views of every sensor's plane over all
+        environments (leading env axis, rows top-down; semantic as int32),
+        the analogue of the reference's gpu2gpu_transfer observations
+        (sensor_wrapper.py:97-124); valid until the next draw()."""
+        n = self.num_environments
+        out = {}
+        for uuid, s in self.sensors.items():
+            full = self.renderer.plane_dlpack(_PLANES[s.sensor_type])
+            full.shape = (n,) + full.shape[1:]  # the grid's trailing empty tiles dropped
+            out[uuid] = full
+        return out
+
+    def get_observations_torch(self, env: Optional[int] = None, copy: bool = False):
+        """get_observations() as torch tensors on the GPU, aliasing the
+        renderer's planes (copy=True clones them); torch's GPU must be
+        initialised before libgbr loads."""
+        import torch
+        out = {}
+        for uuid, arr in self.observations_dlpack().items():
+            t = torch.from_dlpack(arr)
+            t = t if env is None else t[env]
+            out[uuid] = t.clone() if copy else t
+        return out
+
     def _plane_host(self, plane: int, kind: str) -> np.ndarray:
         from . import hip
         h, w = self.resolution

@@ -93,7 +93,8 @@ typedef struct gbr_config {
   uint32_t flags;         /* GBR_FLAG_* */
   uint32_t max_light_count; /* 0: every material flat; at most GBR_MAX_LIGHTS */
   float ambient_factor;   /* default 0.1 */
-  int32_t device;         /* HIP device ordinal; -1 = current device */
+  int32_t device;         /* HIP device ordinal; -1 = current device
+                             (gbr_get_config reports the ordinal in use) */
   uint32_t standalone_flags; /* GBR_STANDALONE_* */
   uint32_t planes;        /* GBR_PLANE_* produced by draw(); default ALL */
   uint32_t top_down;      /* new: per-scene planes stored top-down */
