@@ -224,7 +224,7 @@ def main():
         ap.error(f"--modes: none of {[m for m, _ in MODES]}")
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "environments shard across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
+              "environments would shard across GPUs with no exchange between them", file=sys.stderr)
         sys.exit(2)
 
     import __graft_entry__
