@@ -2221,7 +2221,9 @@ __device__ __forceinline__ uint32_t f2bf_bits(float f) {
   return u >> 16;
 }
 
-__device__ __forceinline__ float log2c(float x) { return log2f(fmaxf(x, 1e-3f)); }
+// features only steer the proposal (any proposal renders exactly), so they
+// use the hardware log2 and reciprocal (~1 ulp)
+__device__ __forceinline__ float log2c(float x) { return __builtin_amdgcn_logf(fmaxf(x, 1e-3f)); }
 
 // The 32 features of a (camera, work item) pair for the visibility model:
 //  0-2   visibility history: visible last frame, within 8 frames, age/64
@@ -2236,7 +2238,7 @@ __device__ __forceinline__ float log2c(float x) { return log2f(fmaxf(x, 1e-3f));
 __device__ void item_features(const FrameParams& p, const WorkItem& wi, const SceneDesc& sc,
                               const ItemBounds& bb, const float* M, const float* PV,
                               const ItemCull& ic, float wmin, float wmax, uint32_t item,
-                              uint16_t* row) {
+                              uint4* row) {
   float f[32];
   const uint32_t seen = p.item_vis[item], frame = *p.frame;
   f[0] = (seen + 1u == frame) ? 1.0f : 0.0f;
@@ -2249,12 +2251,13 @@ __device__ void item_features(const FrameParams& p, const WorkItem& wi, const Sc
   f[6] = (ic.y1 + 1) * iH;
   f[7] = (f[5] - f[3]) * (f[6] - f[4]);
   const float un0 = p.unprojections[2 * wi.scene], un1 = p.unprojections[2 * wi.scene + 1];
-  f[8] = log2c(un1 / (fmaxf(ic.zmin, 0.0f) + un0)) / 8.0f;
+  f[8] = log2c(__fdividef(un1, fmaxf(ic.zmin, 0.0f) + un0)) / 8.0f;
   const float cx = 0.5f * (bb.lo[0] + bb.hi[0]), cy = 0.5f * (bb.lo[1] + bb.hi[1]),
               cz = 0.5f * (bb.lo[2] + bb.hi[2]);
   const float4 cc = vertex_clip(M, PV, cx, cy, cz);
-  f[9] = fmaxf(fminf(cc.x / cc.w, 4.0f), -4.0f);
-  f[10] = fmaxf(fminf(cc.y / cc.w, 4.0f), -4.0f);
+  const float icw = __builtin_amdgcn_rcpf(cc.w);
+  f[9] = fmaxf(fminf(cc.x * icw, 4.0f), -4.0f);
+  f[10] = fmaxf(fminf(cc.y * icw, 4.0f), -4.0f);
   f[11] = log2c(cc.w) / 8.0f;
   f[12] = log2c(wmin) / 8.0f;
   f[13] = log2c(wmax) / 8.0f;
@@ -2262,7 +2265,7 @@ __device__ void item_features(const FrameParams& p, const WorkItem& wi, const Sc
   float wc[4], wd[4];
   mat4_vec(M, cx, cy, cz, 1.0f, wc);
   mat4_vec(M, bb.hi[0] - bb.lo[0], bb.hi[1] - bb.lo[1], bb.hi[2] - bb.lo[2], 0.0f, wd);
-  f[15] = log2c(sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2])) / 4.0f;
+  f[15] = log2c(__builtin_amdgcn_sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2])) / 4.0f;
   const float* cam = p.cam_info + (size_t)wi.scene * 8;
   float d[3], dd = 0.0f, cosf_ = 0.0f;
 #pragma unroll
@@ -2275,12 +2278,11 @@ __device__ void item_features(const FrameParams& p, const WorkItem& wi, const Sc
     dd += d[k] * d[k];
     cosf_ += d[k] * cam[3 + k];
   }
-  dd = sqrtf(dd);
+  dd = __builtin_amdgcn_sqrtf(dd);
   f[28] = dd / 8.0f;
-  f[29] = dd > 0.0f ? cosf_ / dd : 1.0f;
+  f[29] = dd > 0.0f ? cosf_ * __builtin_amdgcn_rcpf(dd) : 1.0f;
   f[30] = (ic.x0 == 0 || ic.y0 == 0 || ic.x1 == sc.width - 1 || ic.y1 == sc.height - 1) ? 1.0f : 0.0f;
   f[31] = 1.0f;
-  uint4* dst = reinterpret_cast<uint4*>(row);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     uint4 v;
@@ -2288,15 +2290,16 @@ __device__ void item_features(const FrameParams& p, const WorkItem& wi, const Sc
     v.y = f2bf_bits(f[8 * q + 2]) | (f2bf_bits(f[8 * q + 3]) << 16);
     v.z = f2bf_bits(f[8 * q + 4]) | (f2bf_bits(f[8 * q + 5]) << 16);
     v.w = f2bf_bits(f[8 * q + 6]) | (f2bf_bits(f[8 * q + 7]) << 16);
-    dst[q] = v;
+    row[q] = v;
   }
 }
 
 __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
   const uint32_t item = blockIdx.x * kListBlock + threadIdx.x;
   bool to1 = false, toc = false, feat = false;
-  ItemCull icf = {};
-  float wmin_f = 0.0f, wmax_f = 0.0f;
+  // the feature row, packed BF16, computed while the item's data is in
+  // registers and stored once its row index is known
+  uint4 frow[4] = {};
   if (item < p.n_items) {
     const WorkItem wi = p.items[item];
     const SceneDesc sc = p.scenes[wi.scene];
@@ -2359,9 +2362,7 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
           ic.pad[1] = 0u;
           if (p.features) {
             feat = true;
-            icf = ic;
-            wmin_f = wmin;
-            wmax_f = wmax;
+            item_features(p, wi, sc, bb, M, PV, ic, wmin, wmax, item, frow);
           }
           if (p.cull_mode == kCullLearned) {
             p.cull[item] = ic;  // k_propose decides from the model's score
@@ -2386,17 +2387,12 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
   block_slots<3>(ctr, flag, slot);
   if (to1) p.list1[slot[0]] = item;
   if (toc) p.cand[slot[1]] = item;
-  if (p.features) {
+  if (feat) {
     const uint32_t row = slot[2];
-    if (feat) {
-      p.prop[row] = item;
-      const WorkItem wi = p.items[item];
-      float M[16], PV[16];
-      load_mat(p.abs_nodes + (size_t)wi.node * 16, M);
-      load_mat(p.cameras + (size_t)wi.scene * 16, PV);
-      item_features(p, wi, p.scenes[wi.scene], p.bounds[item], M, PV, icf, wmin_f, wmax_f, item,
-                    p.features + (size_t)row * 32);
-    }
+    p.prop[row] = item;
+    uint4* dst = reinterpret_cast<uint4*>(p.features + (size_t)row * 32);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = frow[q];
   }
 }
 
