@@ -149,7 +149,9 @@ def stage_bytes(stage, spec, st, n_items, n_nodes, learned=False):
 def run_mode(r, mode, spec, views, proj, warmup, steps):
     """Times `steps` camera-update + draw() steps after `warmup` untimed
     ones, with stage timing off (its events sit between kernels); then
-    replays the same views with stage timing on for the per-stage split."""
+    replays the same views with stage timing on for the per-stage split,
+    and once more one frame at a time for the p50/p99 frame latency."""
+    import numpy as np
     from habitat_sim_amd import hip
     r.set_tuning(cull_mode=mode, timing=0)
     stream = r.stream()
@@ -184,7 +186,19 @@ def run_mode(r, mode, spec, views, proj, warmup, steps):
     r.synchronize()
     stages = r.frame_timings()
     r.set_tuning(timing=0)
+    # frame latency: camera update + draw() to the frame's completion, one
+    # frame at a time (not part of the timed run)
+    lat = []
+    for k in range(min(steps, 30)):
+        t1 = time.perf_counter()
+        r.update_cameras(0, proj, views[warmup + k])
+        r.draw()
+        r.synchronize()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    latency = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+               "frames": len(lat)}
     return {"wall": wall, "gpu_ms": gpu_ms, "stages": stages, "stats": stats, "planes": planes,
+            "latency_ms": latency,
             "tuning": {k: tuned["best"][k] for k in ("bin_size", "bin_capacity_factor",
                                                      "lane_pixels", "queue_grain", "raster_mode")},
             "autotune": {"candidates": tuned["candidates"], "default_ms": tuned["initial_ms"],
@@ -283,6 +297,7 @@ def main():
             "triangles_setup": x["stats"]["triangles_setup"],
             "bin_entries": x["stats"]["bin_entries"],
             "stage_ms": {k: round(v, 4) for k, v in x["stages"].items() if v > 0},
+            "latency_ms": x["latency_ms"],
             "tuning": x["tuning"],
             "autotune": x["autotune"],
         }
