@@ -176,6 +176,7 @@ def run_mode(r, mode, spec, views, proj, warmup, steps):
     wall = time.perf_counter() - t0
     gpu_ms = start.elapsed_ms(stop) / steps
     stats = r.frame_stats()
+    visibility = r.frame_visibility()
     planes = {"color": r.color_image(), "semantic": r.semantic_image(),
               "linear_depth": r.linear_depth_image()}
     # stage split (events between the stages; not part of the timed run)
@@ -198,7 +199,7 @@ def run_mode(r, mode, spec, views, proj, warmup, steps):
     latency = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                "frames": len(lat)}
     return {"wall": wall, "gpu_ms": gpu_ms, "stages": stages, "stats": stats, "planes": planes,
-            "latency_ms": latency,
+            "latency_ms": latency, "visibility": visibility,
             "tuning": {k: tuned["best"][k] for k in ("bin_size", "bin_capacity_factor",
                                                      "lane_pixels", "queue_grain", "raster_mode")},
             "autotune": {"candidates": tuned["candidates"], "default_ms": tuned["initial_ms"],
@@ -298,6 +299,8 @@ def main():
             "bin_entries": x["stats"]["bin_entries"],
             "stage_ms": {k: round(v, 4) for k, v in x["stages"].items() if v > 0},
             "latency_ms": x["latency_ms"],
+            "proposal_recall": x["visibility"]["recall"],
+            "visible_items": x["visibility"]["visible_items"],
             "tuning": x["tuning"],
             "autotune": x["autotune"],
         }

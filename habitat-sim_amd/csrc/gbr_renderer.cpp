@@ -1824,6 +1824,43 @@ gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
   return GBR_OK;
 }
 
+// Visible work items of the last frame (those owning a pixel) and how many
+// of them phase 1 set up: the proposal's recall is visible_phase1 / visible.
+gbr_status gbr_frame_visibility(gbr_renderer* r, uint64_t* visible, uint64_t* visible_phase1) {
+  if (!r || !visible || !visible_phase1) return fail(GBR_ERR_ARGUMENT, "frame_visibility: null argument");
+  *visible = *visible_phase1 = 0;
+  if (!r->d_frame.ptr || !r->d_item_vis.ptr || !r->n_items) return GBR_OK;
+  TRY(set_device(r));
+  uint32_t frame = 0;
+  std::vector<uint32_t> vis(r->n_items);
+  HIP_TRY(hipMemcpyAsync(&frame, r->d_frame.ptr, 4, hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipMemcpyAsync(vis.data(), r->d_item_vis.ptr, vis.size() * 4, hipMemcpyDeviceToHost,
+                         r->stream));
+  std::vector<uint32_t> list;
+  if (r->tun.cull_mode != GBR_CULL_NONE && r->d_counters.ptr && r->d_list1.ptr) {
+    uint32_t n1 = 0;
+    HIP_TRY(hipMemcpyAsync(&n1, r->d_counters.ptr + kCtrList1 * kCtrStride, 4, hipMemcpyDeviceToHost,
+                           r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    list.resize(std::min(n1, r->n_items));
+    if (!list.empty())
+      HIP_TRY(hipMemcpyAsync(list.data(), r->d_list1.ptr, list.size() * 4, hipMemcpyDeviceToHost,
+                             r->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  uint64_t nv = 0;
+  for (uint32_t v : vis) nv += v == frame;
+  *visible = nv;
+  if (r->tun.cull_mode == GBR_CULL_NONE) {
+    *visible_phase1 = nv;
+  } else {
+    uint64_t n = 0;
+    for (uint32_t it : list) n += it < r->n_items && vis[it] == frame;
+    *visible_phase1 = n;
+  }
+  return GBR_OK;
+}
+
 gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
   if (!r || !out) return fail(GBR_ERR_ARGUMENT, "frame_stats: null argument");
   std::memset(out, 0, sizeof *out);

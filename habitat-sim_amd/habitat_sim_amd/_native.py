@@ -215,6 +215,7 @@ SIGNATURES = {
     "gbr_vis_loss_grad": (S, [R, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_float,
                               ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
     "gbr_vis_predict": (S, [R, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "gbr_frame_visibility": (S, [R, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     # Redwood depth noise
     "gbr_redwood_create": (S, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64,
                                ctypes.POINTER(ctypes.c_void_p)]),

@@ -474,6 +474,14 @@ class RendererStandalone:
         N.call("gbr_plane_device_pointer", self._h, plane, ctypes.byref(out))
         return out.value
 
+    def frame_visibility(self) -> dict:
+        """Work items of the last frame owning a visible pixel, how many of
+        them phase 1 set up, and the ratio (the proposal's recall)."""
+        v, v1 = ctypes.c_uint64(), ctypes.c_uint64()
+        N.call("gbr_frame_visibility", self._h, ctypes.byref(v), ctypes.byref(v1))
+        return {"visible_items": v.value, "visible_in_phase1": v1.value,
+                "recall": v1.value / v.value if v.value else 1.0}
+
     @property
     def device_id(self) -> int:
         """The HIP device ordinal the renderer runs on."""

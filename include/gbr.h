@@ -367,6 +367,10 @@ gbr_status gbr_unproject_depth(const float* unprojection2, float* depth, int64_t
 gbr_status gbr_get_tuning(const gbr_renderer* r, gbr_tuning* out);
 gbr_status gbr_set_tuning(gbr_renderer* r, const gbr_tuning* tuning);
 gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out);
+/* new: of the last frame's work items, those owning a visible pixel, and how
+   many of them phase 1 set up (the proposal's recall is the ratio; 1 without
+   culling and in frustum mode); synchronises the stream */
+gbr_status gbr_frame_visibility(gbr_renderer* r, uint64_t* visible, uint64_t* visible_phase1);
 
 /* new: autotuner.  Times gbr_draw() of the current scenes and cameras under
    candidate launch parameters, one parameter at a time (bin size, per-bin

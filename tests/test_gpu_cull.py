@@ -137,3 +137,25 @@ def test_learned_mode_needs_a_model():
         r.set_tuning(cull_mode=N.CULL_LEARNED)
     assert e.value.status == N.ERR_STATE
     r.close()
+
+
+def test_visible_items_agree_across_modes_and_recall():
+    """The work items owning a visible pixel are the same in every mode (the
+    pixels are); phase 1 holds all of them without culling and in frustum
+    mode, and most of them with the previous-frame proposal."""
+    spec = scenes.apartments(n_envs=16)
+    walk = _walk(spec, 3, seed=5)
+    vis = {}
+    for mode in (N.CULL_NONE, N.CULL_FRUSTUM, N.CULL_HIZ):
+        r = _renderer(spec, mode)
+        for views in walk:
+            for e, v in enumerate(views):
+                r.update_camera(e, spec.envs[e].projection, v)
+            r.draw()
+        vis[mode] = r.frame_visibility()
+        r.close()
+    n = vis[N.CULL_NONE]["visible_items"]
+    assert n > 0
+    assert all(v["visible_items"] == n for v in vis.values()), vis
+    assert vis[N.CULL_NONE]["recall"] == 1.0 and vis[N.CULL_FRUSTUM]["recall"] == 1.0
+    assert 0.5 < vis[N.CULL_HIZ]["recall"] <= 1.0, vis
