@@ -1067,18 +1067,27 @@ __device__ __forceinline__ float gbr_log2(float x) {
   return (float)(e - 127) + l;
 }
 
-__device__ __forceinline__ int wrap_coord(int i, int size, int mode) {
-  if (mode == GL_CLAMP_TO_EDGE) return i < 0 ? 0 : (i >= size ? size - 1 : i);
+// mirrored repeat, and repeat of a size that is not a power of two
+__device__ __noinline__ int wrap_coord_slow(int i, int size, int mode) {
   if (mode == GL_MIRRORED_REPEAT) {
     int pp = 2 * size;
     int m = i % pp;
     if (m < 0) m += pp;
     return m < size ? m : pp - 1 - m;
   }
-  if ((size & (size - 1)) == 0) return i & (size - 1);  // non-negative i mod size
   int m = i % size;
   if (m < 0) m += size;
   return m;
+}
+
+// GL wrap of texel index i: clamp-to-edge and power-of-two repeat (the
+// common cases) by selects, the rest out of line
+__device__ __forceinline__ int wrap_coord(int i, int size, int mode) {
+  const bool clamp = mode == GL_CLAMP_TO_EDGE;
+  const bool fast = clamp || (mode != GL_MIRRORED_REPEAT && (size & (size - 1)) == 0);
+  if (__builtin_expect(fast, 1))
+    return clamp ? min(max(i, 0), size - 1) : (i & (size - 1));  // non-negative i mod size
+  return wrap_coord_slow(i, size, mode);
 }
 
 __device__ __forceinline__ int floor_to_int(float u) {
