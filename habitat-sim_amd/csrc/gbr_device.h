@@ -96,15 +96,28 @@ struct alignas(32) RasterRec {
   float z[3];
 };
 
-// Resolve part (64 bytes): clip-space x, y, w and texture coordinates of the
-// unclipped triangle, and its work item (whose draw gives the material).
+// Resolve part (64 bytes): the homogeneous barycentric rows of the
+// unclipped triangle (e[i] = cross products of its clip-space (x, y, w),
+// computed once per triangle by setup), its texture coordinates, and its
+// work item (whose draw gives the material).
 struct alignas(64) ResolveRec {
-  float cx0, cy0, cw0, cx1;
-  float cy1, cw1, cx2, cy2;
-  float cw2, s0, t0, s1;
-  float t1, s2, t2;
+  float e[9];  // e[3*i + k]: row i, coefficient k (of nx, ny, 1)
+  float s0, t0, s1, t1, s2, t2;
   uint32_t item;
 };
+
+// Rows of the homogeneous barycentric form from the clip-space x, y, w of
+// the three vertices (op order fixed: products, then the difference).
+__host__ __device__ inline void bary_rows_of(const float* cx, const float* cy, const float* cw,
+                                             float* e) {
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    float pp, qq;
+    pp = cy[j] * cw[k]; qq = cw[j] * cy[k]; e[3 * i + 0] = pp - qq;
+    pp = cw[j] * cx[k]; qq = cx[j] * cw[k]; e[3 * i + 1] = pp - qq;
+    pp = cx[j] * cy[k]; qq = cy[j] * cx[k]; e[3 * i + 2] = pp - qq;
+  }
+}
 
 // 24-bit fields of RasterRec::xy; window coordinates stay within the guard
 // band, |X|, |Y| < 2^23

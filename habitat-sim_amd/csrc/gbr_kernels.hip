@@ -713,9 +713,12 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
           p.rrecs[(size_t)sc.rec_base + prim] = rr;
           ResolveRec rec;
           const uint32_t l0 = vi[0] - wi.vmin, l1 = vi[1] - wi.vmin, l2 = vi[2] - wi.vmin;
-          rec.cx0 = s_cx[l0]; rec.cy0 = s_cy[l0]; rec.cw0 = s_cw[l0];
-          rec.cx1 = s_cx[l1]; rec.cy1 = s_cy[l1]; rec.cw1 = s_cw[l1];
-          rec.cx2 = s_cx[l2]; rec.cy2 = s_cy[l2]; rec.cw2 = s_cw[l2];
+          {
+            const float cx[3] = {s_cx[l0], s_cx[l1], s_cx[l2]};
+            const float cy[3] = {s_cy[l0], s_cy[l1], s_cy[l2]};
+            const float cw[3] = {s_cw[l0], s_cw[l1], s_cw[l2]};
+            bary_rows_of(cx, cy, cw, rec.e);
+          }
           rec.s0 = s_s[l0]; rec.t0 = s_t[l0];
           rec.s1 = s_s[l1]; rec.t1 = s_t[l1];
           rec.s2 = s_s[l2]; rec.t2 = s_t[l2];
@@ -1229,21 +1232,9 @@ __device__ void sample_tex(const FrameParams& p, const float* lut, const Texture
 // ---- resolve ----------------------------------------------------------------
 
 struct TriAttr {
-  float cx[3], cy[3], cw[3];
   float s[3], t[3];
-  float e[3][3];
+  float e[3][3];  // homogeneous barycentric rows (bary_rows_of)
 };
-
-__device__ __forceinline__ void bary_rows(TriAttr& A) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    int j = (i + 1) % 3, k = (i + 2) % 3;
-    float pp, qq;
-    pp = A.cy[j] * A.cw[k]; qq = A.cw[j] * A.cy[k]; A.e[i][0] = pp - qq;
-    pp = A.cw[j] * A.cx[k]; qq = A.cx[j] * A.cw[k]; A.e[i][1] = pp - qq;
-    pp = A.cx[j] * A.cy[k]; qq = A.cy[j] * A.cx[k]; A.e[i][2] = pp - qq;
-  }
-}
 
 __device__ __forceinline__ void bary_at(const TriAttr& A, float hw, float hh, float ihw, float ihh,
                                         float fx, float fy, float* b) {
@@ -1610,9 +1601,12 @@ __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int ph
       for (int q = 0; q < 9; ++q) tm[q] = mat.texmat[q];
       ResolveRec rec;
       rec.item = ref.item;
-      rec.cx0 = c[0].x; rec.cy0 = c[0].y; rec.cw0 = c[0].w;
-      rec.cx1 = c[1].x; rec.cy1 = c[1].y; rec.cw1 = c[1].w;
-      rec.cx2 = c[2].x; rec.cy2 = c[2].y; rec.cw2 = c[2].w;
+      {
+        const float cx[3] = {c[0].x, c[1].x, c[2].x};
+        const float cy[3] = {c[0].y, c[1].y, c[2].y};
+        const float cw[3] = {c[0].w, c[1].w, c[2].w};
+        bary_rows_of(cx, cy, cw, rec.e);
+      }
       vertex_uv(p, d.flags, ip[0], tm, rec.s0, rec.t0);
       vertex_uv(p, d.flags, ip[1], tm, rec.s1, rec.t1);
       vertex_uv(p, d.flags, ip[2], tm, rec.s2, rec.t2);
@@ -2103,13 +2097,13 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
       const uint32_t item = min(rs.item, p.n_items - 1u);
       d = p.item_draws[item];
       if (p.item_vis) p.item_vis[item] = frame;  // visibility feedback for the next frame
-      A.cx[0] = rs.cx0; A.cy[0] = rs.cy0; A.cw[0] = rs.cw0;
-      A.cx[1] = rs.cx1; A.cy[1] = rs.cy1; A.cw[1] = rs.cw1;
-      A.cx[2] = rs.cx2; A.cy[2] = rs.cy2; A.cw[2] = rs.cw2;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) A.e[i][k] = rs.e[3 * i + k];
       A.s[0] = rs.s0; A.t[0] = rs.t0;
       A.s[1] = rs.s1; A.t[1] = rs.t1;
       A.s[2] = rs.s2; A.t[2] = rs.t2;
-      bary_rows(A);
       bary_at(A, hw, hh, ihw, ihh, (float)px + 0.5f, (float)py + 0.5f, bc);
       textured = !p.no_textures && d.texture > 0;
       if (textured) {
