@@ -124,3 +124,57 @@ def test_uneven_batch_learned_cull_on_1024_triangle_chunks():
     assert st["items_phase1"] > 0
     plain.close()
     learned.close()
+
+
+def test_full_size_uneven_batch_1024_envs():
+    """The uneven batch at full size (1024 envs of 10k-1M triangles, 64/128/
+    256 px viewports, 1-3 sensors; one graph launch per frame): depth-pyramid
+    culling changes no produced pixel, pixels outside a viewport read back
+    clear, and a seeded sample of envs is bit-exact with the oracle."""
+    from habitat_sim_amd import math3d as m3
+    spec = scenes.heterogeneous(n_envs=1024)
+    assert sum(spec.triangles(e) for e in range(1024)) > 1e8
+    r = _renderer(spec)
+    r.draw()
+    plain = {name: getattr(r, f"{name}_image")() for name in PLANE_OF}
+    r.set_tuning(cull_mode=N.CULL_HIZ)
+    r.draw()
+    r.draw()
+    tw, th = spec.tile
+    for name, bit in PLANE_OF.items():
+        culled = getattr(r, f"{name}_image")()
+        for s, env in enumerate(spec.envs):
+            if env.planes and not env.planes & bit:
+                continue
+            a, b = _tile(plain[name], spec, s), _tile(culled, spec, s)
+            d = a != b
+            assert not (d.any(-1) if d.ndim == 3 else d).any(), f"scene {s} {name}"
+            w, h = env.viewport or spec.tile
+            clear = np.asarray(CLEAR[name], a.dtype)
+            assert (a[h:] == clear).all() and (a[:, w:] == clear).all(), (s, name)
+    r.close()
+    # a sample of every resolution through the oracle
+    rng = np.random.default_rng(12)
+    by_res = {}
+    for s, env in enumerate(spec.envs):
+        by_res.setdefault(env.viewport, []).append(s)
+    sample = sorted(int(rng.choice(v)) for v in by_res.values())
+    gx, gy = spec.grid
+    o = OracleRenderer(spec.tile, (gx, gy))
+    o.add_file_data(spec.file)
+    for s in sample:
+        env = spec.envs[s]
+        for inst in env.instances:
+            o.add_node_hierarchy(s, inst.template, inst.bake, inst.semantic)
+        o.update_camera(s, env.projection, m3.inverted_rigid(env.camera_pose))
+        if env.viewport is not None:
+            o.set_scene_viewport(s, env.viewport)
+    planes = o.render(NTHREADS, scenes=sample)
+    key = {"color": "color", "depth": "depth", "linear_depth": "lindepth", "semantic": "semantic"}
+    for k, s in enumerate(sample):
+        env = spec.envs[s]
+        for name, bit in PLANE_OF.items():
+            if env.planes and not env.planes & bit:
+                continue
+            np.testing.assert_array_equal(_tile(plain[name], spec, s), planes[key[name]][k],
+                                          err_msg=f"env {s} {name}")
