@@ -206,6 +206,78 @@ def run_mode(r, mode, spec, views, proj, warmup, steps):
                          "tuned_ms": tuned["best_ms"]}}
 
 
+def rollout(args):
+    """Trajectory rollout: `--rollout` envs of the apartment workload with
+    128x128 linear depth + semantic sensors, `--rollout-steps` steps of the
+    seeded random walk (0.25 m forward / 10 degrees left / right, clamped to
+    the apartment), learned proposal with the previous-frame depth pyramid;
+    visibility rows are collected every frame and the proposal MLP is
+    fine-tuned on them every 50 steps.  Prints one JSON line: env-frames/s
+    over the whole rollout (row collection and fine-tuning included, the
+    recall readback excluded), p50/p99 frame latency (camera update to the
+    frame's completion) and the proposal's recall (visible work items that
+    phase 1 set up)."""
+    import numpy as np
+    from habitat_sim_amd import RendererConfiguration, RendererStandalone, scenes, visibility
+    from habitat_sim_amd import _native as N
+
+    t_setup = time.perf_counter()
+    spec = scenes.apartments(n_envs=args.rollout, tile=128)
+    gx, gy = spec.grid
+    cfg = RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy))
+    r = RendererStandalone(cfg, planes=N.PLANE_LINEAR_DEPTH | N.PLANE_SEMANTIC, top_down=True)
+    scenes.populate(r, spec)
+    steps = args.rollout_steps
+    views = camera_walk(spec, steps, seed=11)
+    proj = np.stack([env.projection for env in spec.envs]).astype(np.float32)
+    print(f"bench: rollout of {len(spec.envs)} envs set up in {time.perf_counter() - t_setup:.0f} s",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    initial = visibility.train(r, spec, frames=args.train_frames, steps=args.train_steps, seed=3)
+    train_s = time.perf_counter() - t0
+    r.set_tuning(cull_mode=N.CULL_LEARNED)
+    r.update_cameras(0, proj, views[0])
+    r.draw()
+    r.synchronize()
+    r.vis_clear_rows()
+    lat, recall, fits = [], [], []
+    excluded = 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        t1 = time.perf_counter()
+        r.update_cameras(0, proj, views[k])
+        r.vis_collect()
+        r.synchronize()
+        lat.append((time.perf_counter() - t1) * 1e3)
+        t2 = time.perf_counter()
+        recall.append(r.frame_visibility()["recall"])
+        excluded += time.perf_counter() - t2
+        if (k + 1) % 50 == 0:
+            fits.append(r.vis_fit(steps=20, batch=4096, learning_rate=1e-3, seed=100 + k))
+            r.vis_clear_rows()
+            print(f"bench: rollout step {k + 1}, recall {recall[-1]:.3f}, "
+                  f"loss {fits[-1]['loss_last']:.3f}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0 - excluded
+    n = len(spec.envs)
+    out = {
+        "metric": "rollout env-frames/s, learned cull with online fine-tuning",
+        "value": n * steps / wall, "unit": "env-frames/s", "n_gpus": 1, "steps": steps,
+        "higher_is_better": True, "data": "synthetic", "dtype": "f32",
+        "config": {"workload": f"apartments-{n}env-rollout-{steps}steps-128x128-depth-semantic",
+                   "envs": n, "resolution": [128, 128],
+                   "triangles_per_frame": int(sum(spec.triangles(e) for e in range(n))),
+                   "sensors": ["linear_depth_f32", "semantic_u32"], "cull": "learned",
+                   "finetune_every": 50},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "proposal_recall": {"mean": float(np.mean(recall)), "min": float(np.min(recall)),
+                            "first": recall[0], "last": recall[-1]},
+        "initial_training": {"seconds": train_s, "loss_last": initial["loss_last"]},
+        "fine_tune_loss_last": [f["loss_last"] for f in fits],
+    }
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
 def diff_pixels(a, b):
     import numpy as np
     d = np.zeros(a["semantic"].shape, bool)
@@ -230,6 +302,10 @@ def main():
     ap.add_argument("--modes", default=",".join(m for m, _ in MODES),
                     help="culling modes to time, comma-separated (profiling runs time "
                          "fewer; the speedup needs plain and the headline mode)")
+    ap.add_argument("--rollout", type=int, default=0, metavar="ENVS",
+                    help="instead of the benchmark, a trajectory rollout of ENVS envs with "
+                         "online fine-tuning (its own JSON line)")
+    ap.add_argument("--rollout-steps", type=int, default=500)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (the reference kernel "
                          "for scaling hardware byte counters) and exit")
@@ -246,6 +322,9 @@ def main():
     __graft_entry__.ensure_built()
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
+        return
+    if args.rollout:
+        rollout(args)
         return
     import numpy as np
     from habitat_sim_amd import RendererConfiguration, RendererStandalone, hip, scenes
