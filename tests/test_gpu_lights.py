@@ -201,3 +201,24 @@ def test_lit_uneven_batch_matches_the_oracle():
                 d = d.any(-1)
             assert not d.any(), f"scene {s} {name}: {int(d.sum())} pixels differ"
     r.close()
+
+
+def test_ambient_factor_and_no_textures_flag():
+    """setAmbientFactor (Renderer.h:107) scales the shaded materials' ambient
+    term; NoTextures drops texture sampling from both terms."""
+    from habitat_sim_amd import _native as NN
+    spec = _shaded(scenes.apartments(n_envs=2))
+    gx, gy = spec.grid
+    for factor, flags in ((0.35, 0), (0.1, NN.FLAG_NO_TEXTURES)):
+        cfg = (RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy))
+               .set_max_light_count(MAX_LIGHTS).set_ambient_factor(factor).set_flags(flags))
+        r = RendererStandalone(cfg)
+        o = OracleRenderer(spec.tile, (gx, gy), flags, max_light_count=MAX_LIGHTS,
+                           ambient_factor=factor)
+        for x in (r, o):
+            scenes.populate(x, spec)
+            _add_lights(x, spec, seed=5)
+        r.draw()
+        o.draw(NTHREADS)
+        _same(r, o, f"factor {factor} flags {flags}")
+        r.close()
