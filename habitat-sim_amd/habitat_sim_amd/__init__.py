@@ -5,5 +5,7 @@ from .renderer import (RendererConfiguration, RendererStandalone,  # noqa: F401
                        calculate_depth_unprojection, unproject_depth)
 from .replay import (BatchReplayRenderer, CameraSensorSpec,  # noqa: F401
                      ReplayRendererConfiguration, environment_grid_size)
+from .noise import RedwoodDepthNoiseModel  # noqa: F401
+from .replay_player import BatchPlayer, Keyframe  # noqa: F401
 
 __version__ = "0.1.0"
