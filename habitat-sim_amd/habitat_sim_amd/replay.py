@@ -19,6 +19,7 @@ colour (H, W, 4) uint8, depth (H, W) float32, semantic (H, W) uint32.
 """
 from __future__ import annotations
 
+import json
 import math
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
@@ -27,6 +28,7 @@ import numpy as np
 
 from . import _native as N
 from . import math3d as m3
+from .replay_player import BatchPlayer, Keyframe, transform_matrix
 from .renderer import RendererConfiguration, RendererStandalone, RendererStandaloneConfiguration
 
 COLOR, DEPTH, SEMANTIC = "color", "depth", "semantic"
@@ -109,6 +111,7 @@ class BatchReplayRenderer:
             self.renderer.set_tuning(cull_mode=cfg.cull_mode)
         self._projection = cfg.sensor_specifications[0].projection()
         self._pose = [m3.identity() for _ in range(cfg.num_environments)]
+        self._players: Dict[int, BatchPlayer] = {}
         for e in range(cfg.num_environments):
             self.renderer.update_camera(e, self._projection, m3.identity())
 
@@ -141,9 +144,63 @@ class BatchReplayRenderer:
         t[node] = np.asarray(transform, np.float32)
 
     def clear_environment(self, env: int) -> None:
-        """BatchPlayerImplementation::clearAssetInstances -> Renderer::clear."""
+        """AbstractReplayRenderer::clearEnvironment (AbstractReplayRenderer.cpp:
+        54-58): closes the environment's player, then Renderer::clear."""
         self._check_env(env)
+        player = self._players.get(env)
+        if player is not None:
+            player.close()
         self.renderer.clear(env)
+
+    # ---- gfx-replay keyframes -----------------------------------------------
+    def player_for(self, env: int) -> BatchPlayer:
+        """The environment's keyframe player (BatchReplayRenderer::doPlayerFor),
+        made on first use."""
+        self._check_env(env)
+        player = self._players.get(env)
+        if player is None:
+            player = self._players[env] = BatchPlayer(self.renderer, env)
+        return player
+
+    def set_environment_keyframe(self, env: int, ser_keyframe: str) -> None:
+        """AbstractReplayRenderer::setEnvironmentKeyframe (:60-66): a JSON
+        document whose "keyframe" member is the keyframe
+        (Player::keyframeFromString, Player.cpp:105-111); replaces the
+        player's keyframes with it and applies it on top of the current
+        frame (Player::setSingleKeyframe, Player.cpp:361-366)."""
+        doc = json.loads(ser_keyframe)
+        if not isinstance(doc, dict) or "keyframe" not in doc:
+            raise ValueError('set_environment_keyframe: no "keyframe" member')
+        self.player_for(env).set_single_keyframe(Keyframe.from_json(doc["keyframe"]))
+
+    def set_environment_keyframe_unwrapped(self, env: int, ser_keyframe: str) -> None:
+        """AbstractReplayRenderer::setEnvironmentKeyframeUnwrapped (:68-74):
+        the document is the keyframe itself (keyframeFromStringUnwrapped,
+        Player.cpp:113-120)."""
+        self.player_for(env).set_single_keyframe(Keyframe.from_json(ser_keyframe))
+
+    def set_sensor_transforms_from_keyframe(self, env: int, prefix: str) -> None:
+        """AbstractReplayRenderer::setSensorTransformsFromKeyframe (:83-92) and
+        BatchReplayRenderer::doSetSensorTransformsFromKeyframe
+        (BatchReplayRenderer.cpp:100-114): the camera pose is the keyframe's
+        user transform named prefix + sensor uuid.  The sensors of one
+        environment share its camera, so the first sensor (in specification
+        order) whose name is in the keyframe gives it."""
+        self._check_env(env)
+        player = self._players.get(env)
+        if player is None or player.num_keyframes != 1:
+            raise RuntimeError(f"set_sensor_transforms_from_keyframe: for environment {env}, "
+                               "you have not yet called set_environment_keyframe.")
+        for spec in self.cfg.sensor_specifications:
+            found = player.get_user_transform(prefix + spec.uuid)
+            if found is not None:
+                translation, rotation = found
+                self.set_sensor_transform(env, spec.uuid,
+                                          transform_matrix(translation, rotation))
+                return
+        first = prefix + self.cfg.sensor_specifications[0].uuid
+        raise KeyError(f'set_sensor_transforms_from_keyframe: couldn\'t find user transform '
+                       f'"{first}" for environment {env}.')
 
     def set_sensor_transform(self, env: int, sensor_uuid: str, transform) -> None:
         """Camera-to-world pose of the environment's sensors

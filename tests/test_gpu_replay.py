@@ -123,3 +123,75 @@ def test_keyframe_lights_on_the_gpu_match_oracle():
         images.append(r.color_image())
     assert not np.array_equal(images[0], images[1]) and not np.array_equal(images[1], images[2])
     r.close()
+
+
+def test_environment_keyframes_and_sensor_transforms_from_keyframe():
+    """setEnvironmentKeyframe (wrapped in {"keyframe": ...}),
+    setEnvironmentKeyframeUnwrapped, setSensorTransformsFromKeyframe and
+    clearEnvironment of AbstractReplayRenderer: the pixels of each step equal
+    the oracle's, driven by one player per environment with the camera taken
+    from the same user transform."""
+    import json
+    from habitat_sim_amd.replay_player import BatchPlayer, transform_matrix
+    from test_replay_player import deletion_keyframes
+    S = replay.CameraSensorSpec
+    cfg = replay.ReplayRendererConfiguration(
+        2, [S("rgba", "color", (64, 64), hfov=70.0), S("depth", "depth", (64, 64), hfov=70.0)])
+    rr = replay.BatchReplayRenderer(cfg)
+    o = OracleRenderer((64, 64), rr.grid)
+    proj = cfg.sensor_specifications[0].projection()
+    half = math.radians(8.0)
+    cams = {0: ("sensor_rgba", [0.5, 0.5, 6.0], [math.cos(half), 0.0, math.sin(half), 0.0]),
+            # only the depth sensor's name: the first sensor that is found gives the pose
+            1: ("agent_depth", [0.2, 0.8, 7.0], [math.cos(half), math.sin(half), 0.0, 0.0])}
+    k0, k1, _ = (json.loads(k) for k in deletion_keyframes())
+    players = [BatchPlayer(o, e) for e in range(2)]
+
+    def frame(env, k):
+        name, t, q = cams[env]
+        return dict(k, userTransforms=[{"name": name, "transform": {"translation": t,
+                                                                    "rotation": q}}])
+
+    def check(step):
+        for e in range(2):
+            name, t, q = cams[e]
+            o.update_camera(e, proj, m3.inverted(transform_matrix(t, q)))
+        colors = [np.zeros((64, 64, 4), np.uint8) for _ in range(2)]
+        depths = [np.zeros((64, 64), np.float32) for _ in range(2)]
+        rr.render(colors, depths)
+        planes = o.render(scenes=[0, 1])
+        for e in range(2):
+            np.testing.assert_array_equal(colors[e], planes["color"][e], err_msg=f"{step} env {e}")
+            np.testing.assert_array_equal(depths[e], planes["lindepth"][e],
+                                          err_msg=f"{step} env {e}")
+        return colors
+
+    with pytest.raises(RuntimeError, match="set_environment_keyframe"):
+        rr.set_sensor_transforms_from_keyframe(0, "sensor_")
+    with pytest.raises(ValueError, match="keyframe"):
+        rr.set_environment_keyframe(0, json.dumps(frame(0, k0)))
+    rr.set_environment_keyframe(0, json.dumps({"keyframe": frame(0, k0)}))
+    rr.set_environment_keyframe_unwrapped(1, json.dumps(frame(1, k0)))
+    players[0].set_single_keyframe(frame(0, k0))
+    players[1].set_single_keyframe(frame(1, k0))
+    rr.set_sensor_transforms_from_keyframe(0, "sensor_")
+    rr.set_sensor_transforms_from_keyframe(1, "agent_")
+    with pytest.raises(KeyError, match="other_rgba"):
+        rr.set_sensor_transforms_from_keyframe(0, "other_")
+    first = check("first keyframe")
+    assert first[0].any() and first[1].any() and not np.array_equal(first[0], first[1])
+    # a second single keyframe applies on top of the first (X deleted, Z made and deleted)
+    rr.set_environment_keyframe(0, json.dumps({"keyframe": frame(0, k1)}))
+    players[0].set_single_keyframe(frame(0, k1))
+    rr.set_sensor_transforms_from_keyframe(0, "sensor_")
+    second = check("second keyframe")
+    assert not np.array_equal(first[0], second[0])
+    np.testing.assert_array_equal(first[1], second[1])
+    # clearEnvironment closes the player: environment 0 renders empty
+    rr.clear_environment(0)
+    players[0].close()
+    cleared = check("cleared")
+    assert (cleared[0] == cleared[0][0, 0]).all() and not np.array_equal(cleared[0], first[0])
+    with pytest.raises(RuntimeError):
+        rr.set_sensor_transforms_from_keyframe(0, "sensor_")
+    rr.close()
