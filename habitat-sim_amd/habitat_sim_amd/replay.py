@@ -311,6 +311,7 @@ class BatchReplayRenderer:
         return self.renderer.plane_device_pointer(_PLANES[self.sensors[sensor_uuid].sensor_type])
 
     def close(self) -> None:
+        self._players.clear()
         self.renderer.close()
 
     def __enter__(self):
