@@ -688,8 +688,13 @@ struct StageTimer {
 
 gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p);
 
+// The executable graph owns the kernel-argument memory of its launches; the
+// last launch may still be running on the stream, so it finishes first.
 void release_graph(gbr_renderer* r) {
-  if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+  if (r->graph_exec) {
+    (void)hipStreamSynchronize(r->stream);
+    (void)hipGraphExecDestroy(r->graph_exec);
+  }
   if (r->graph) (void)hipGraphDestroy(r->graph);
   r->graph_exec = nullptr;
   r->graph = nullptr;
@@ -801,13 +806,19 @@ gbr_status do_draw(gbr_renderer* r) {
   // HIP graph and replayed while the parameters stay the same
   const bool graphable = r->tun.graph && !r->tun.timing && !r->collecting;
   if (!graphable) return enqueue_frame(r, p);
-  std::vector<uint8_t> key(sizeof p + 8 * sizeof(uint64_t));
+  // the key holds every value a captured launch takes as an argument: the
+  // frame parameters, the node buffers, and the scorer's weight buffers
+  constexpr int kExtra = 12;
+  std::vector<uint8_t> key(sizeof p + kExtra * sizeof(uint64_t));
   {
-    const uint64_t extra[8] = {(uint64_t)(uintptr_t)r->d_parents.ptr, (uint64_t)(uintptr_t)r->d_local.ptr,
-                               (uint64_t)(uintptr_t)r->d_abs.ptr, (uint64_t)(uintptr_t)r->d_ritems.ptr,
-                               ((uint64_t)r->n_nodes << 32) | r->n_ritems,
-                               ((uint64_t)r->cu_count << 32) | (uint32_t)r->setup_bpc,
-                               (uint64_t)(uintptr_t)r->vis.Xi.ptr, (uint64_t)(uintptr_t)r->vis.logits_i.ptr};
+    const uint64_t extra[kExtra] = {
+        (uint64_t)(uintptr_t)r->d_parents.ptr, (uint64_t)(uintptr_t)r->d_local.ptr,
+        (uint64_t)(uintptr_t)r->d_abs.ptr, (uint64_t)(uintptr_t)r->d_ritems.ptr,
+        ((uint64_t)r->n_nodes << 32) | r->n_ritems,
+        ((uint64_t)r->cu_count << 32) | (uint32_t)r->setup_bpc,
+        (uint64_t)(uintptr_t)r->vis.Xi.ptr, (uint64_t)(uintptr_t)r->vis.logits_i.ptr,
+        (uint64_t)(uintptr_t)r->vis.w.ptr, (uint64_t)(uintptr_t)r->vis.W1b.ptr,
+        (uint64_t)(uintptr_t)r->vis.W2b.ptr, (uint64_t)(uintptr_t)r->vis.w3b.ptr};
     std::memcpy(key.data(), &p, sizeof p);
     std::memcpy(key.data() + sizeof p, extra, sizeof extra);
   }

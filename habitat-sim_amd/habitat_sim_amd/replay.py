@@ -46,6 +46,32 @@ def environment_grid_size(num_environments: int) -> tuple:
 
 
 @dataclass
+class Ray:
+    """esp::geo::Ray (src/esp/geo/Geo.h:300-310): origin and direction."""
+    origin: np.ndarray
+    direction: np.ndarray
+
+
+def _capsule(address: int):
+    """A PyCapsule with no name around a device address, as pybind11's
+    py::capsule(const void*) makes for the reference's pointer getters."""
+    import ctypes
+    new = ctypes.pythonapi.PyCapsule_New
+    new.restype = ctypes.py_object
+    new.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+    return new(ctypes.c_void_p(address), None, None)
+
+
+def capsule_address(capsule) -> int:
+    """The address a device-pointer capsule holds."""
+    import ctypes
+    get = ctypes.pythonapi.PyCapsule_GetPointer
+    get.restype = ctypes.c_void_p
+    get.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    return get(capsule, None) or 0
+
+
+@dataclass
 class CameraSensorSpec:
     """The fields of habitat's CameraSensorSpec the batch path uses
     (src/esp/sensor/CameraSensor.cpp:45-66, VisualSensor.h:52-89)."""
@@ -112,13 +138,36 @@ class BatchReplayRenderer:
         self._projection = cfg.sensor_specifications[0].projection()
         self._pose = [m3.identity() for _ in range(cfg.num_environments)]
         self._players: Dict[int, BatchPlayer] = {}
+        self._depth_fb = None
         for e in range(cfg.num_environments):
             self.renderer.update_camera(e, self._projection, m3.identity())
 
     # ---- environments -------------------------------------------------------
+    @staticmethod
+    def create_batch_replay_renderer(cfg: ReplayRendererConfiguration) -> "BatchReplayRenderer":
+        """ReplayRenderer.create_batch_replay_renderer(cfg) (SimBindings.cpp:
+        478-483): the factory habitat-lab calls."""
+        return BatchReplayRenderer(cfg)
+
     @property
     def num_environments(self) -> int:
         return self.cfg.num_environments
+
+    @property
+    def environment_count(self) -> int:
+        """ReplayRenderer.environment_count (SimBindings.cpp:493-495,
+        AbstractReplayRenderer::environmentCount)."""
+        return self.cfg.num_environments
+
+    def sensor_size(self, env: int) -> tuple:
+        """ReplayRenderer.sensor_size(env) (SimBindings.cpp:496-497): the
+        tile size (width, height), the same for every environment
+        (BatchReplayRenderer::doSensorSize, BatchReplayRenderer.cpp:82-86).
+        The environment index is checked as AbstractReplayRenderer::sensorSize
+        asserts it (AbstractReplayRenderer.cpp:49-52)."""
+        self._check_env(env)
+        h, w = self.resolution
+        return (w, h)
 
     @staticmethod
     def environment_grid_size(num_environments: int) -> tuple:
@@ -304,6 +353,57 @@ class BatchReplayRenderer:
         hip.memcpy_d2h(out, ptr)
         return out
 
+    def cuda_color_buffer_device_pointer(self):
+        """ReplayRenderer.cuda_color_buffer_device_pointer() (SimBindings.cpp:
+        531-536): a capsule holding the device address of the colour
+        framebuffer (RGBA8, tile grid, rows bottom-up, the layout of
+        RendererStandalone::colorCudaBufferDevicePointer), valid until the
+        next render()/draw()."""
+        return _capsule(self.renderer.color_device_pointer())
+
+    def cuda_depth_buffer_device_pointer(self):
+        """ReplayRenderer.cuda_depth_buffer_device_pointer() (SimBindings.cpp:
+        537-542): a capsule holding the device address of the depth
+        framebuffer (float32, tile grid, rows bottom-up), valid until the
+        next render()/draw().  The reference binding returns the colour
+        pointer here by mistake; this returns depth.  The values are the
+        depth sensor's: linear metres with far -> 0 (what render() writes),
+        the window depth when the renderer keeps no linear plane."""
+        if DEPTH not in self._kinds():
+            return _capsule(self.renderer.depth_device_pointer())
+        w, h = self.renderer.framebuffer_size
+        if self._depth_fb is None:
+            from . import hip
+            self._depth_fb = hip.DeviceBuffer(w * h * 4)
+        rect = np.asarray((0, 0, w, h), np.int32)
+        N.call("gbr_linear_depth_image_into", self.renderer._h, rect.ctypes.data,
+               self._depth_fb.value, w * 4, N.MEMORY_DEVICE)
+        return _capsule(self._depth_fb.value)
+
+    def unproject(self, env: int, viewport_position) -> Ray:
+        """ReplayRenderer.unproject(env, viewport_position) (SimBindings.cpp:
+        544-545, AbstractReplayRenderer::unproject :137-141): the world-space
+        ray from the environment's camera through a viewport pixel given in
+        top-down window coordinates.  The batch reference returns a
+        placeholder here (BatchReplayRenderer::doUnproject,
+        BatchReplayRenderer.cpp:161-171); this follows the classic renderer's
+        RenderCamera::unproject (RenderCamera.cpp:186-216): origin = camera
+        position, direction = (inverse(P) of the pixel's far-plane NDC point,
+        through the camera pose) - origin, normalised."""
+        self._check_env(env)
+        h, w = self.resolution
+        x, y = int(viewport_position[0]), int(viewport_position[1])
+        ndc = np.array([2.0 * x / w - 1.0, 2.0 * (h - y - 1) / h - 1.0, 1.0, 1.0])
+        pose = np.asarray(self._pose[env], np.float64)
+        inv_p = np.linalg.inv(np.asarray(self._projection, np.float64).T)
+        q = inv_p @ ndc
+        q = q[:3] / q[3]
+        world = (pose.T @ np.append(q, 1.0))[:3]
+        origin = pose[3, :3].astype(np.float64)
+        d = world - origin
+        n = np.linalg.norm(d)
+        return Ray(origin.astype(np.float32), (d / n if n > 0 else d).astype(np.float32))
+
     def plane_device_pointer(self, sensor_uuid: str) -> int:
         """Device address of a sensor's env-major plane [env][H][W] (rows
         top-down), valid until the next draw() (the analogue of
@@ -312,6 +412,9 @@ class BatchReplayRenderer:
 
     def close(self) -> None:
         self._players.clear()
+        if self._depth_fb is not None:
+            self._depth_fb.close()
+            self._depth_fb = None
         self.renderer.close()
 
     def __enter__(self):

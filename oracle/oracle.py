@@ -61,7 +61,9 @@ class _Job(ctypes.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "_build", "liborc.so")
+        # ORC_LIB names another build of raster_ref.c (the planted-mistake
+        # checks of tests/test_oracle_mutants.py load mutated copies)
+        path = os.environ.get("ORC_LIB") or os.path.join(_HERE, "_build", "liborc.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
         _LIB = ctypes.CDLL(path)

@@ -38,3 +38,37 @@ def test_autotune_rejects_bad_parameters():
     with pytest.raises(Exception):
         r.autotune(frames=0)
     r.close()
+
+
+def test_autotune_learned_mode_with_graph_replay():
+    """Learned culling with graph replay on: every autotune candidate
+    re-captures the frame graph (bin size and capacity changes rebuild the
+    frame layout, so the captured buffers move), then new weights are set
+    between replays.  Regression test for the graph lifetime: an executable
+    graph is destroyed only after its last launch has finished, and the
+    graph key covers the scorer's weight buffers."""
+    from habitat_sim_amd import visibility
+
+    spec = scenes.apartments(n_envs=16, tile=64, instances=40)
+    gx, gy = spec.grid
+    cfg = RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy))
+    plain = RendererStandalone(cfg)
+    scenes.populate(plain, spec)
+    plain.draw()
+    ref = {k: getattr(plain, f"{k}_image")() for k in ("color", "depth", "semantic")}
+    r = RendererStandalone(cfg)
+    scenes.populate(r, spec)
+    visibility.train(r, spec, frames=4, steps=20, seed=3, walk_length=2, batch=1024)
+    r.set_tuning(cull_mode=N.CULL_LEARNED, graph=1)
+    rep = r.autotune(warmup=1, frames=2)
+    assert rep["candidates"] >= 10
+    t = r.tuning()
+    assert t.cull_mode == N.CULL_LEARNED and t.graph == 1
+    w = r.vis_weights()
+    for k, scale in enumerate((1.0, 0.5, -1.0)):
+        r.set_vis_weights(w * scale)
+        r.draw()
+        for name, v in ref.items():
+            assert np.array_equal(getattr(r, f"{name}_image")(), v), (k, name)
+    r.close()
+    plain.close()

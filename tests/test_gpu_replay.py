@@ -195,3 +195,65 @@ def test_environment_keyframes_and_sensor_transforms_from_keyframe():
     with pytest.raises(RuntimeError):
         rr.set_sensor_transforms_from_keyframe(0, "sensor_")
     rr.close()
+
+
+def test_reference_binding_names():
+    """The names habitat-lab calls on ReplayRenderer (SimBindings.cpp:
+    478-545): create_batch_replay_renderer, environment_count, sensor_size,
+    cuda_{color,depth}_buffer_device_pointer (capsules of framebuffer
+    addresses; depth is depth, not the reference binding's colour pointer)
+    and unproject."""
+    from habitat_sim_amd import hip
+
+    spec = scenes.small_rooms(n_envs=3)
+    S = replay.CameraSensorSpec
+    cfg = replay.ReplayRendererConfiguration(
+        3, [S("rgba", "color", (96, 128)), S("depth", "depth", (96, 128))])
+    rr = replay.BatchReplayRenderer.create_batch_replay_renderer(cfg)
+    assert rr.environment_count == 3
+    assert rr.sensor_size(2) == (128, 96)  # (width, height)
+    with pytest.raises(IndexError):
+        rr.sensor_size(3)
+    assert rr.add_file_data(spec.file)
+    for e, env in enumerate(spec.envs):
+        for inst in env.instances:
+            rr.add_instance(e, inst.template, inst.bake, inst.semantic)
+    rr.set_sensor_transforms(np.stack([env.camera_pose for env in spec.envs]))
+    colors = [np.zeros((96, 128, 4), np.uint8) for _ in range(3)]
+    depths = [np.zeros((96, 128), np.float32) for _ in range(3)]
+    rr.render(colors, depths)
+    gx, gy = rr.grid
+    fw, fh = 128 * gx, 96 * gy
+    c_ptr = replay.capsule_address(rr.cuda_color_buffer_device_pointer())
+    d_ptr = replay.capsule_address(rr.cuda_depth_buffer_device_pointer())
+    assert c_ptr and d_ptr and c_ptr != d_ptr
+    fb_c = np.empty((fh, fw, 4), np.uint8)
+    fb_d = np.empty((fh, fw), np.float32)
+    hip.memcpy_d2h(fb_c, c_ptr)
+    hip.memcpy_d2h(fb_d, d_ptr)
+    for e in range(3):
+        x0, y0 = (e % gx) * 128, (e // gx) * 96
+        np.testing.assert_array_equal(fb_c[y0:y0 + 96, x0:x0 + 128], colors[e])
+        np.testing.assert_array_equal(fb_d[y0:y0 + 96, x0:x0 + 128], depths[e])
+    # unproject: the ray through the far-plane point of a pixel's lower-left
+    # corner (RenderCamera.cpp:191-197 maps the pixel index, not its
+    # centre), from the camera position; projected back it lands on that
+    # corner, and the middle pixel's ray is the camera's forward axis
+    proj = S("x", resolution=(96, 128)).projection()
+    for e, env in enumerate(spec.envs):
+        pose = env.camera_pose
+        view = m3.inverted(pose)
+        for (px, py) in ((0, 0), (64, 48), (127, 95), (17, 80)):
+            ray = rr.unproject(e, (px, py))
+            np.testing.assert_allclose(ray.origin, pose[3, :3], atol=1e-6)
+            assert abs(np.linalg.norm(ray.direction) - 1.0) < 1e-5
+            p = np.append(ray.origin + 5.0 * ray.direction, 1.0)
+            clip = proj.T.astype(np.float64) @ (view.T.astype(np.float64) @ p)
+            ndc = clip[:2] / clip[3]
+            assert abs((ndc[0] + 1) * 128 / 2 - px) < 1e-3
+            assert abs((ndc[1] + 1) * 96 / 2 - (96 - py - 1)) < 1e-3
+        mid = rr.unproject(e, (64, 47)).direction
+        assert np.dot(mid, -pose[2, :3]) > 0.999
+    with pytest.raises(IndexError):
+        rr.unproject(3, (0, 0))
+    rr.close()
