@@ -202,6 +202,8 @@ struct alignas(16) ItemCull {
 
 // culling modes (same values as GBR_CULL_*)
 enum : int32_t { kCullNone = 0, kCullFrustum = 1, kCullHiZ = 2, kCullLearned = 3 };
+// visibility-model feature sets (same values as GBR_VIS_FEATURES_*)
+enum : int32_t { kFeaturesPose = 0, kFeaturesHistory = 1 };
 
 // depth pyramid: level 0 = max depth per 8x8 pixels, level 1 per 32x32
 constexpr int kHiZ0 = 3, kHiZ1 = 5;  // log2 block edges
@@ -315,6 +317,7 @@ struct FrameParams {
   uint16_t* features;      // null: no feature rows this frame
   uint32_t* prop;
   const float* logits;     // model output per feature row (learned mode)
+  int32_t feature_set;     // kFeatures*
   // outputs (env-major planes), any may be null
   uint32_t* out_color;
   float* out_depth;

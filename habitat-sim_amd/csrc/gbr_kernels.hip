@@ -2228,7 +2228,8 @@ __device__ __forceinline__ uint32_t f2bf_bits(float f) {
 // use the hardware log2 and reciprocal (~1 ulp)
 __device__ __forceinline__ float log2c(float x) { return __builtin_amdgcn_logf(fmaxf(x, 1e-3f)); }
 
-// The 32 features of a (camera, work item) pair for the visibility model:
+// The 32 features of a (camera, work item) pair for the visibility model,
+// history feature set (an ablation of the pose set below):
 //  0-2   visibility history: visible last frame, within 8 frames, age/64
 //  3-7   conservative pixel rectangle (normalised) and its area fraction
 //  8     log2 of the nearest linear depth / 8
@@ -2238,7 +2239,7 @@ __device__ __forceinline__ float log2c(float x) { return __builtin_amdgcn_logf(f
 //  25-28 centre minus camera / 8 and its length / 8
 //  29    cosine between forward and the direction to the centre
 //  30    rectangle touches the tile border, 31 constant 1
-__device__ void item_features(const FrameParams& p, const WorkItem& wi, const SceneDesc& sc,
+__device__ void item_features_history(const FrameParams& p, const WorkItem& wi, const SceneDesc& sc,
                               const ItemBounds& bb, const float* M, const float* PV,
                               const ItemCull& ic, float wmin, float wmax, uint32_t item,
                               uint4* row) {
@@ -2285,6 +2286,95 @@ __device__ void item_features(const FrameParams& p, const WorkItem& wi, const Sc
   f[28] = dd / 8.0f;
   f[29] = dd > 0.0f ? cosf_ * __builtin_amdgcn_rcpf(dd) : 1.0f;
   f[30] = (ic.x0 == 0 || ic.y0 == 0 || ic.x1 == sc.width - 1 || ic.y1 == sc.height - 1) ? 1.0f : 0.0f;
+  f[31] = 1.0f;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint4 v;
+    v.x = f2bf_bits(f[8 * q + 0]) | (f2bf_bits(f[8 * q + 1]) << 16);
+    v.y = f2bf_bits(f[8 * q + 2]) | (f2bf_bits(f[8 * q + 3]) << 16);
+    v.z = f2bf_bits(f[8 * q + 4]) | (f2bf_bits(f[8 * q + 5]) << 16);
+    v.w = f2bf_bits(f[8 * q + 6]) | (f2bf_bits(f[8 * q + 7]) << 16);
+    row[q] = v;
+  }
+}
+
+// The 32 features of a (camera, work item) pair, pose feature set: the
+// camera pose and the chunk's world box relative to the camera, nothing
+// from earlier frames.
+//  0-2   camera position / 8
+//  3-4   sin, cos of the camera yaw; 5-6 sin, cos of its pitch
+//  7-9   box centre - camera position / 8
+//  10-12 box extents (world axis-aligned) / 4
+//  13-15 centre - camera in the camera frame (right, up, forward) / 8
+//  16    distance to the centre / 8, 17 its log2 / 4
+//  18    cosine between the view direction and the direction to the centre
+//  19-20 centre direction in the camera frame: right/forward, up/forward
+//        (clamped to +-4)
+//  21    box half-diagonal / distance (angular size, clamped to 4)
+//  22-24 extents along the camera's right, up and forward axes / 4
+//  25-27 box corner nearest the camera - camera position / 8 (per axis)
+//  28-30 farthest corner - camera position / 8, 31 constant 1
+__device__ void item_features_pose(const FrameParams& p, const WorkItem& wi,
+                                   const ItemBounds& bb, const float* M, uint4* row) {
+  float f[32];
+  const float* cam = p.cam_info + (size_t)wi.scene * 8;
+  const float fx = cam[3], fy = cam[4], fz = cam[5];
+  // yaw about +y with the camera looking down -z at yaw 0; pitch up positive
+  const float hz = __builtin_amdgcn_sqrtf(fx * fx + fz * fz);
+  const float ih = hz > 1e-6f ? __builtin_amdgcn_rcpf(hz) : 0.0f;
+  const float syaw = hz > 1e-6f ? -fx * ih : 0.0f, cyaw = hz > 1e-6f ? -fz * ih : 1.0f;
+  // camera frame: right = forward x world up (normalised), up = right x forward
+  const float rx = -fz * ih, rz = fx * ih;  // (ry = 0)
+  const float ux = -rz * fy, uy = rz * fx - rx * fz, uz = rx * fy;
+  const float cx = 0.5f * (bb.lo[0] + bb.hi[0]), cy = 0.5f * (bb.lo[1] + bb.hi[1]),
+              cz = 0.5f * (bb.lo[2] + bb.hi[2]);
+  const float ex = bb.hi[0] - bb.lo[0], ey = bb.hi[1] - bb.lo[1], ez = bb.hi[2] - bb.lo[2];
+  float wc[4];
+  mat4_vec(M, cx, cy, cz, 1.0f, wc);
+  float we[3];  // world axis-aligned extents of the transformed box
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    we[k] = fabsf(M[k]) * ex + fabsf(M[4 + k]) * ey + fabsf(M[8 + k]) * ez;
+  float d[3], lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    d[k] = wc[k] - cam[k];
+    const float a = d[k] - 0.5f * we[k], b = d[k] + 0.5f * we[k];
+    // per axis, the box coordinate nearest to / farthest from the camera
+    lo[k] = (a > 0.0f) ? a : (b < 0.0f ? b : 0.0f);
+    hi[k] = fabsf(a) > fabsf(b) ? a : b;
+  }
+  const float dr = d[0] * rx + d[2] * rz;
+  const float du = d[0] * ux + d[1] * uy + d[2] * uz;
+  const float df = d[0] * fx + d[1] * fy + d[2] * fz;
+  const float dist = __builtin_amdgcn_sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const float idist = dist > 1e-6f ? __builtin_amdgcn_rcpf(dist) : 0.0f;
+  const float half_diag = 0.5f * __builtin_amdgcn_sqrtf(we[0] * we[0] + we[1] * we[1] + we[2] * we[2]);
+  const float idf = fabsf(df) > 1e-6f ? __builtin_amdgcn_rcpf(df) : 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f[k] = cam[k] * 0.125f;
+    f[7 + k] = d[k] * 0.125f;
+    f[10 + k] = we[k] * 0.25f;
+    f[25 + k] = lo[k] * 0.125f;
+    f[28 + k] = hi[k] * 0.125f;
+  }
+  f[3] = syaw;
+  f[4] = cyaw;
+  f[5] = fy;  // sin(pitch) of a unit forward vector
+  f[6] = hz;  // cos(pitch)
+  f[13] = dr * 0.125f;
+  f[14] = du * 0.125f;
+  f[15] = df * 0.125f;
+  f[16] = dist * 0.125f;
+  f[17] = log2c(dist) * 0.25f;
+  f[18] = dist > 1e-6f ? df * idist : 1.0f;
+  f[19] = fmaxf(fminf(dr * idf, 4.0f), -4.0f);
+  f[20] = fmaxf(fminf(du * idf, 4.0f), -4.0f);
+  f[21] = dist > 1e-6f ? fminf(half_diag * idist, 4.0f) : 4.0f;
+  f[22] = (fabsf(rx) * we[0] + fabsf(rz) * we[2]) * 0.25f;
+  f[23] = (fabsf(ux) * we[0] + fabsf(uy) * we[1] + fabsf(uz) * we[2]) * 0.25f;
+  f[24] = (fabsf(fx) * we[0] + fabsf(fy) * we[1] + fabsf(fz) * we[2]) * 0.25f;
   f[31] = 1.0f;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -2365,7 +2455,10 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
           ic.pad[1] = 0u;
           if (p.features) {
             feat = true;
-            item_features(p, wi, sc, bb, M, PV, ic, wmin, wmax, item, frow);
+            if (p.feature_set == kFeaturesHistory)
+              item_features_history(p, wi, sc, bb, M, PV, ic, wmin, wmax, item, frow);
+            else
+              item_features_pose(p, wi, bb, M, frow);
           }
           if (p.cull_mode == kCullLearned) {
             p.cull[item] = ic;  // k_propose decides from the model's score

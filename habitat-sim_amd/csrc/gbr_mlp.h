@@ -7,7 +7,7 @@
 namespace gbr {
 
 constexpr int kMlpIn = 32;
-constexpr int kMlpHidden = 128;
+constexpr int kMlpHidden = 256;  // GBR_VIS_HIDDEN
 
 // C[M][N] = alpha * sum_k A[m][k] * Bt[n][k]  (bf16 in, fp32 accumulate)
 // rows M = min(M, *M_dev) when M_dev is set (a row count produced on the

@@ -1,4 +1,4 @@
-// Chunk-visibility MLP (32 -> 128 -> 128 -> 1, ReLU) on gfx950 MFMA:
+// Chunk-visibility MLP (32 -> 256 -> 256 -> 1, ReLU) on gfx950 MFMA:
 // BF16 operands, FP32 accumulation, FP32 master weights with Adam.
 //
 // Every matrix product is an "NT" GEMM, C[M][N] = sum_k A[M][K] * Bt[N][K],

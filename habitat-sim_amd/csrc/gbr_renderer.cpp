@@ -221,6 +221,7 @@ struct gbr_renderer {
   std::vector<uint8_t> graph_key;
   DevBuf<float> d_caminfo;
   VisModel vis;
+  int32_t vis_features = GBR_VIS_FEATURES_POSE;  // feature set of the rows and the model
   bool collecting = false;
   DevBuf<RasterItem> d_ritems;
   DevBuf<int32_t> d_parents;
@@ -770,6 +771,7 @@ gbr_status do_draw(gbr_renderer* r) {
     p.prop = r->vis.prop.ptr;
     p.logits = r->vis.logits_i.ptr;
   }
+  p.feature_set = r->vis_features;
   p.raster_mode = r->tun.raster_mode;
   p.lane_pixels = r->tun.lane_pixels;
   {
@@ -1448,6 +1450,20 @@ gbr_status gbr_add_node_hierarchy(gbr_renderer* r, uint32_t scene, const char* n
   return gbr_add_node_hierarchy_semantic(r, scene, name, bake, 0u, out_node);
 }
 
+gbr_status gbr_add_node_hierarchies(gbr_renderer* r, uint32_t scene, uint32_t count,
+                                    const char* const* names, const float* bakes,
+                                    const uint32_t* semantics, uint64_t* out_nodes) {
+  TRY(check_scene(r, scene, "addNodeHierarchy()"));
+  if (count && !names) return fail(GBR_ERR_ARGUMENT, "addNodeHierarchy(): null names");
+  for (uint32_t i = 0; i < count; ++i) {
+    uint64_t node = 0;
+    TRY(gbr_add_node_hierarchy_semantic(r, scene, names[i], bakes ? bakes + 16 * (size_t)i : nullptr,
+                                        semantics ? semantics[i] : 0u, &node));
+    if (out_nodes) out_nodes[i] = node;
+  }
+  return GBR_OK;
+}
+
 gbr_status gbr_set_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, uint64_t node,
                                            uint32_t semantic) {
   TRY(check_scene(r, scene, "setNodeHierarchySemantic()"));
@@ -1962,6 +1978,28 @@ gbr_status gbr_vis_collect(gbr_renderer* r, uint64_t* rows_added) {
   const uint64_t added = std::min<uint64_t>(n, r->vis.cap - r->vis.rows);
   r->vis.rows += added;
   if (rows_added) *rows_added = added;
+  return GBR_OK;
+}
+
+gbr_status gbr_vis_set_feature_set(gbr_renderer* r, int32_t feature_set) {
+  if (!r) return fail(GBR_ERR_ARGUMENT, "vis_set_feature_set: null renderer");
+  if (feature_set != GBR_VIS_FEATURES_POSE && feature_set != GBR_VIS_FEATURES_HISTORY)
+    return fail(GBR_ERR_ARGUMENT, "vis_set_feature_set: unknown feature set");
+  if (feature_set != r->vis_features) {
+    // rows of the other set would mix two meanings of every column
+    r->vis.rows = 0;
+    if (r->vis.positives.ptr) {
+      TRY(set_device(r));
+      HIP_TRY(hipMemsetAsync(r->vis.positives.ptr, 0, sizeof(uint32_t), r->stream));
+    }
+  }
+  r->vis_features = feature_set;
+  return GBR_OK;
+}
+
+gbr_status gbr_vis_feature_set(const gbr_renderer* r, int32_t* out) {
+  if (!r || !out) return fail(GBR_ERR_ARGUMENT, "vis_feature_set: null argument");
+  *out = r->vis_features;
   return GBR_OK;
 }
 

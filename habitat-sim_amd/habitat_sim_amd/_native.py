@@ -67,7 +67,9 @@ class AutotuneReport(ctypes.Structure):
 
 
 VIS_FEATURES = 32
-VIS_PARAMS = 20865
+VIS_PARAMS = 74497
+VIS_HIDDEN = 256
+VIS_FEATURES_POSE, VIS_FEATURES_HISTORY = 0, 1
 
 
 class VisFitParams(ctypes.Structure):
@@ -159,6 +161,8 @@ SIGNATURES = {
                                    ctypes.POINTER(ctypes.c_uint64)]),
     "gbr_add_node_hierarchy_semantic": (S, [R, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_void_p,
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    "gbr_add_node_hierarchies": (S, [R, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "gbr_add_empty_node": (S, [R, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     "gbr_add_light": (S, [R, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
                           ctypes.POINTER(ctypes.c_uint64)]),
@@ -208,6 +212,8 @@ SIGNATURES = {
     "gbr_vis_init": (S, [R, ctypes.c_uint64]),
     "gbr_vis_collect": (S, [R, ctypes.POINTER(ctypes.c_uint64)]),
     "gbr_vis_clear_rows": (S, [R]),
+    "gbr_vis_set_feature_set": (S, [R, ctypes.c_int32]),
+    "gbr_vis_feature_set": (S, [R, ctypes.POINTER(ctypes.c_int32)]),
     "gbr_vis_rows": (S, [R, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "gbr_vis_fit": (S, [R, ctypes.POINTER(VisFitParams), ctypes.POINTER(VisFitReport)]),
     "gbr_vis_get_weights": (S, [R, ctypes.c_void_p]),

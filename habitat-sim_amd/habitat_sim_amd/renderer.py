@@ -292,6 +292,21 @@ class RendererStandalone:
                None if b is None else b.ctypes.data, semantic_id, ctypes.byref(out))
         return out.value
 
+    def add_node_hierarchies(self, scene: int, names, bakes=None, semantic_ids=None) -> np.ndarray:
+        """add_node_hierarchy() for many instances of one scene in one call:
+        names[i], bakes (n, 4, 4) or None, semantic_ids (n,) or None; returns
+        the top-level node ids."""
+        n = len(names)
+        enc = (ctypes.c_char_p * max(n, 1))(*[x if isinstance(x, bytes) else x.encode()
+                                             for x in names])
+        b = None if bakes is None else np.ascontiguousarray(bakes, np.float32).reshape(n, 16)
+        sem = None if semantic_ids is None else np.ascontiguousarray(semantic_ids, np.uint32)
+        out = np.empty(max(n, 1), np.uint64)
+        N.call("gbr_add_node_hierarchies", self._h, scene, n, ctypes.cast(enc, ctypes.c_void_p),
+               None if b is None else b.ctypes.data, None if sem is None else sem.ctypes.data,
+               out.ctypes.data)
+        return out[:n]
+
     def set_node_hierarchy_semantic(self, scene: int, node: int, semantic_id: int) -> None:
         """Semantic id of every draw of the hierarchy under top-level `node`."""
         N.call("gbr_set_node_hierarchy_semantic", self._h, scene, node, semantic_id)
@@ -544,6 +559,18 @@ class RendererStandalone:
 
     def vis_clear_rows(self) -> None:
         N.call("gbr_vis_clear_rows", self._h)
+
+    def vis_set_feature_set(self, feature_set: int) -> None:
+        """VIS_FEATURES_POSE (default: camera pose and the chunk's box
+        relative to the camera) or VIS_FEATURES_HISTORY (ablation: earlier
+        frames' visibility and the projected rectangle); switching drops the
+        training rows."""
+        N.call("gbr_vis_set_feature_set", self._h, feature_set)
+
+    def vis_feature_set(self) -> int:
+        out = ctypes.c_int32()
+        N.call("gbr_vis_feature_set", self._h, ctypes.byref(out))
+        return out.value
 
     def vis_rows(self) -> tuple:
         """(training rows, rows labelled visible)."""

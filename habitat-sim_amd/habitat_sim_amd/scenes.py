@@ -63,6 +63,9 @@ class EnvSpec:
     projection: np.ndarray
     viewport: Optional[Tuple[int, int]] = None  # (w, h) inside the tile; None = the tile
     planes: int = 0              # sensor set (PLANE_* bits; 0 = every plane)
+    # the instances as arrays (template names, bakes (n, 4, 4), semantic ids),
+    # for adding them in one call where the renderer offers it
+    arrays: Optional[Tuple[list, np.ndarray, np.ndarray]] = None
 
 
 @dataclass
@@ -88,10 +91,14 @@ def populate(renderer, spec: BatchSpec, first_scene: int = 0, add_file: bool = T
     reference's method names (product or oracle)."""
     if add_file:
         assert renderer.add_file_data(spec.file), "add_file_data failed"
+    batched = hasattr(renderer, "add_node_hierarchies")
     for e, env in enumerate(spec.envs):
         s = first_scene + e
-        for inst in env.instances:
-            renderer.add_node_hierarchy(s, inst.template, inst.bake, inst.semantic)
+        if batched and env.arrays is not None:
+            renderer.add_node_hierarchies(s, *env.arrays)
+        else:
+            for inst in env.instances:
+                renderer.add_node_hierarchy(s, inst.template, inst.bake, inst.semantic)
         renderer.update_camera(s, env.projection, m3.inverted_rigid(env.camera_pose))
         if env.viewport is not None:
             renderer.set_scene_viewport(s, env.viewport)
@@ -397,6 +404,44 @@ def apartment_templates(config_index: int = 1, n_templates: int = 20, tex_size: 
     return fd, tris, weights
 
 
+_OBJ_NAMES = [f"obj{i}" for i in range(64)]
+
+
+def _scatter(rng: np.random.Generator, picks: np.ndarray):
+    """The instances of an apartment: the shell, then per pick a template
+    scaled by s ~ U[0.3, 1.2], turned by yaw ~ U[0, 2 pi) and standing on
+    the floor at (x, z) ~ U[0.5, 11.5]^2, drawn from the env's stream in the
+    order s, x, z, yaw per instance.  Bakes are the float32 product
+    translation * rotation_y * scaling written out (each entry one product,
+    as the 4x4 product gives it).  Returns (instances, arrays)."""
+    n = len(picks)
+    u = rng.random((n, 4))
+    span = ROOM * GRID - 0.5
+    s = 0.3 + (1.2 - 0.3) * u[:, 0]
+    x = 0.5 + (span - 0.5) * u[:, 1]
+    z = 0.5 + (span - 0.5) * u[:, 2]
+    yaw = 0.0 + (2 * math.pi - 0.0) * u[:, 3]
+    c = np.array([math.cos(v) for v in yaw], np.float32)
+    sn = np.array([math.sin(v) for v in yaw], np.float32)
+    s32 = s.astype(np.float32)
+    bakes = np.zeros((n + 1, 4, 4), np.float32)
+    bakes[0] = m3.identity()
+    b = bakes[1:]
+    b[:, 0, 0] = c * s32
+    b[:, 0, 2] = -sn * s32
+    b[:, 1, 1] = s32
+    b[:, 2, 0] = sn * s32
+    b[:, 2, 2] = c * s32
+    b[:, 3, 0] = x.astype(np.float32)
+    b[:, 3, 1] = (0.5 * s).astype(np.float32)
+    b[:, 3, 2] = z.astype(np.float32)
+    b[:, 3, 3] = 1.0
+    names = ["apartment"] + [_OBJ_NAMES[int(k)] for k in picks]
+    sems = np.arange(1, n + 2, dtype=np.uint32)
+    insts = [Instance(names[i], bakes[i], int(sems[i])) for i in range(n + 1)]
+    return insts, (names, bakes, sems)
+
+
 def apartments(n_envs: int = 512, tile: int = 128, instances: int = 300,
                config_index: int = 1, template_file=None) -> BatchSpec:
     fd, tris, weights = template_file or apartment_templates(config_index)
@@ -405,20 +450,12 @@ def apartments(n_envs: int = 512, tile: int = 128, instances: int = 300,
     envs = []
     for e in range(n_envs):
         rng = env_rng(config_index, e)
-        insts = [Instance("apartment", m3.identity(), 1)]
         picks = rng.choice(n_t, size=instances, p=weights)
-        for k in range(instances):
-            s = rng.uniform(0.3, 1.2)
-            x = rng.uniform(0.5, ROOM * GRID - 0.5)
-            z = rng.uniform(0.5, ROOM * GRID - 0.5)
-            yaw = rng.uniform(0, 2 * math.pi)
-            bake = m3.mul(m3.mul(m3.translation((x, 0.5 * s, z)), m3.rotation_y(yaw)),
-                          m3.scaling(s))
-            insts.append(Instance(f"obj{int(picks[k])}", bake, 2 + k))
+        insts, arrays = _scatter(rng, picks)
         room = rng.integers(0, GRID, 2)
         lo = (room[0] * ROOM + 0.3, room[1] * ROOM + 0.3)
         hi = ((room[0] + 1) * ROOM - 0.3, (room[1] + 1) * ROOM - 0.3)
-        envs.append(EnvSpec(insts, random_pose_in_box(rng, lo, hi), P))
+        envs.append(EnvSpec(insts, random_pose_in_box(rng, lo, hi), P, arrays=arrays))
     return BatchSpec("apartments", (tile, tile), fd, envs, tris)
 
 
@@ -452,19 +489,11 @@ def heterogeneous(n_envs: int = 1024, tri_range=(1e4, 1e6), resolutions=(64, 128
         for c in chosen:
             planes |= sensor_sets[int(c)]
         P = m3.pinhole_projection((res, res), math.radians(90.0), 0.01, 1000.0)
-        insts = [Instance("apartment", m3.identity(), 1)]
         picks = rng.choice(n_t, size=count, p=weights)
-        for i in range(count):
-            s = rng.uniform(0.3, 1.2)
-            x = rng.uniform(0.5, ROOM * GRID - 0.5)
-            z = rng.uniform(0.5, ROOM * GRID - 0.5)
-            yaw = rng.uniform(0, 2 * math.pi)
-            bake = m3.mul(m3.mul(m3.translation((x, 0.5 * s, z)), m3.rotation_y(yaw)),
-                          m3.scaling(s))
-            insts.append(Instance(f"obj{int(picks[i])}", bake, 2 + i))
+        insts, arrays = _scatter(rng, picks)
         room = rng.integers(0, GRID, 2)
         lo = (room[0] * ROOM + 0.3, room[1] * ROOM + 0.3)
         hi = ((room[0] + 1) * ROOM - 0.3, (room[1] + 1) * ROOM - 0.3)
         vp = None if res == tile else (res, res)
-        envs.append(EnvSpec(insts, random_pose_in_box(rng, lo, hi), P, vp, planes))
+        envs.append(EnvSpec(insts, random_pose_in_box(rng, lo, hi), P, vp, planes, arrays))
     return BatchSpec("heterogeneous", (tile, tile), fd, envs, tris)

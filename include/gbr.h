@@ -264,6 +264,13 @@ gbr_status gbr_add_node_hierarchy(gbr_renderer* r, uint32_t scene, const char* n
 gbr_status gbr_add_node_hierarchy_semantic(gbr_renderer* r, uint32_t scene, const char* name,
                                            const float* bake, uint32_t semantic_id,
                                            uint64_t* out_node);
+/* new: addNodeHierarchy() `count` times into one scene, instance i from
+   names[i], bakes[16*i] (NULL: identity for all) and semantics[i] (NULL: 0);
+   top-level ids into out_nodes (may be NULL).  Stops at the first failure,
+   keeping the instances added before it. */
+gbr_status gbr_add_node_hierarchies(gbr_renderer* r, uint32_t scene, uint32_t count,
+                                    const char* const* names, const float* bakes,
+                                    const uint32_t* semantics, uint64_t* out_nodes);
 /* new: sets the semantic id of every draw of the hierarchy whose top-level
    node is `node` (a gfx-replay instance's metadata, which the reference's
    BatchPlayerImplementation drops, BatchPlayerImplementation.cpp:51-91) */
@@ -394,15 +401,25 @@ gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
                         gbr_autotune_report* report);
 /* ---- learned visibility proposal (new) -------------------------------------
    No counterpart in the reference, whose batch renderer submits every mesh
-   of every scene (Renderer.cpp:1298-1351).  A 32 -> 128 -> 128 -> 1 MLP
-   (BF16 MFMA, FP32 accumulation and master weights) scores each chunk that
-   passes the frustum test from per-(camera, chunk) features; with
-   GBR_CULL_LEARNED the chunks it scores above gbr_tuning.proposal_threshold
-   form phase 1 of the two-phase depth-pyramid cull.  The proposal only
-   orders work: output pixels never depend on it.  Training rows are
-   (features, chunk visible) pairs taken from frames the renderer draws. */
+   of every scene (Renderer.cpp:1298-1351); the nearest precedent is the
+   classic renderer's per-object frustum cull (RenderCamera.cpp:104-125).
+   A 32 -> 256 -> 256 -> 1 MLP (BF16 MFMA, FP32 accumulation and master
+   weights) scores each chunk that passes the frustum test from
+   per-(camera, chunk) features; with GBR_CULL_LEARNED the chunks it scores
+   above gbr_tuning.proposal_threshold form phase 1 of the two-phase
+   depth-pyramid cull.  The proposal only orders work: output pixels never
+   depend on it.  Training rows are (features, chunk visible) pairs taken
+   from frames the renderer draws. */
 #define GBR_VIS_FEATURES 32
-#define GBR_VIS_PARAMS 20865 /* W1[128][32] b1[128] W2[128][128] b2[128] w3[128] b3 */
+#define GBR_VIS_HIDDEN 256
+#define GBR_VIS_PARAMS 74497 /* W1[256][32] b1[256] W2[256][256] b2[256] w3[256] b3 */
+/* feature sets of the 32 inputs */
+#define GBR_VIS_FEATURES_POSE 0    /* default: camera position, sin/cos of yaw and
+                                      pitch, the chunk's world box centre and extents
+                                      relative to the camera (also in the camera
+                                      frame), nothing from earlier frames */
+#define GBR_VIS_FEATURES_HISTORY 1 /* ablation: visibility in earlier frames, the
+                                      projected rectangle and depth, box and camera */
 
 typedef struct gbr_vis_fit_params {
   int32_t steps;            /* optimiser steps */
@@ -428,6 +445,10 @@ gbr_status gbr_vis_init(gbr_renderer* r, uint64_t seed);
    chunk that passed the frustum test strictly inside the clip volume */
 gbr_status gbr_vis_collect(gbr_renderer* r, uint64_t* rows_added);
 gbr_status gbr_vis_clear_rows(gbr_renderer* r);
+/* selects the feature set (GBR_VIS_FEATURES_*) of collected rows and of the
+   learned proposal; switching drops the training rows (the weights stay) */
+gbr_status gbr_vis_set_feature_set(gbr_renderer* r, int32_t feature_set);
+gbr_status gbr_vis_feature_set(const gbr_renderer* r, int32_t* out);
 gbr_status gbr_vis_rows(gbr_renderer* r, uint64_t* rows, uint64_t* positives);
 /* Adam on the weighted binary cross-entropy of the training rows */
 gbr_status gbr_vis_fit(gbr_renderer* r, const gbr_vis_fit_params* params,

@@ -4,11 +4,11 @@ Nothing in the product imports this module; tests use it to check the GPU
 forward and backward passes (habitat-sim_amd/csrc/gbr_mlp.hip, gbr_vis.cpp).
 The model has no counterpart in the reference (its batch renderer draws
 every mesh, src/esp/gfx_batch/Renderer.cpp:1298-1351); it is the learned
-proposal of the two-phase chunk cull: 32 -> 128 -> 128 -> 1, ReLU, weighted
+proposal of the two-phase chunk cull: 32 -> 256 -> 256 -> 1, ReLU, weighted
 binary cross-entropy with logits.
 
-Parameter layout (flat float32, 20865 values):
-  W1 [128][32], b1 [128], W2 [128][128], b2 [128], w3 [128], b3
+Parameter layout (flat float32, 74497 values):
+  W1 [256][32], b1 [256], W2 [256][256], b2 [256], w3 [256], b3
 
 ``rounding=True`` rounds to bfloat16 (round to nearest even) exactly where
 the GPU does: inputs, weights W1 W2 w3, hidden activations H1 and H2 (after
@@ -18,7 +18,7 @@ whose gradient the tests pin against central finite differences.
 """
 import numpy as np
 
-IN, HID = 32, 128
+IN, HID = 32, 256
 SIZES = (("W1", (HID, IN)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)),
          ("w3", (HID,)), ("b3", (1,)))
 PARAMS = sum(int(np.prod(s)) for _, s in SIZES)
