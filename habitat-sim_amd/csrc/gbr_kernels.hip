@@ -1,20 +1,35 @@
 // HIP kernels of the batch renderer for gfx950 (CDNA4, wave64).
 //
-// Pipeline per frame (all scenes in one launch per stage):
-//   k_nodes      per node: absolute transform = parent chain product
-//                (Renderer.cpp:1214-1220 moved to the GPU)
-//   k_setup_bin  per work item (scene, draw, chunk of <= 256 triangles):
-//                vertex transform of the chunk's vertex range into LDS
-//                (coalesced SoA position streams), fixed-point triangle setup
-//                (8 sub-pixel bits, top-left rule), trivial/back-face/empty
-//                rejection, screen-bin append with wave-aggregated atomics
-//                (ballot + prefix popcount)
-//   k_raster     per (scene, bin): re-setup of the bin's triangles, depth test
-//                as a 64-bit atomicMin of (window depth bits, primitive order)
-//                in an LDS visibility tile, then the resolve in the same
-//                workgroup: perspective-correct attributes, mip-mapped
-//                texture sampling, RGBA8 / window depth / linear depth /
-//                semantic written once to HBM
+// Pipeline per frame (all scenes in one launch per stage, on one stream):
+//   k_nodes       per node: absolute transform = parent chain product
+//                 (Renderer.cpp:1214-1220 moved to the GPU); resets the
+//                 frame's counters
+//   k_cull        culled modes, per work item (<= 1024 consecutive triangles
+//                 of one draw): box-vs-frustum test with a float error bound,
+//                 conservative pixel rectangle and nearest depth, phase-1 /
+//                 candidate lists, learned mode: 32 feature rows for the
+//                 visibility MLP (k_mlp_score in gbr_mlp.hip, then k_propose)
+//   k_setup_bin   persistent, per work item from a device queue: vertex
+//                 transform of the item's vertex range into LDS (coalesced
+//                 SoA position streams), fixed-point triangle setup (8
+//                 sub-pixel bits, top-left rule), trivial/back-face/empty
+//                 rejection, per-triangle raster and resolve records, bin
+//                 append with one global atomic per touched bin and item
+//   k_setup_clip  triangles crossing the near/far plane or the guard band:
+//                 Sutherland-Hodgman fan, each fan triangle binned on its own
+//   k_raster_lane per (scene, bin): one binned triangle per lane, large
+//                 rectangles from an LDS queue by the whole workgroup; depth
+//                 test as a 64-bit LDS atomicMin of (window depth bits,
+//                 primitive order); visibility keys to HBM (k_raster_pairs:
+//                 selectable pair-parallel form; k_raster_full: exact rescan
+//                 of overflowed bins)
+//   k_hiz, k_cull_hiz  two-phase culling: depth pyramid of the phase-1 keys,
+//                 then phase-2 candidates not hidden by it go through setup
+//                 and raster again
+//   k_resolve     per bin: each pixel once, key -> record -> perspective-
+//                 correct attributes, mip-mapped texture sampling, RGBA8 /
+//                 window depth / linear depth / semantic written once to HBM
+//   k_gather      framebuffer-layout readback of a rectangle
 //
 // Semantics follow OpenGL as driven by the reference (Renderer.cpp:307-308,
 // 1298-1351; RendererStandalone.cpp:155-159; DepthUnprojection.cpp:98-130);
