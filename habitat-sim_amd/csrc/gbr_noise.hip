@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -166,6 +167,40 @@ __global__ __launch_bounds__(kNoiseBlock) void k_philox_bits(uint64_t seed, uint
 }
 
 }  // namespace
+
+// Staging of gbr_redwood_simulate_host_noalloc(): the input depth image in
+// the first kRwScratchPixels floats, the output image in the second.  Every
+// GPU the library runs on holds its own copy (8 MiB) until the process ends.
+// Host code never names the array in a launch or copy: it uses the per-device
+// address from hipGetSymbolAddress (scratch_address below).
+constexpr long long kRwScratchPixels = GBR_REDWOOD_NOALLOC_MAX_PIXELS;
+static __device__ __align__(256) float scratch[2 * kRwScratchPixels];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch[kScratchDevices] = {};
+
+// the device address of `scratch` on device `dev`, which must be current;
+// looked up once per device
+gbr_status scratch_address(int dev, float** out) {
+  if (dev < 0 || dev >= kScratchDevices)
+    return fail(GBR_ERR_UNSUPPORTED, "redwood_create: device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GBR_ERR_DEVICE, std::string("redwood_create: hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(e));
+    g_scratch[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch[dev];
+  return GBR_OK;
+}
+
+}  // namespace
 }  // namespace gbr
 
 struct gbr_redwood {
@@ -175,6 +210,7 @@ struct gbr_redwood {
   uint64_t calls = 0;  // generator calls so far (the counter's high word)
   gbr::DevBuf<float> d_model;
   gbr::DevBuf<float> d_in, d_out;  // staging of the host entry point
+  float* scratch = nullptr;        // this device's `scratch` (no-allocation entry point)
 };
 
 using gbr::fail;
@@ -224,7 +260,8 @@ gbr_status gbr_redwood_create(const float* model, int32_t device, float noise_mu
   m->mult = noise_multiplier;
   m->seed = seed;
   const size_t n = (size_t)GBR_REDWOOD_MODEL_FLOATS;
-  gbr_status st = m->d_model.reserve(n);
+  gbr_status st = gbr::scratch_address(dev, &m->scratch);
+  if (st == GBR_OK) st = m->d_model.reserve(n);
   if (st == GBR_OK) {
     const hipError_t e = hipMemcpy(m->d_model.ptr, model, n * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) st = fail(GBR_ERR_DEVICE, std::string("redwood_create: ") + hipGetErrorString(e));
@@ -277,6 +314,28 @@ gbr_status gbr_redwood_simulate_host(gbr_redwood* m, const float* depth, int32_t
   HIP_TRY(hipMemcpy(m->d_in.ptr, depth, n * sizeof(float), hipMemcpyHostToDevice));
   TRY(launch(m, m->d_in.ptr, 1, rows, cols, nullptr, m->d_out.ptr, nullptr));
   HIP_TRY(hipMemcpy(out, m->d_out.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+  return GBR_OK;
+}
+
+gbr_status gbr_redwood_simulate_host_noalloc(gbr_redwood* m, const float* depth, int32_t rows,
+                                             int32_t cols, float* out) {
+  if (!m || !depth || !out)
+    return fail(GBR_ERR_ARGUMENT, "redwood_simulate_host_noalloc: null argument");
+  TRY(check_dims(1, rows, cols, "redwood_simulate_host_noalloc"));
+  const long long n = (long long)rows * cols;
+  if (n > gbr::kRwScratchPixels)
+    return fail(GBR_ERR_SIZE, "redwood_simulate_host_noalloc: rows x cols = " + std::to_string(n) +
+                                  " exceeds GBR_REDWOOD_NOALLOC_MAX_PIXELS (" +
+                                  std::to_string(gbr::kRwScratchPixels) + ")");
+  if (!m->scratch) return fail(GBR_ERR_STATE, "redwood_simulate_host_noalloc: no scratch address");
+  HIP_TRY(hipSetDevice(m->device));
+  // the input region is written whole before the kernel reads it, and the
+  // kernel writes every output pixel: nothing in the scratch needs clearing
+  float* in = m->scratch;
+  float* res = m->scratch + gbr::kRwScratchPixels;
+  HIP_TRY(hipMemcpy(in, depth, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  TRY(launch(m, in, 1, rows, cols, nullptr, res, nullptr));
+  HIP_TRY(hipMemcpy(out, res, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
   return GBR_OK;
 }
 

@@ -230,6 +230,8 @@ SIGNATURES = {
                                  ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
     "gbr_redwood_simulate_host": (S, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                       ctypes.c_int32, ctypes.c_void_p]),
+    "gbr_redwood_simulate_host_noalloc": (S, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                              ctypes.c_int32, ctypes.c_void_p]),
     "gbr_redwood_simulate_normals": (S, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),

@@ -493,6 +493,19 @@ gbr_status gbr_redwood_simulate(gbr_redwood* m, const float* depth, int32_t imag
    image in, host image out, synchronous */
 gbr_status gbr_redwood_simulate_host(gbr_redwood* m, const float* depth, int32_t rows, int32_t cols,
                                      float* out);
+/* new, no counterpart in the reference: gbr_redwood_simulate_host() for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  It stages the
+   image in a fixed device array of the library (2 x
+   GBR_REDWOOD_NOALLOC_MAX_PIXELS floats, 8 MiB per GPU, held until the
+   process ends) and calls no allocator: rows x cols above
+   GBR_REDWOOD_NOALLOC_MAX_PIXELS give GBR_ERR_SIZE.  Same output and
+   generator sequence as gbr_redwood_simulate_host().  Every call on one GPU
+   shares that array: calls that overlap on one GPU (several threads, or
+   models on the same device) race and must be serialised by the caller. */
+#define GBR_REDWOOD_NOALLOC_MAX_PIXELS (1 << 20)
+gbr_status gbr_redwood_simulate_host_noalloc(gbr_redwood* m, const float* depth, int32_t rows,
+                                             int32_t cols, float* out);
 /* new: gbr_redwood_simulate with caller-given standard normal deviates, 3
    per pixel (y shift, x shift, disparity), in device memory */
 gbr_status gbr_redwood_simulate_normals(gbr_redwood* m, const float* depth, int32_t images,
