@@ -163,7 +163,12 @@ enum : uint32_t {
   kCtrList2 = 6,     // work items set up in phase 2
   kCtrClipMark = 7,  // clipped primitives handled by the phase-1 clip pass
   kCtrProp = 8,      // work items with a visibility-feature row this frame
-  kCtrCount = 9
+  kCtrHeavy1 = 9,    // bins listed as heavy by phase-1 setup (raster takes them first)
+  kCtrHeavy2 = 10,   // the same for phase 2
+  kCtrRaster1 = 11,  // raster work queue of phase 1 (claims by persistent workgroups)
+  kCtrRaster2 = 12,  // and of phase 2
+  kCtrActive2 = 13,  // bins that received entries in phase 2 (the only ones its raster visits)
+  kCtrCount = 14
 };
 constexpr uint32_t kCtrStride = 32;
 // The setup work queue of each phase is split into kQueueStripes contiguous
@@ -290,6 +295,13 @@ struct FrameParams {
   PrimRef* clipped;             // primitives that need clipping
   uint32_t* counters;           // kCtr*
   unsigned long long* vis;      // per-pixel visibility keys, scene planes
+  // heavy bins (entries >= heavy_entries in a phase): listed by setup when
+  // a bin's count crosses the threshold, flagged per bin; the persistent
+  // raster takes the list first (longest first), then every other bin
+  uint32_t* heavy;              // [raster items] bins listed this phase
+  uint32_t* bin_heavy;          // [raster items] 1 = listed (cleared by raster)
+  uint32_t* active;             // [raster items] bins with phase-2 entries
+  uint32_t heavy_entries;       // 0: no heavy list
   uint32_t clipped_cap, sub_cap;
   uint32_t queue_grain;         // setup work items fetched per queue atomic
   uint32_t n_items;             // work items of the frame
@@ -308,6 +320,7 @@ struct FrameParams {
   int32_t hiz_w0, hiz_h0, hiz_w1, hiz_h1;  // pyramid level sizes in blocks
   uint32_t hiz_stride;
   float proposal_threshold;
+  int32_t proposal_history;  // learned mode: phase 1 also takes last frame's visible items
   int32_t raster_mode;  // 0: a lane per triangle, 1: a lane per (triangle, pixel)
   int32_t lane_pixels;  // mode 0: rectangles above this go to the workgroup queue
   uint32_t lds_bins;    // setup: scenes with at most this many bins count in LDS

@@ -294,14 +294,18 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
                                                   uint32_t* __restrict__ counters,
                                                   unsigned long long* __restrict__ stats,
                                                   uint32_t* __restrict__ frame,
-                                                  float* __restrict__ nrm_out) {
+                                                  float* __restrict__ nrm_out,
+                                                  const uint32_t* __restrict__ dirty) {
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   // first kernel of every frame: reset the frame's counters, advance the
   // frame number (kept on the device, so a captured frame replays as is)
   if (i < kCtrTotal) counters[i * kCtrStride] = 0u;
   if (i == 0) *frame += 1u;
   for (uint32_t j = i; j < kStatCount * kStatStripes; j += gridDim.x * kBlock) stats[j] = 0ull;
-  if (i >= n) return;
+  // the host uploads *dirty with the cameras of every frame: 0 when no local
+  // transform and no node list changed since the previous frame, whose
+  // absolute transforms (and normal matrices) are still in place
+  if (i >= n || *dirty == 0u) return;
   int depth = 0;
   for (int32_t q = parents[i]; q >= 0 && depth < 1024; q = parents[q]) ++depth;
   float acc[16];
@@ -526,6 +530,24 @@ __device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counte
   }
 }
 
+// A bin's entry count went from `old` to `old + n` in this phase: the one
+// reservation that crosses the heavy threshold lists the bin for the raster
+// to take first, and in phase 2 the first reservation lists the bin as
+// active (counts only grow within a phase, so a bin is listed at most once
+// per list and no list exceeds the raster items).
+__device__ __forceinline__ void note_heavy(const FrameParams& p, uint32_t gbin, uint32_t old,
+                                           uint32_t n, int phase) {
+  if (phase == 2 && old == 0u && n != 0u)
+    p.active[atomicAdd(&p.counters[kCtrActive2 * kCtrStride], 1u)] = gbin;
+  const uint32_t t = p.heavy_entries;
+  if (t != 0u && old < t && n >= t - old) {
+    const uint32_t slot =
+        atomicAdd(&p.counters[(phase == 2 ? kCtrHeavy2 : kCtrHeavy1) * kCtrStride], 1u);
+    p.heavy[slot] = gbin;
+    p.bin_heavy[gbin] = 1u;
+  }
+}
+
 // --------------------------------------------------------------------------
 // k_setup_bin: persistent workgroups draining a queue of work items (chunks
 // of <= 1024 consecutive triangles of one draw)
@@ -565,7 +587,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
   uint32_t written = 0;
   unsigned long long tri_acc = 0, vtx_acc = 0;  // triangles / vertices set up (uniform)
   uint32_t grab = 0, grab_end = 0;  // items claimed from the queue (uniform)
-  const bool hiz_test = queue_ctr == kCtrQueue2 && p.hiz != nullptr;
+  const int phase = queue_ctr == kCtrQueue2 ? 2 : 1;
+  const bool hiz_test = phase == 2 && p.hiz != nullptr;
   // the whole frame, or the work items listed for this phase
   const uint32_t n_items = list ? min(p.counters[list_ctr * kCtrStride], p.n_items) : p.n_items;
 
@@ -750,6 +773,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
                 s_pair[pi] = (b << 21) | (slot << 10) | j;
               } else {
                 const uint32_t slot = atomicAdd(&p.bin_count[sc.bin_base + b], 1u);
+                note_heavy(p, sc.bin_base + b, slot, 1u, phase);
                 if (slot < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b * sc.bin_cap + slot] = prim;
               }
               ++written;
@@ -764,7 +788,11 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
     // one global reservation per touched bin
     for (int b = threadIdx.x; b < nlb; b += kBlock) {
       const uint32_t n = s_bcount[b];
-      if (n) s_bbase[b] = atomicAdd(&p.bin_count[sc.bin_base + b], n);
+      if (n) {
+        const uint32_t old = atomicAdd(&p.bin_count[sc.bin_base + b], n);
+        s_bbase[b] = old;
+        note_heavy(p, sc.bin_base + b, old, n, phase);
+      }
     }
     __syncthreads();
     const uint32_t np = min(s_npairs, (uint32_t)kPairCap);
@@ -870,17 +898,22 @@ __device__ __forceinline__ void make_big(const RasterCtx& rc, const int32_t* X, 
   b.narrow = narrow ? 1 : 0;
 }
 
-// The queued triangles, by the whole workgroup: each thread owns the pixels
-// (lx, ly0 + k * step) of the bin and tests them against every triangle, with
-// the edge values stepped by additions (exact: 32-bit when narrow).
-__device__ void raster_big_all(const RasterCtx& rc, const BigTri* big, int nb) {
+// The queued triangles, by the whole workgroup.  Each thread owns the pixels
+// (lx, ly0 + k * step), k < kOwn, of the bin and keeps their nearest keys in
+// registers (`keys`) across every batch of the bin; merge_big_keys() folds
+// them into the LDS tile once, after the bin's last batch.  The edge values
+// are stepped by additions (exact: 32-bit when narrow).  No LDS atomics:
+// a pixel has one owner.
+constexpr int kOwn = kMaxBin * kMaxBin / kBlock;
+
+__device__ void raster_big_regs(const RasterCtx& rc, const BigTri* big, int nb,
+                                unsigned long long (&keys)[kOwn]) {
   const int B = rc.B;
   const int lx = (int)threadIdx.x % B, ly0 = (int)threadIdx.x / B, step = kBlock / B;
   if (threadIdx.x == 0)
     for (int q = 0; q < nb; ++q) *rc.pairs += big[q].npx;
   if (ly0 >= B) return;  // (8-pixel bins: the first 64 threads cover the bin)
-  const int px = rc.bin.x0 + lx;
-  const long long Px = (long long)px * 256 + 128;
+  const long long Px = (long long)(rc.bin.x0 + lx) * 256 + 128;
   for (int q = 0; q < nb; ++q) {
     const BigTri& t = big[q];
     if (lx < t.x0 || lx > t.x1) continue;
@@ -892,14 +925,15 @@ __device__ void raster_big_all(const RasterCtx& rc, const BigTri* big, int nb) {
         e[i] = (uint32_t)t.e[i] + (uint32_t)t.sx[i] * (uint32_t)lx + (uint32_t)t.sy[i] * (uint32_t)ly0;
         d[i] = (uint32_t)t.sy[i] * (uint32_t)step;
       }
-      for (int ly = ly0; ly < B; ly += step) {
-        if (ly >= ylo && ly <= yhi && (int32_t)(e[0] | e[1] | e[2]) >= 0) {
+#pragma unroll
+      for (int k = 0; k < kOwn; ++k) {
+        const int ly = ly0 + k * step;
+        if (ly < B && ly >= ylo && ly <= yhi && (int32_t)(e[0] | e[1] | e[2]) >= 0) {
           ++*rc.frags;
-          const int py = rc.bin.y0 + ly;
           const float z = depth_at(t.z0, t.dzdx, t.dzdy, t.zlo, t.X0, t.Y0, Px,
-                                   (long long)py * 256 + 128);
-          if (z < 1.0f)
-            atomicMin(&rc.vis[ly * B + lx], ((unsigned long long)__float_as_uint(z) << 32) | t.prim);
+                                   (long long)(rc.bin.y0 + ly) * 256 + 128);
+          const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | t.prim;
+          if (z < 1.0f && key < keys[k]) keys[k] = key;
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) e[i] += d[i];
@@ -911,14 +945,15 @@ __device__ void raster_big_all(const RasterCtx& rc, const BigTri* big, int nb) {
         e[i] = t.e[i] + (long long)t.sx[i] * lx + (long long)t.sy[i] * ly0;
         d[i] = (long long)t.sy[i] * step;
       }
-      for (int ly = ly0; ly < B; ly += step) {
-        if (ly >= ylo && ly <= yhi && (e[0] | e[1] | e[2]) >= 0) {
+#pragma unroll
+      for (int k = 0; k < kOwn; ++k) {
+        const int ly = ly0 + k * step;
+        if (ly < B && ly >= ylo && ly <= yhi && (e[0] | e[1] | e[2]) >= 0) {
           ++*rc.frags;
-          const int py = rc.bin.y0 + ly;
           const float z = depth_at(t.z0, t.dzdx, t.dzdy, t.zlo, t.X0, t.Y0, Px,
-                                   (long long)py * 256 + 128);
-          if (z < 1.0f)
-            atomicMin(&rc.vis[ly * B + lx], ((unsigned long long)__float_as_uint(z) << 32) | t.prim);
+                                   (long long)(rc.bin.y0 + ly) * 256 + 128);
+          const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | t.prim;
+          if (z < 1.0f && key < keys[k]) keys[k] = key;
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) e[i] += d[i];
@@ -927,23 +962,24 @@ __device__ void raster_big_all(const RasterCtx& rc, const BigTri* big, int nb) {
   }
 }
 
-// rasterise one front-facing triangle with a known depth plane: this lane
-// walks the clipped rectangle row by row with edge functions stepped by
-// additions; rectangles above rc.lane_limit pixels go to the workgroup's
-// queue (a full queue leaves them to the lane)
-__device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32_t* Y, float z0,
-                              float dzdx, float dzdy, float zlo, uint32_t prim) {
-  PixRect r = tri_pixels(X, Y, rc.bin);
-  if (r.x0 > r.x1 || r.y0 > r.y1) return;
-  const int npx = (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
-  if (npx > rc.lane_limit) {
-    const int slot = atomicAdd(rc.nbig, 1);
-    if (slot < kBigQueue) {
-      make_big(rc, X, Y, z0, dzdx, dzdy, zlo, prim, r, npx, rc.big[slot]);
-      return;
-    }
+// the owned keys into the LDS tile (after a barrier that ends every other
+// write to it; each pixel has one owner, so plain stores suffice)
+__device__ __forceinline__ void merge_big_keys(const RasterCtx& rc,
+                                               const unsigned long long (&keys)[kOwn]) {
+  const int B = rc.B;
+  const int lx = (int)threadIdx.x % B, ly0 = (int)threadIdx.x / B, step = kBlock / B;
+  if (ly0 >= B) return;
+#pragma unroll
+  for (int k = 0; k < kOwn; ++k) {
+    const int ly = ly0 + k * step;
+    if (ly < B && keys[k] < rc.vis[ly * B + lx]) rc.vis[ly * B + lx] = keys[k];
   }
-  *rc.pairs += (uint32_t)npx;
+}
+
+// this lane walks the clipped rectangle r of one front-facing triangle with
+// a known depth plane row by row, edge functions stepped by additions
+__device__ void raster_walk(const RasterCtx& rc, const int32_t* X, const int32_t* Y, float z0,
+                            float dzdx, float dzdy, float zlo, uint32_t prim, const PixRect& r) {
   if (small_box(X, Y)) {
     int32_t dx[3], dy[3], bias[3];
 #pragma unroll
@@ -981,6 +1017,35 @@ __device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32
       depth_key_min(rc, z0, dzdx, dzdy, zlo, X, Y, prim, px, py);
     }
   }
+}
+
+// The clipped rectangle of a triangle in this bin and its pixel count (0:
+// none); rectangles above rc.lane_limit pixels go to the workgroup's queue
+// (returns -1; a full queue leaves them to the lane)
+__device__ __forceinline__ int raster_classify(const RasterCtx& rc, const int32_t* X,
+                                               const int32_t* Y, float z0, float dzdx, float dzdy,
+                                               float zlo, uint32_t prim, PixRect& r) {
+  r = tri_pixels(X, Y, rc.bin);
+  if (r.x0 > r.x1 || r.y0 > r.y1) return 0;
+  const int npx = (r.x1 - r.x0 + 1) * (r.y1 - r.y0 + 1);
+  if (npx > rc.lane_limit) {
+    const int slot = atomicAdd(rc.nbig, 1);
+    if (slot < kBigQueue) {
+      make_big(rc, X, Y, z0, dzdx, dzdy, zlo, prim, r, npx, rc.big[slot]);
+      return -1;
+    }
+  }
+  *rc.pairs += (uint32_t)npx;
+  return npx;
+}
+
+// rasterise one front-facing triangle with a known depth plane: queued, or
+// walked by this lane
+__device__ void raster_planed(const RasterCtx& rc, const int32_t* X, const int32_t* Y, float z0,
+                              float dzdx, float dzdy, float zlo, uint32_t prim) {
+  PixRect r;
+  if (raster_classify(rc, X, Y, z0, dzdx, dzdy, zlo, prim, r) > 0)
+    raster_walk(rc, X, Y, z0, dzdx, dzdy, zlo, prim, r);
 }
 
 __device__ __forceinline__ void raster_tri(const RasterCtx& rc, const int32_t* X, const int32_t* Y,
@@ -1098,14 +1163,28 @@ __device__ __noinline__ int wrap_coord_slow(int i, int size, int mode) {
   return m;
 }
 
-// GL wrap of texel index i: clamp-to-edge and power-of-two repeat (the
-// common cases) by selects, the rest out of line
-__device__ __forceinline__ int wrap_coord(int i, int size, int mode) {
+// GL wrap of texel indices along one axis of one level, decided once per
+// level: clamp-to-edge and power-of-two repeat share one form,
+// clamp(i & mask, 0, size - 1) (mask = size - 1 for repeat, all ones for
+// clamp), the rest (mirrored repeat, repeat of other sizes) goes out of line.
+struct AxisWrap {
+  int size, mask, mode;
+  bool fast;
+};
+
+__device__ __forceinline__ AxisWrap axis_wrap(int size, int mode) {
   const bool clamp = mode == GL_CLAMP_TO_EDGE;
-  const bool fast = clamp || (mode != GL_MIRRORED_REPEAT && (size & (size - 1)) == 0);
-  if (__builtin_expect(fast, 1))
-    return clamp ? min(max(i, 0), size - 1) : (i & (size - 1));  // non-negative i mod size
-  return wrap_coord_slow(i, size, mode);
+  AxisWrap a;
+  a.size = size;
+  a.mode = mode;
+  a.fast = clamp || (mode != GL_MIRRORED_REPEAT && (size & (size - 1)) == 0);
+  a.mask = clamp ? -1 : size - 1;
+  return a;
+}
+
+__device__ __forceinline__ int wrap_coord(const AxisWrap& a, int i) {
+  if (__builtin_expect(a.fast, 1)) return min(max(i & a.mask, 0), a.size - 1);
+  return wrap_coord_slow(i, a.size, a.mode);
 }
 
 __device__ __forceinline__ int floor_to_int(float u) {
@@ -1128,10 +1207,11 @@ __device__ __forceinline__ TexHeader load_tex_header(const TextureDesc* t) {
   return TexHeader{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 }
 
-// texel (i, j) of `layer` at a level of size w x h starting at byte `base`
-__device__ __forceinline__ void fetch(const float* lut, const uint8_t* base, int w, int h,
-                                      int layer, int i, int j, float* out) {
-  const uint32_t idx = ((uint32_t)layer * (uint32_t)h + (uint32_t)j) * (uint32_t)w + (uint32_t)i;
+// texel `row + i` of a level whose first byte is `base`: row = (layer * h
+// + j) * w in 32-bit arithmetic, as the texel index always was
+__device__ __forceinline__ void fetch(const float* lut, const uint8_t* base, uint32_t row,
+                                      int i, float* out) {
+  const uint32_t idx = row + (uint32_t)i;
   const uint32_t texel = *reinterpret_cast<const uint32_t*>(base + (size_t)idx * 4);
   out[0] = lut[texel & 0xff];
   out[1] = lut[(texel >> 8) & 0xff];
@@ -1139,15 +1219,20 @@ __device__ __forceinline__ void fetch(const float* lut, const uint8_t* base, int
   out[3] = lut[texel >> 24];
 }
 
+__device__ __forceinline__ uint32_t texel_row(int layer, int h, int j, int w) {
+  return ((uint32_t)layer * (uint32_t)h + (uint32_t)j) * (uint32_t)w;
+}
+
 __device__ void sample_level(const float* lut, const TexHeader& t, const uint8_t* base, int level,
                              int layer, float s, float tc, bool linear, float* out) {
   int w = max(t.width >> level, 1), h = max(t.height >> level, 1);
+  const AxisWrap ws = axis_wrap(w, t.wrap_s), wt = axis_wrap(h, t.wrap_t);
   float u = s * (float)w;
   float v = tc * (float)h;
   if (!linear) {
-    int i = wrap_coord(floor_to_int(u), w, t.wrap_s);
-    int j = wrap_coord(floor_to_int(v), h, t.wrap_t);
-    fetch(lut, base, w, h, layer, i, j, out);
+    int i = wrap_coord(ws, floor_to_int(u));
+    int j = wrap_coord(wt, floor_to_int(v));
+    fetch(lut, base, texel_row(layer, h, j, w), i, out);
     return;
   }
   float uu = u - 0.5f;
@@ -1156,15 +1241,16 @@ __device__ void sample_level(const float* lut, const TexHeader& t, const uint8_t
   int j0 = floor_to_int(vv);
   float a = uu - (float)i0;
   float b = vv - (float)j0;
-  int i1 = wrap_coord(i0 + 1, w, t.wrap_s);
-  int j1 = wrap_coord(j0 + 1, h, t.wrap_t);
-  i0 = wrap_coord(i0, w, t.wrap_s);
-  j0 = wrap_coord(j0, h, t.wrap_t);
+  int i1 = wrap_coord(ws, i0 + 1);
+  int j1 = wrap_coord(wt, j0 + 1);
+  i0 = wrap_coord(ws, i0);
+  j0 = wrap_coord(wt, j0);
+  const uint32_t r0 = texel_row(layer, h, j0, w), r1 = texel_row(layer, h, j1, w);
   float t00[4], t10[4], t01[4], t11[4];
-  fetch(lut, base, w, h, layer, i0, j0, t00);
-  fetch(lut, base, w, h, layer, i1, j0, t10);
-  fetch(lut, base, w, h, layer, i0, j1, t01);
-  fetch(lut, base, w, h, layer, i1, j1, t11);
+  fetch(lut, base, r0, i0, t00);
+  fetch(lut, base, r0, i1, t10);
+  fetch(lut, base, r1, i0, t01);
+  fetch(lut, base, r1, i1, t11);
   float ia = 1.0f - a;
   float ib = 1.0f - b;
   float w00 = ia * ib, w10 = a * ib, w01 = ia * b, w11 = a * b;
@@ -1479,7 +1565,7 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
 // flight together (each returns the entry's slot; taken one at a time, their
 // round trips add up along a thread).
 __device__ __forceinline__ void bin_rect_atomic(const FrameParams& p, const SceneDesc& sc,
-                                                const PixRect& r, uint32_t entry) {
+                                                const PixRect& r, uint32_t entry, int phase) {
   const int sh = p.bin_shift;
   const int bx0 = r.x0 >> sh, by0 = r.y0 >> sh;
   const int nx = (r.x1 >> sh) - bx0 + 1, nb = nx * ((r.y1 >> sh) - by0 + 1);
@@ -1492,8 +1578,10 @@ __device__ __forceinline__ void bin_rect_atomic(const FrameParams& p, const Scen
       slot[g] = j < nb ? atomicAdd(&p.bin_count[sc.bin_base + b[g]], 1u) : sc.bin_cap;
     }
 #pragma unroll
-    for (int g = 0; g < 8; ++g)
+    for (int g = 0; g < 8; ++g) {
+      if (j0 + g < nb) note_heavy(p, sc.bin_base + b[g], slot[g], 1u, phase);
       if (slot[g] < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b[g] * sc.bin_cap + slot[g]] = entry;
+    }
   }
 }
 
@@ -1603,7 +1691,7 @@ __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int ph
           rr.z[2] = PZ[b];
           p.sub_recs[slot] = rr;
           p.sub_prims[slot] = ref.prim;
-          bin_rect_atomic(p, sc, r, kSubTri | slot);
+          bin_rect_atomic(p, sc, r, kSubTri | slot, phase);
         } else {
           overflow_rect(p, sc, r);
         }
@@ -1793,20 +1881,25 @@ __device__ void bin_hiz(const FrameParams& p, const BinInfo& b, const unsigned l
 }
 
 // --------------------------------------------------------------------------
-// k_raster_lane: per (scene, bin), one binned triangle per lane walking its
-// clipped rectangle; rectangles above the lane limit (gbr_tuning.lane_pixels)
-// are rasterised by the whole block from an LDS queue.  Phase 1 starts every bin from the
-// clear key; phase 2 merges into the phase-1 keys and leaves bins without
-// new entries untouched.
+// k_raster_lane: persistent workgroups draining the phase's raster queue:
+// first the bins setup listed as heavy (most entries), then every other bin
+// in order, so that the longest bins start first and the frame's tail is
+// short.  Per bin, one binned triangle per lane walking its clipped
+// rectangle; rectangles above the lane limit (gbr_tuning.lane_pixels) are
+// rasterised by the whole workgroup from an LDS queue.  Phase 1 starts every
+// bin from the clear key; phase 2 merges into the phase-1 keys and leaves
+// bins without new entries untouched.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
-                                                        const RasterItem* __restrict__ items,
-                                                        int phase) {
-  __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
-  __shared__ BigTri s_big[kBigQueue];
-  __shared__ int s_nbig;
 
-  const BinInfo b = bin_info(items[blockIdx.x]);
+struct RasterLds {
+  unsigned long long vis[kMaxBin * kMaxBin];
+  BigTri big[kBigQueue];
+  int nbig[2];  // queue fill of even / odd batches
+};
+
+__device__ void raster_bin(const FrameParams& p, const RasterItem& item, int phase, RasterLds& L,
+                           uint32_t& pairs, uint32_t& frags) {
+  const BinInfo b = bin_info(item);
   const uint32_t* entries = p.bin_entries + b.entries;
   // the first entries load beside the count (both addresses are known; an
   // entry at or past the count is never used)
@@ -1815,11 +1908,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
   if (n > b.bin_cap) return;  // overflowed: k_raster_full rescans this bin
   if (phase == 2 && n == 0) return;
   const int B = p.bin_size;
-  uint32_t pairs = 0, frags = 0;
   RasterCtx rc;
-  rc.vis = s_vis;
-  rc.big = s_big;
-  rc.nbig = &s_nbig;
+  rc.vis = L.vis;
+  rc.big = L.big;
   rc.B = B;
   rc.lane_limit = p.lane_pixels;
   rc.bin = b.bin;
@@ -1846,15 +1937,25 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
     for (int q = 0; q < kPer; ++q) {
       const int l = threadIdx.x + q * kBlock;
       const int ly = div_small(l, inv_bw), lx = l - ly * bw;
-      if (l < bw * bh) s_vis[ly * B + lx] = keys[q];
+      if (l < bw * bh) L.vis[ly * B + lx] = keys[q];
     }
   } else {
-    for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
+    for (int i = threadIdx.x; i < B * B; i += kBlock) L.vis[i] = kClearKey;
   }
-  if (threadIdx.x == 0) s_nbig = 0;
+  if (threadIdx.x == 0) L.nbig[0] = L.nbig[1] = 0;
+  unsigned long long own[kOwn];
+#pragma unroll
+  for (int k = 0; k < kOwn; ++k) own[k] = ~0ull;
   __syncthreads();
-  for (uint32_t base = 0; base < n; base += kBlock) {
+  // per batch of kBlock entries: lanes append large rectangles to the queue
+  // (fill counter of this batch's parity; the other one, read before the
+  // previous batch's second barrier, is cleared for the next batch), then
+  // the whole workgroup rasterises the queue into its owned keys
+  int parity = 0;
+  for (uint32_t base = 0; base < n; base += kBlock, parity ^= 1) {
     const uint32_t k = base + threadIdx.x;
+    rc.nbig = &L.nbig[parity];
+    if (threadIdx.x == 0) L.nbig[parity ^ 1] = 0;
     if (k < n) {
       int32_t X[3], Y[3];
       float z0, dzdx, dzdy, zlo;
@@ -1868,14 +1969,52 @@ __global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
       raster_planed(rc, X, Y, z0, dzdx, dzdy, zlo, prim);
     }
     __syncthreads();
-    raster_big_all(rc, s_big, min(s_nbig, kBigQueue));
-    __syncthreads();
-    if (threadIdx.x == 0) s_nbig = 0;
+    raster_big_regs(rc, L.big, min(L.nbig[parity], kBigQueue), own);
     __syncthreads();
   }
-  store_vis(p, b, s_vis, B);
-  if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
+  merge_big_keys(rc, own);
+  __syncthreads();
+  store_vis(p, b, L.vis, B);
+  if (phase == 1 && p.hiz_fused) bin_hiz(p, b, L.vis);
   if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
+}
+
+__global__ __launch_bounds__(kBlock) void k_raster_lane(FrameParams p,
+                                                        const RasterItem* __restrict__ items,
+                                                        uint32_t n_ritems, int phase) {
+  __shared__ RasterLds L;
+  __shared__ uint32_t s_claim;
+  const uint32_t n_heavy =
+      min(p.counters[(phase == 2 ? kCtrHeavy2 : kCtrHeavy1) * kCtrStride], n_ritems);
+  // phase 1 visits every bin (each gets its keys and pyramid); phase 2 only
+  // the bins that received entries
+  const uint32_t n_rest =
+      phase == 2 ? min(p.counters[kCtrActive2 * kCtrStride], n_ritems) : n_ritems;
+  uint32_t* const queue = &p.counters[(phase == 2 ? kCtrRaster2 : kCtrRaster1) * kCtrStride];
+  uint32_t pairs = 0, frags = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const uint32_t q = atomicAdd(queue, 1u);
+      uint32_t claim = ~0u;  // the queue is drained
+      if (q < n_heavy) {
+        claim = p.heavy[q];
+      } else if (q - n_heavy < n_rest) {
+        claim = phase == 2 ? p.active[q - n_heavy] : q - n_heavy;
+        // a listed bin was (or is being) rasterised from the heavy list
+        if (p.bin_heavy[claim]) {
+          p.bin_heavy[claim] = 0u;
+          claim |= 0x80000000u;
+        }
+      }
+      s_claim = claim;
+    }
+    __syncthreads();
+    const uint32_t claim = s_claim;
+    __syncthreads();  // s_claim is rewritten by the next claim; the LDS tile is free
+    if (claim == ~0u) break;
+    if (claim & 0x80000000u) continue;
+    raster_bin(p, items[claim], phase, L, pairs, frags);
+  }
   add_raster_stats(p, pairs, frags);
 }
 
@@ -2029,6 +2168,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
     rc.bin = b.bin;
     rc.pairs = &pairs;
     rc.frags = &frags;
+    unsigned long long own[kOwn];
+#pragma unroll
+    for (int k = 0; k < kOwn; ++k) own[k] = ~0ull;
     for (int i = threadIdx.x; i < B * B; i += kBlock) s_vis[i] = kClearKey;
     if (sc.draw_count <= (uint32_t)kDrawCache)
       for (uint32_t i = threadIdx.x; i < sc.draw_count; i += kBlock)
@@ -2041,11 +2183,13 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
       const uint32_t k = base + threadIdx.x;
       if (k < sc.prim_count) raster_prim(p, sc, PV, rc, s_base, k);
       __syncthreads();
-      raster_big_all(rc, s_big, min(s_nbig, kBigQueue));
+      raster_big_regs(rc, s_big, min(s_nbig, kBigQueue), own);
       __syncthreads();
       if (threadIdx.x == 0) s_nbig = 0;
       __syncthreads();
     }
+    merge_big_keys(rc, own);
+    __syncthreads();
     store_vis(p, b, s_vis, B);
     if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
     if (threadIdx.x == 0) {
@@ -2507,8 +2651,9 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
   }
 }
 
-// learned mode: the model's score splits the feature rows into phase 1 and
-// the depth-pyramid candidates
+// learned mode: the model's score (with proposal_history, also visibility
+// in the previous frame) splits the feature rows into phase 1 and the
+// depth-pyramid candidates
 __global__ __launch_bounds__(kListBlock) void k_propose(FrameParams p) {
   const uint32_t row = blockIdx.x * kListBlock + threadIdx.x;
   const uint32_t n = min(p.counters[kCtrProp * kCtrStride], p.n_items);
@@ -2517,7 +2662,8 @@ __global__ __launch_bounds__(kListBlock) void k_propose(FrameParams p) {
   uint32_t item = 0;
   if (row < n) {
     item = p.prop[row];
-    to1 = p.logits[row] > p.proposal_threshold;
+    to1 = p.logits[row] > p.proposal_threshold ||
+          (p.proposal_history && p.item_vis[item] + 1u == *p.frame);
     toc = !to1;
   }
   uint32_t* const ctr[2] = {&p.counters[kCtrList1 * kCtrStride], &p.counters[kCtrCand * kCtrStride]};
@@ -2643,10 +2789,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g) {
 
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
                         uint32_t* counters, unsigned long long* stats, uint32_t* frame,
-                        float* nrm_out, hipStream_t s) {
+                        float* nrm_out, const uint32_t* dirty, hipStream_t s) {
   const uint32_t blocks = n == 0 ? 1 : (n + kBlock - 1) / kBlock;  // always runs: resets counters
   hipLaunchKernelGGL(k_nodes, dim3(blocks), dim3(kBlock), 0, s, parents, local, abs_out, n,
-                     counters, stats, frame, nrm_out);
+                     counters, stats, frame, nrm_out, dirty);
   return hipGetLastError();
 }
 
@@ -2674,13 +2820,23 @@ hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream
   return hipGetLastError();
 }
 
+int raster_blocks_per_cu() {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane, kBlock, 0) != hipSuccess || b < 1)
+    b = 4;
+  return b;
+}
+
+// raster_mode 0: `grid` persistent workgroups (at most one per bin) drain
+// the phase's queue; mode 1: one workgroup per bin
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
-                         hipStream_t s) {
+                         uint32_t grid, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (p.raster_mode == 1)
     hipLaunchKernelGGL(k_raster_pairs, dim3(n), dim3(kBlock), 0, s, p, items, phase);
   else
-    hipLaunchKernelGGL(k_raster_lane, dim3(n), dim3(kBlock), 0, s, p, items, phase);
+    hipLaunchKernelGGL(k_raster_lane, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items, n,
+                       phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_raster_full, dim3((n + kFullGroup - 1) / kFullGroup), dim3(kBlock), 0, s, p,

@@ -289,147 +289,106 @@ hipError_t mlp_adam(float* w, const float* g, float* m, float* v, size_t n, floa
 namespace gbr {
 
 // --------------------------------------------------------------------------
-// Fused scorer for rendering: logits of up to min(rows, *rows_dev) feature
-// rows in one pass, one wave per 32 rows.  Layer 1 (K = 32) and layer 2
-// (K = 256) on v_mfma_f32_32x32x16_bf16 in two halves of 128 columns; H1
-// (bf16, after bias and ReLU) is staged in LDS as the A operand of layer 2;
-// H2 is rounded to bf16 like the training forward, multiplied by w3 and
-// reduced across the 32 lanes that hold a row's columns.
+// Fused scorer for rendering: logits of min(rows, *rows_dev) feature rows in
+// one persistent pass with the weights in registers.  A workgroup of 8 waves
+// holds the model: wave w owns hidden columns [32w, 32w + 32) of both layers,
+// with its slices of W1 (2 MFMA B fragments) and W2 (16 fragments) in VGPRs
+// for the whole kernel.  Per tile of 32 rows: layer 1 (K = 32, 2
+// v_mfma_f32_32x32x16_bf16) -> bias, ReLU, bf16 -> the tile's H1 in LDS;
+// layer 2 (K = 256, 16 MFMAs, A fragments from LDS) -> bias, ReLU, bf16,
+// times w3 -> per-column products in LDS; then each row's logit is summed in
+// k_last_layer's order (lane l: columns l, l+64, l+128, l+192, then a
+// butterfly over 64 lanes).  The K loops run k = 0, 16, ... like k_gemm_nt,
+// so H1, H2 and the logits equal the training forward's bit for bit.
 // --------------------------------------------------------------------------
 
-constexpr int kScoreRows = 32;              // rows per wave and round
-constexpr int kH1Stride = kMlpHidden + 8;   // padded LDS row (bank spread)
-constexpr int kW2Group = 128;               // W2 rows (output columns) staged at a time
-static_assert(kMlpHidden % kW2Group == 0, "hidden width in groups of 4 tiles");
+constexpr int kScoreRows = 32;                  // rows per tile
+constexpr int kScoreWaves = kMlpHidden / 32;    // one wave per 32 hidden columns
+constexpr int kScoreThreads = kScoreWaves * 64;
+constexpr int kH1Stride = kMlpHidden + 8;       // padded LDS row of H1 (bf16)
+constexpr int kPartStride = kMlpHidden + 4;     // padded LDS row of the products (fp32)
+static_assert(kMlpIn == 32 && kMlpHidden % 32 == 0, "scorer tiling");
+static_assert(kScoreRows % kScoreWaves == 0, "rows of the logit reduction per wave");
 
-// Per round a workgroup scores 4 tiles of 32 rows (one per wave): layer 1
-// from L2-resident W1 into the wave's H1 tile in LDS, then layer 2 in groups
-// of 128 output columns, each group of W2 rows staged in LDS once for the 4
-// waves; the output layer folds into the layer-2 epilogue.
-__global__ __launch_bounds__(256) void k_mlp_score(const uint16_t* __restrict__ X,
-                                                   const uint16_t* __restrict__ W1,
-                                                   const float* __restrict__ b1,
-                                                   const uint16_t* __restrict__ W2,
-                                                   const float* __restrict__ b2,
-                                                   const uint16_t* __restrict__ w3,
-                                                   const float* __restrict__ b3, int rows,
-                                                   const uint32_t* __restrict__ rows_dev,
-                                                   float* __restrict__ logits) {
-  __shared__ uint16_t s_h1[4][kScoreRows * kH1Stride];
-  __shared__ uint16_t s_w2[kW2Group * kH1Stride];
+__global__ __launch_bounds__(kScoreThreads) void k_mlp_score(const uint16_t* __restrict__ X,
+                                                             const uint16_t* __restrict__ W1,
+                                                             const float* __restrict__ b1,
+                                                             const uint16_t* __restrict__ W2,
+                                                             const float* __restrict__ b2,
+                                                             const uint16_t* __restrict__ w3,
+                                                             const float* __restrict__ b3,
+                                                             int rows,
+                                                             const uint32_t* __restrict__ rows_dev,
+                                                             float* __restrict__ logits) {
+  __shared__ uint16_t s_h1[kScoreRows * kH1Stride];
+  __shared__ float s_part[kScoreRows * kPartStride];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lr = lane & 31, kh = 8 * (lane >> 5);
+  const int n = wave * 32 + lr;  // this lane's hidden column
   const int n_rows = rows_of(rows, rows_dev);
   const int n_tiles = (n_rows + kScoreRows - 1) / kScoreRows;
-  uint16_t* h1 = s_h1[wave];
-  const int lr = lane & 31, kh = 8 * (lane >> 5);
-  // block-uniform rounds: every wave reaches every barrier
-  for (int tb = blockIdx.x * 4; tb < n_tiles; tb += gridDim.x * 4) {
-    const int row0 = (tb + wave) * kScoreRows;
-    const bool active = tb + wave < n_tiles;
-    if (active) {
-      // layer 1: A fragments of the wave's 32 rows (zero past the last row)
-      bf16x8 xa[2];
-      const int xr = row0 + lr;
+  if ((int)blockIdx.x >= n_tiles) return;  // (whole workgroup)
+  // the wave's weight slices, loaded once
+  bf16x8 w1f[2], w2f[kMlpHidden / 16];
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        bf16x8 z = {};
-        xa[s] = xr < n_rows ? *reinterpret_cast<const bf16x8*>(X + (size_t)xr * kMlpIn + 16 * s + kh)
-                            : z;
-      }
+  for (int s = 0; s < 2; ++s)
+    w1f[s] = *reinterpret_cast<const bf16x8*>(W1 + (size_t)n * kMlpIn + 16 * s + kh);
 #pragma unroll
-      for (int grp = 0; grp < kMlpHidden / kW2Group; ++grp) {
-        floatx16 acc[4];
+  for (int s = 0; s < kMlpHidden / 16; ++s)
+    w2f[s] = *reinterpret_cast<const bf16x8*>(W2 + (size_t)n * kMlpHidden + 16 * s + kh);
+  const float bias1 = b1[n], bias2 = b2[n], wn = bf2f(w3[n]), bias3 = *b3;
+  // the A fragments of a tile's feature rows, fetched one tile ahead (rows
+  // past the last read the last row; they are zeroed when used)
+  bf16x8 xnext[2];
+  auto load_x = [&](int tile, bf16x8 (&xa)[2]) {
+    const int rr = min(tile * kScoreRows + lr, n_rows - 1);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          acc[t] = floatx16{};
-          const int n = grp * kW2Group + t * 32 + lr;
+    for (int s = 0; s < 2; ++s)
+      xa[s] = *reinterpret_cast<const bf16x8*>(X + (size_t)rr * kMlpIn + 16 * s + kh);
+  };
+  load_x(blockIdx.x, xnext);
+  for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const int row0 = tile * kScoreRows;
+    bf16x8 xa[2] = {xnext[0], xnext[1]};
+    if (row0 + lr >= n_rows) xa[0] = xa[1] = bf16x8{};
+    if (tile + (int)gridDim.x < n_tiles) load_x(tile + gridDim.x, xnext);
+    // layer 1: the tile's 32 rows (zero past the last row) x the wave's columns
+    floatx16 acc = {};
 #pragma unroll
-          for (int s = 0; s < 2; ++s) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(W1 + (size_t)n * kMlpIn + 16 * s + kh);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[s], b, acc[t], 0, 0, 0);
-          }
-        }
+    for (int s = 0; s < 2; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[s], w1f[s], acc, 0, 0, 0);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int n = grp * kW2Group + t * 32 + lr;
-          const float bias = b1[n];
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            const int m = 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
-            float x = acc[t][v] + bias;
-            x = x > 0.0f ? x : 0.0f;
-            h1[m * kH1Stride + n] = f2bf(x);
-          }
-        }
-      }
+    for (int v = 0; v < 16; ++v) {
+      const int m = 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+      float x = acc[v] + bias1;
+      x = x > 0.0f ? x : 0.0f;
+      s_h1[m * kH1Stride + n] = f2bf(x);
     }
-    // layer 2 and the output layer
-    float part[16];
+    __syncthreads();  // H1 complete (and the previous tile's products are consumed)
+    // layer 2: A fragments of every row from LDS, the wave's W2 slice from VGPRs
+    acc = floatx16{};
 #pragma unroll
-    for (int v = 0; v < 16; ++v) part[v] = 0.0f;
-#pragma unroll 1
-    for (int grp = 0; grp < kMlpHidden / kW2Group; ++grp) {
-      __syncthreads();  // H1 written; the previous group's readers are done with s_w2
-      {
-        // all loads of a thread in flight before its LDS stores
-        constexpr int kStage = kW2Group * (kMlpHidden / 8) / 256;
-        bf16x8 buf[kStage];
-#pragma unroll
-        for (int j = 0; j < kStage; ++j) {
-          const int c = threadIdx.x + 256 * j;
-          const int r = c / (kMlpHidden / 8), k = (c % (kMlpHidden / 8)) * 8;
-          buf[j] = *reinterpret_cast<const bf16x8*>(W2 + (size_t)(grp * kW2Group + r) * kMlpHidden + k);
-        }
-#pragma unroll
-        for (int j = 0; j < kStage; ++j) {
-          const int c = threadIdx.x + 256 * j;
-          const int r = c / (kMlpHidden / 8), k = (c % (kMlpHidden / 8)) * 8;
-          *reinterpret_cast<bf16x8*>(s_w2 + r * kH1Stride + k) = buf[j];
-        }
-      }
-      __syncthreads();
-      if (active) {
-        floatx16 acc[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = floatx16{};
-#pragma unroll 4
-        for (int s = 0; s < kMlpHidden / 16; ++s) {
-          const bf16x8 a = *reinterpret_cast<const bf16x8*>(h1 + lr * kH1Stride + 16 * s + kh);
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const bf16x8 b =
-                *reinterpret_cast<const bf16x8*>(s_w2 + (t * 32 + lr) * kH1Stride + 16 * s + kh);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int n = grp * kW2Group + t * 32 + lr;
-          const float bias = b2[n], wn = bf2f(w3[n]);
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            float x = acc[t][v] + bias;
-            x = x > 0.0f ? x : 0.0f;
-            part[v] += bf2f(f2bf(x)) * wn;
-          }
-        }
-      }
+    for (int s = 0; s < kMlpHidden / 16; ++s) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(s_h1 + lr * kH1Stride + 16 * s + kh);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w2f[s], acc, 0, 0, 0);
     }
-    if (active) {
-      // sum each row over the 32 lanes of its half-wave
 #pragma unroll
-      for (int v = 0; v < 16; ++v)
-        for (int o = 1; o < 32; o <<= 1) part[v] += __shfl_xor(part[v], o);
-      if (lr == 0) {
-        const float bb = *b3;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int m = row0 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
-          if (m < n_rows) logits[m] = part[v] + bb;
-        }
-      }
+    for (int v = 0; v < 16; ++v) {
+      const int m = 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+      float x = acc[v] + bias2;
+      x = x > 0.0f ? x : 0.0f;
+      s_part[m * kPartStride + n] = bf2f(f2bf(x)) * wn;
     }
-    __syncthreads();  // the next round rewrites the H1 tiles
+    __syncthreads();  // products complete; H1 may be rewritten
+    // logits: wave w sums rows w, w + 8, ... in k_last_layer's order
+#pragma unroll
+    for (int q = 0; q < kScoreRows / kScoreWaves; ++q) {
+      const int m = wave + q * kScoreWaves;
+      float sum = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kMlpHidden; k += 64) sum += s_part[m * kPartStride + k + lane];
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+      if (lane == 0 && row0 + m < n_rows) logits[row0 + m] = sum + bias3;
+    }
   }
 }
 
@@ -438,9 +397,9 @@ hipError_t mlp_score(const uint16_t* X, const uint16_t* W1, const float* b1, con
                      const uint32_t* rows_dev, float* logits, int max_blocks, hipStream_t s) {
   if (rows == 0) return hipSuccess;
   const int tiles = (rows + kScoreRows - 1) / kScoreRows;
-  const int blocks = max_blocks > 0 ? std::min((tiles + 3) / 4, max_blocks) : (tiles + 3) / 4;
-  hipLaunchKernelGGL(k_mlp_score, dim3(blocks), dim3(256), 0, s, X, W1, b1, W2, b2, w3, b3, rows,
-                     rows_dev, logits);
+  const int blocks = max_blocks > 0 ? std::min(tiles, max_blocks) : tiles;
+  hipLaunchKernelGGL(k_mlp_score, dim3(blocks), dim3(kScoreThreads), 0, s, X, W1, b1, W2, b2, w3,
+                     b3, rows, rows_dev, logits);
   return hipGetLastError();
 }
 

@@ -34,12 +34,14 @@
 namespace gbr {
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
                         uint32_t* counters, unsigned long long* stats, uint32_t* frame,
-                        float* nrm_out, hipStream_t s);
+                        float* nrm_out, const uint32_t* dirty, hipStream_t s);
 hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s);
 int setup_blocks_per_cu(uint32_t lds_bins);
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
                         uint32_t queue_ctr, uint32_t grid, hipStream_t s);
+int raster_blocks_per_cu();
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
+                         uint32_t grid,
                          hipStream_t s);
 hipError_t launch_resolve(const FrameParams& p, const RasterItem* items, uint32_t n,
                           hipStream_t s);
@@ -161,6 +163,7 @@ struct gbr_renderer {
   int device = 0;
   int cu_count = 256;
   int setup_bpc = 4;      // setup workgroups per CU (occupancy with lds_bins)
+  int raster_bpc = 4;     // persistent raster workgroups per CU (occupancy)
   uint32_t lds_bins = 0;  // bins per scene counted in setup's LDS (0: none)
   hipStream_t stream = nullptr;
 
@@ -183,6 +186,8 @@ struct gbr_renderer {
 
   std::vector<Scene> scenes;
   bool structure_dirty = true;
+  bool nodes_dirty = true;  // absolute transforms must be recomputed next frame
+  DevBuf<uint32_t> d_nodes_dirty;  // the flag k_nodes reads, uploaded with the cameras
 
   // frame layout (rebuilt on structural change)
   std::vector<uint32_t> scene_node_base;
@@ -228,6 +233,8 @@ struct gbr_renderer {
   DevBuf<float> d_local, d_abs;
   DevBuf<float> d_cameras, d_unproj;
   DevBuf<uint32_t> d_bin_count, d_bin_entries;
+  DevBuf<uint32_t> d_heavy, d_bin_heavy;  // heavy-bin list and per-bin flags (raster items)
+  DevBuf<uint32_t> d_active;              // bins with phase-2 entries (raster items)
   DevBuf<RasterRec> d_rrecs;
   DevBuf<ResolveRec> d_srecs;
   DevBuf<RasterRec> d_sub_recs;
@@ -399,6 +406,7 @@ gbr_status rebuild_structure(gbr_renderer* r) {
     if (nb <= (uint32_t)kMaxSceneBins) r->lds_bins = std::max(r->lds_bins, nb);
   }
   r->setup_bpc = setup_blocks_per_cu(r->lds_bins);
+  r->raster_bpc = raster_blocks_per_cu();
 
   std::vector<int32_t> parents;
   std::vector<float> local;
@@ -546,6 +554,12 @@ gbr_status rebuild_structure(gbr_renderer* r) {
     HIP_TRY(hipMemsetAsync(r->d_bin_count.ptr, 0, r->d_bin_count.cap * sizeof(uint32_t), s));
   }
   r->n_bins = bins;
+  // heavy-bin list and flags: the raster clears every flag it meets, so they
+  // are zero between phases; cleared here for a new layout
+  TRY(r->d_heavy.reserve(std::max<size_t>(ritems.size(), 1)));
+  TRY(r->d_bin_heavy.reserve(std::max<size_t>(ritems.size(), 1)));
+  TRY(r->d_active.reserve(std::max<size_t>(ritems.size(), 1)));
+  HIP_TRY(hipMemsetAsync(r->d_bin_heavy.ptr, 0, r->d_bin_heavy.cap * sizeof(uint32_t), s));
   TRY(r->d_bin_entries.reserve(std::max<uint64_t>(entries, 1)));
   r->entries_cap = entries;
   TRY(r->d_rrecs.reserve(std::max<uint64_t>(tris, 1)));
@@ -559,6 +573,7 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   TRY(r->d_sub_prims.reserve(r->sub_cap));
   HIP_TRY(hipStreamSynchronize(s));
   r->structure_dirty = false;
+  r->nodes_dirty = true;  // new node arrays: every absolute transform is stale
   return GBR_OK;
 }
 
@@ -568,6 +583,7 @@ gbr_status upload_dirty_transforms(gbr_renderer* r) {
     if (!sc.xform_dirty) continue;
     sc.xform_dirty = false;
     if (sc.transforms.empty()) continue;
+    r->nodes_dirty = true;
     HIP_TRY(hipMemcpyAsync(r->d_local.ptr + (size_t)r->scene_node_base[si] * 16, sc.transforms.data(),
                            sc.transforms.size() * sizeof(Mat4), hipMemcpyHostToDevice, r->stream));
   }
@@ -576,7 +592,9 @@ gbr_status upload_dirty_transforms(gbr_renderer* r) {
 
 gbr_status upload_cameras(gbr_renderer* r) {
   const size_t n = r->scene_count();
-  const size_t need = n * 26;  // projection*view, unprojection, camera position + forward
+  // projection*view, unprojection, camera position + forward, then the
+  // frame's node-recompute flag
+  const size_t need = n * 26 + 1;
   if (r->cam_event_pending) {
     HIP_TRY(hipEventSynchronize(r->cam_event));
     r->cam_event_pending = false;
@@ -600,6 +618,14 @@ gbr_status upload_cameras(gbr_renderer* r) {
   HIP_TRY(hipMemcpyAsync(r->d_unproj.ptr, r->h_cam + n * 16, n * 8, hipMemcpyHostToDevice, r->stream));
   HIP_TRY(hipMemcpyAsync(r->d_caminfo.ptr, r->h_cam + n * 18, n * 32, hipMemcpyHostToDevice,
                          r->stream));
+  {
+    const uint32_t flag = r->nodes_dirty ? 1u : 0u;
+    std::memcpy(r->h_cam + n * 26, &flag, sizeof flag);
+    TRY(r->d_nodes_dirty.reserve(1));
+    HIP_TRY(hipMemcpyAsync(r->d_nodes_dirty.ptr, r->h_cam + n * 26, sizeof flag,
+                           hipMemcpyHostToDevice, r->stream));
+    r->nodes_dirty = false;
+  }
   if (r->cfg.max_light_count) {
     // per scene its first light and light count (at most max_light_count,
     // the count the reference's shaders are compiled for), then the lights
@@ -739,6 +765,10 @@ gbr_status do_draw(gbr_renderer* r) {
     p.scene_lights = r->d_scene_lights.ptr;
   }
   p.bin_count = r->d_bin_count.ptr;
+  p.heavy = r->d_heavy.ptr;
+  p.bin_heavy = r->d_bin_heavy.ptr;
+  p.active = r->d_active.ptr;
+  p.heavy_entries = r->tun.raster_mode == 0 ? (uint32_t)r->tun.heavy_entries : 0u;
   p.bin_entries = r->d_bin_entries.ptr;
   p.rrecs = r->d_rrecs.ptr;
   p.srecs = r->d_srecs.ptr;
@@ -763,6 +793,7 @@ gbr_status do_draw(gbr_renderer* r) {
   p.item_vis = r->d_item_vis.ptr;
   p.frame = r->d_frame.ptr;
   p.proposal_threshold = r->tun.proposal_threshold;
+  p.proposal_history = r->tun.proposal_history;
   p.cam_info = r->d_caminfo.ptr;
   const bool learned = p.cull_mode == kCullLearned;
   if (learned || r->collecting) {
@@ -810,7 +841,7 @@ gbr_status do_draw(gbr_renderer* r) {
   if (!graphable) return enqueue_frame(r, p);
   // the key holds every value a captured launch takes as an argument: the
   // frame parameters, the node buffers, and the scorer's weight buffers
-  constexpr int kExtra = 12;
+  constexpr int kExtra = 13;
   std::vector<uint8_t> key(sizeof p + kExtra * sizeof(uint64_t));
   {
     const uint64_t extra[kExtra] = {
@@ -820,7 +851,8 @@ gbr_status do_draw(gbr_renderer* r) {
         ((uint64_t)r->cu_count << 32) | (uint32_t)r->setup_bpc,
         (uint64_t)(uintptr_t)r->vis.Xi.ptr, (uint64_t)(uintptr_t)r->vis.logits_i.ptr,
         (uint64_t)(uintptr_t)r->vis.w.ptr, (uint64_t)(uintptr_t)r->vis.W1b.ptr,
-        (uint64_t)(uintptr_t)r->vis.W2b.ptr, (uint64_t)(uintptr_t)r->vis.w3b.ptr};
+        (uint64_t)(uintptr_t)r->vis.W2b.ptr, (uint64_t)(uintptr_t)r->vis.w3b.ptr,
+        (uint64_t)(uintptr_t)r->d_nodes_dirty.ptr};
     std::memcpy(key.data(), &p, sizeof p);
     std::memcpy(key.data() + sizeof p, extra, sizeof extra);
   }
@@ -850,10 +882,12 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
   {
     StageTimer t(r, GBR_TIMING_NODES);
     HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, p.counters,
-                         p.stats, p.frame, const_cast<float*>(p.normal_mats), r->stream));
+                         p.stats, p.frame, const_cast<float*>(p.normal_mats),
+                         r->d_nodes_dirty.ptr, r->stream));
   }
   const uint32_t setup_grid = (uint32_t)(r->cu_count * r->setup_bpc);
   const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
+  const uint32_t raster_grid = (uint32_t)(r->cu_count * r->raster_bpc);
   const bool culled = p.cull_mode != kCullNone;
   if (culled || r->collecting) {  // (unculled frames run it for the feature rows only)
     // phase 1: items that pass the frustum test and are proposed visible
@@ -878,7 +912,7 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
   }
   {
     StageTimer t(r, GBR_TIMING_RASTER1);
-    HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 1, r->stream));
+    HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 1, raster_grid, r->stream));
   }
   if (p.cull_mode >= kCullHiZ) {
     // phase 2: candidates not hidden behind the phase-1 depth pyramid
@@ -897,7 +931,7 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
     }
     {
       StageTimer t(r, GBR_TIMING_RASTER2);
-      HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 2, r->stream));
+      HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 2, raster_grid, r->stream));
     }
   }
   {
@@ -1016,6 +1050,7 @@ gbr_status gbr_tuning_init(gbr_tuning* t) {
   t->queue_grain = 4;
   t->lane_pixels = 96;
   t->graph = 1;
+  t->heavy_entries = 512;
   return GBR_OK;
 }
 
@@ -1040,6 +1075,10 @@ static gbr_status validate_tuning(const gbr_tuning* t) {
     return fail(GBR_ERR_ARGUMENT, "tuning: unknown cull_mode");
   if (!std::isfinite(t->proposal_threshold))
     return fail(GBR_ERR_ARGUMENT, "tuning: proposal_threshold must be finite");
+  if (t->proposal_history != 0 && t->proposal_history != 1)
+    return fail(GBR_ERR_ARGUMENT, "tuning: proposal_history must be 0 or 1");
+  if (t->heavy_entries < 0)
+    return fail(GBR_ERR_ARGUMENT, "tuning: heavy_entries must be >= 0");
   return GBR_OK;
 }
 
@@ -1103,11 +1142,13 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   release_graph(r);
   r->d_frame.release();
+  r->d_nodes_dirty.release();
   DevBuf<float>* fb[] = {&r->d_pos_x, &r->d_pos_y, &r->d_pos_z, &r->d_lut, &r->d_local, &r->d_abs,
                          &r->d_cameras, &r->d_unproj, &r->d_depth, &r->d_lin};
   for (auto* b : fb) b->release();
   DevBuf<uint32_t>* ub[] = {&r->d_uv16, &r->d_col8, &r->d_indices, &r->d_bin_count,
-                            &r->d_bin_entries, &r->d_color, &r->d_sem};
+                            &r->d_bin_entries, &r->d_color, &r->d_sem, &r->d_heavy,
+                            &r->d_bin_heavy, &r->d_active};
   for (auto* b : ub) b->release();
   r->d_uv32.release();
   r->d_col32.release();
@@ -1822,9 +1863,13 @@ gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
   const int lanes[] = {32, 64, 96, 128, 192};
   const int grains[] = {1, 2, 4, 8, 16};
   const int modes[] = {0, 1};
+  const int heavies[] = {0, 256, 512, 1024};
+  const float thresholds[] = {-2.0f, -1.0f, 0.0f, 1.0f};
+  // the proposal threshold only matters for the learned proposal
+  const int nparams = best.cull_mode == GBR_CULL_LEARNED ? 7 : 6;
   for (int pass = 0; pass < params->passes; ++pass) {
-    for (int param = 0; param < 5; ++param) {
-      const int n = param == 0 ? 3 : param == 1 ? 4 : param == 2 ? 5 : param == 3 ? 5 : 2;
+    for (int param = 0; param < nparams; ++param) {
+      const int n = param == 0 ? 3 : param == 1 ? 4 : param == 2 ? 5 : param == 3 ? 5 : param == 4 ? 2 : 4;
       for (int k = 0; k < n; ++k) {
         gbr_tuning t = best;
         switch (param) {
@@ -1832,7 +1877,9 @@ gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
           case 1: t.bin_capacity_factor = caps[k]; break;
           case 2: t.lane_pixels = lanes[k]; break;
           case 3: t.queue_grain = grains[k]; break;
-          default: t.raster_mode = modes[k]; break;
+          case 4: t.raster_mode = modes[k]; break;
+          case 5: t.heavy_entries = heavies[k]; break;
+          default: t.proposal_threshold = thresholds[k]; break;
         }
         if (std::memcmp(&t, &best, sizeof t) == 0) continue;
         float ms = 0.0f;

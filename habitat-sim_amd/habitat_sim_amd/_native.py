@@ -53,7 +53,8 @@ class Tuning(ctypes.Structure):
                 ("cull_mode", ctypes.c_int32), ("queue_grain", ctypes.c_int32),
                 ("timing", ctypes.c_int32), ("raster_mode", ctypes.c_int32),
                 ("proposal_threshold", ctypes.c_float), ("lane_pixels", ctypes.c_int32),
-                ("graph", ctypes.c_int32)]
+                ("graph", ctypes.c_int32), ("proposal_history", ctypes.c_int32),
+                ("heavy_entries", ctypes.c_int32)]
 
 
 class AutotuneParams(ctypes.Structure):

@@ -128,6 +128,13 @@ typedef struct gbr_tuning {
                                  captured HIP graph while nothing but cameras and
                                  transforms changed (timing and row collection
                                  draw directly) */
+  int32_t proposal_history;   /* GBR_CULL_LEARNED: 1 = phase 1 also takes the
+                                 chunks visible in the previous frame (camera
+                                 walks), 0 = the model's proposal alone (default;
+                                 the right choice when cameras jump) */
+  int32_t heavy_entries;      /* raster_mode 0: bins that receive at least this many
+                                 entries in a phase are rasterised first (0: bins in
+                                 order); default 512 */
 } gbr_tuning;
 
 /* stages timed when gbr_tuning.timing is set (gbr_frame_timings) */
@@ -382,9 +389,10 @@ gbr_status gbr_frame_visibility(gbr_renderer* r, uint64_t* visible, uint64_t* vi
 /* new: autotuner.  Times gbr_draw() of the current scenes and cameras under
    candidate launch parameters, one parameter at a time (bin size, per-bin
    capacity factor, lane/workgroup raster split, setup claim size, raster
-   mode; each from the best values found so far), and leaves the fastest set
-   (every candidate renders the same pixels).  cull_mode, timing, graph,
-   chunk_triangles and proposal_threshold are kept. */
+   mode, heavy-bin threshold, and with GBR_CULL_LEARNED the proposal
+   threshold; each from the best values found so far), and leaves the
+   fastest set (every candidate renders the same pixels).  cull_mode,
+   timing, graph, chunk_triangles and proposal_history are kept. */
 typedef struct gbr_autotune_params {
   int32_t warmup; /* untimed draws per candidate (>= 1; absorbs rebuilds) */
   int32_t frames; /* timed draws per candidate (>= 1) */
