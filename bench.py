@@ -30,7 +30,6 @@ on this host.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -45,42 +44,72 @@ HBM_PEAK_GBS = 8000.0
 
 
 def camera_walk(spec, steps, seed=7):
-    """Per step and env a view matrix: the env's pose turned by a seeded
-    random yaw walk (10 degree steps) and moved 0.25 m forward, clamped to the
-    apartment."""
+    """Per step and env a view matrix (steps, n, 4, 4): from the env's pose a
+    seeded random walk of 0.25 m forward / 10 degrees left / 10 degrees right
+    steps, clamped to the apartment."""
     import numpy as np
-    from habitat_sim_amd import math3d as m3
+    from habitat_sim_amd import visibility
 
-    rng = np.random.default_rng(seed)
-    n = len(spec.envs)
-    poses = [env.camera_pose.copy() for env in spec.envs]
-    views = np.empty((steps, n, 4, 4), np.float32)
-    for s in range(steps):
-        acts = rng.integers(0, 3, n)
-        for e in range(n):
-            p = poses[e]
-            if acts[e] == 1:
-                p = m3.mul(p, m3.rotation_y(math.radians(10)))
-            elif acts[e] == 2:
-                p = m3.mul(p, m3.rotation_y(math.radians(-10)))
-            else:
-                fwd = -p[2, :3]
-                pos = np.clip(p[3, :3] + 0.25 * fwd, 0.3, 11.7)
-                p = p.copy()
-                p[3, :3] = pos.astype(np.float32)
-                p[3, 1] = 1.5
-            poses[e] = p
-            views[s, e] = m3.inverted_rigid(p)
-    return views
+    return visibility.camera_walk(spec, steps, seed)
+
+
+def host_cpu():
+    """The host's CPU model, logical CPU count and the CPU share this
+    process may use (its affinity mask, capped by OMP_NUM_THREADS when the
+    host sets it: a GPU box grants each GPU a fixed share of its cores)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    share = affinity
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity, "share": share}
+
+
+def llvmpipe_probe():
+    """Whether the reference's CPU path (its batch renderer on Mesa's
+    llvmpipe through EGL) could run on this host: an EGL library, Mesa's
+    EGL vendor library and the habitat_sim module, looked up by name only
+    (nothing is loaded)."""
+    import ctypes.util
+    import glob
+    import importlib.util
+    dirs = ["/usr/lib/x86_64-linux-gnu", "/usr/lib64", "/usr/lib", "/usr/local/lib"]
+    mesa = sorted({os.path.basename(p) for d in dirs for p in glob.glob(os.path.join(d, "libEGL_mesa.so*"))})
+    egl = ctypes.util.find_library("EGL")
+    swrast = sorted({p for d in dirs for p in glob.glob(os.path.join(d, "dri", "swrast_dri.so"))})
+    found = {"libEGL": egl, "libEGL_mesa": mesa or None, "swrast_dri": swrast or None,
+             "habitat_sim": importlib.util.find_spec("habitat_sim") is not None}
+    found["usable"] = bool(egl and mesa and found["habitat_sim"])
+    found["used"] = "none: " + ("the reference's renderer on llvmpipe would be timed here"
+                                if found["usable"] else
+                                "no EGL + llvmpipe + habitat_sim on this host, so the CPU "
+                                "baseline is the oracle (the project's C restatement of the "
+                                "GL pipeline)")
+    return found
 
 
 def cpu_baseline(spec, seconds=12.0, max_seconds=30.0):
-    """Oracle env-frames/s on a sample of the workload on this host."""
+    """Oracle env-frames/s on a sample of the workload on this host, on every
+    CPU of this process's share (OpenMP over environments), with the same
+    planes the GPU writes (colour, window and linear depth, semantic)."""
     import numpy as np
     from habitat_sim_amd import math3d as m3
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    threads = host["share"]
     sample = list(range(min(64, len(spec.envs))))
     o = O.OracleRenderer(spec.tile, spec.grid)
     o.add_file_data(spec.file)
@@ -104,7 +133,8 @@ def cpu_baseline(spec, seconds=12.0, max_seconds=30.0):
             break
     return {"value": frames / el, "unit": "env-frames/s", "cores": threads, "kind": "port",
             "sample": f"{n} of the {len(spec.envs)} apartment envs (~{spec.triangles(0) // 1000}k "
-                      f"tris each), {frames // n} passes, {el:.1f} s, all planes"}
+                      f"tris each), {frames // n} passes, {el:.1f} s, all planes",
+            "host": host, "llvmpipe_probe": llvmpipe_probe()}
 
 
 def copy_probe(nbytes: int) -> None:
@@ -123,27 +153,56 @@ MODES = (("plain", 0), ("frustum", 1), ("hiz", 2), ("learned", 3))
 HEADLINE = "learned"
 
 
-def stage_bytes(stage, spec, st, n_items, n_nodes, learned=False):
-    """Algorithmic bytes a stage is responsible for in one frame (the frozen
-    frame formula split by stage; intermediate buffers count nothing)."""
+def frame_bytes(spec, st, mode, sensors=3):
+    """Algorithmic bytes of one frame by the frozen formula
+
+      B_frame = sum_env [ sum_drawn (12 T + 16 V + 64) + sum_sensor W H 4 + 8 W H ]
+
+    drawn = the work items (chunks) actually set up, T / V their triangles
+    and vertex-range entries; `sensors` planes of 4 B per pixel; 8 B of
+    texture footprint per pixel.  Culled modes add 24 B of chunk bounds per
+    chunk tested; the two-phase modes the depth pyramid, 4/3 W H 4 B written
+    and as much read per env; the learned mode 64 B of features per scored
+    chunk and the model's bf16 weights."""
     w, h = spec.tile
     envs = len(spec.envs)
-    t2, v2 = st["triangles_phase2"], st["vertices_phase2"]
-    t1, v1 = st["triangles_setup"] - t2, st["vertices_setup"] - v2
-    if stage == "setup1":
-        return 12 * t1 + 16 * v1 + 64 * st["items_phase1"]
-    if stage == "setup2":
-        return 12 * t2 + 16 * v2 + 64 * st["items_phase2"]
-    if stage == "cull":  # work item, bounds, transform; learned: feature rows + bf16 weights
-        rows = st.get("items_phase1", 0) + st.get("items_candidates", 0)
-        return n_items * (32 + 24 + 64) + (64 * rows + 2 * 20865 if learned else 0)
-    if stage == "hiz":  # depth pyramid: read the phase-1 depth, write 4/3 of a float per 16 px
-        return envs * (w * h * 8 + w * h * 4 // 12)
-    if stage == "resolve":
-        return envs * (3 * w * h * 4 + 8 * w * h)
-    if stage == "nodes":
-        return n_nodes * 128
-    return 0
+    items = st["items_phase1"] + st["items_phase2"]
+    b = 12 * st["triangles_setup"] + 16 * st["vertices_setup"] + 64 * items
+    b += envs * (sensors * w * h * 4 + 8 * w * h)
+    if mode != "plain":
+        b += 24 * st["work_items"]
+    if mode in ("hiz", "learned"):
+        b += envs * 2 * (4 * w * h * 4 // 3)
+    if mode == "learned":
+        from habitat_sim_amd import _native as N
+        b += 64 * (st["items_phase1"] + st["items_candidates"]) + 2 * N.VIS_PARAMS
+    return int(b)
+
+
+def build_sha():
+    """Hash of the sources libgbr.so is built from: a hardware-counter
+    profile counts only for the build it was taken on."""
+    import hashlib
+    hsh = hashlib.sha256()
+    csrc = os.path.join(ROOT, "habitat-sim_amd", "csrc")
+    for name in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, name), "rb") as f:
+            hsh.update(name.encode() + b"\0" + f.read())
+    with open(os.path.join(ROOT, "include", "gbr.h"), "rb") as f:
+        hsh.update(f.read())
+    return hsh.hexdigest()[:16]
+
+
+def profile_of_this_build(name):
+    """A counter summary under profiles/ if it was taken on this build."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, "missing"
+    with open(path) as f:
+        prof = json.load(f)
+    if prof.get("build_sha") != build_sha():
+        return None, f"profiles/{name} was taken on another build ({prof.get('build_sha')})"
+    return prof, f"profiles/{name}"
 
 
 def run_mode(r, mode, spec, views, proj, warmup, steps):
@@ -201,28 +260,30 @@ def run_mode(r, mode, spec, views, proj, warmup, steps):
     return {"wall": wall, "gpu_ms": gpu_ms, "stages": stages, "stats": stats, "planes": planes,
             "latency_ms": latency, "visibility": visibility,
             "tuning": {k: tuned["best"][k] for k in ("bin_size", "bin_capacity_factor",
-                                                     "lane_pixels", "queue_grain", "raster_mode")},
+                                                     "lane_pixels", "queue_grain", "raster_mode",
+                                                     "heavy_entries", "proposal_threshold")},
             "autotune": {"candidates": tuned["candidates"], "default_ms": tuned["initial_ms"],
                          "tuned_ms": tuned["best_ms"]}}
 
 
 def rollout(args):
-    """Trajectory rollout: `--rollout` envs of the apartment workload with
-    128x128 linear depth + semantic sensors, `--rollout-steps` steps of the
-    seeded random walk (0.25 m forward / 10 degrees left / right, clamped to
-    the apartment), learned proposal with the previous-frame depth pyramid;
-    visibility rows are collected every frame and the proposal MLP is
-    fine-tuned on them every 50 steps.  Prints one JSON line: env-frames/s
+    """Trajectory rollout: `--rollout-envs` envs of the apartment workload
+    with 128x128 linear depth + semantic sensors, `--rollout-steps` steps of
+    the seeded random walk (0.25 m forward / 10 degrees left / right, clamped
+    to the apartment), learned proposal together with the previous frame's
+    visible chunks; visibility rows are collected every frame and the
+    proposal MLP is fine-tuned on them every 50 steps.  Returns env-frames/s
     over the whole rollout (row collection and fine-tuning included, the
     recall readback excluded), p50/p99 frame latency (camera update to the
-    frame's completion) and the proposal's recall (visible work items that
-    phase 1 set up)."""
+    frame's completion), the proposal's recall (visible work items that
+    phase 1 set up) and the pixels in which the last frame differs from a
+    plain render of the same views."""
     import numpy as np
     from habitat_sim_amd import RendererConfiguration, RendererStandalone, scenes, visibility
     from habitat_sim_amd import _native as N
 
     t_setup = time.perf_counter()
-    spec = scenes.apartments(n_envs=args.rollout, tile=128)
+    spec = scenes.apartments(n_envs=args.rollout_envs, tile=128)
     gx, gy = spec.grid
     cfg = RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy))
     r = RendererStandalone(cfg, planes=N.PLANE_LINEAR_DEPTH | N.PLANE_SEMANTIC, top_down=True)
@@ -230,12 +291,13 @@ def rollout(args):
     steps = args.rollout_steps
     views = camera_walk(spec, steps, seed=11)
     proj = np.stack([env.projection for env in spec.envs]).astype(np.float32)
-    print(f"bench: rollout of {len(spec.envs)} envs set up in {time.perf_counter() - t_setup:.0f} s",
+    setup_s = time.perf_counter() - t_setup
+    print(f"bench: rollout of {len(spec.envs)} envs set up in {setup_s:.1f} s",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     initial = visibility.train(r, spec, frames=args.train_frames, steps=args.train_steps, seed=3)
     train_s = time.perf_counter() - t0
-    r.set_tuning(cull_mode=N.CULL_LEARNED)
+    r.set_tuning(cull_mode=N.CULL_LEARNED, proposal_history=1)
     r.update_cameras(0, proj, views[0])
     r.draw()
     r.synchronize()
@@ -255,27 +317,126 @@ def rollout(args):
         if (k + 1) % 50 == 0:
             fits.append(r.vis_fit(steps=20, batch=4096, learning_rate=1e-3, seed=100 + k))
             r.vis_clear_rows()
-            print(f"bench: rollout step {k + 1}, recall {recall[-1]:.3f}, "
-                  f"loss {fits[-1]['loss_last']:.3f}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0 - excluded
+    print(f"bench: rollout {steps} steps in {wall:.1f} s, recall {np.mean(recall):.3f}",
+          file=sys.stderr, flush=True)
+    last = {"linear_depth": r.linear_depth_image(), "semantic": r.semantic_image()}
+    r.set_tuning(cull_mode=N.CULL_NONE)
+    r.draw()
+    plain = {"linear_depth": r.linear_depth_image(), "semantic": r.semantic_image()}
     n = len(spec.envs)
     out = {
-        "metric": "rollout env-frames/s, learned cull with online fine-tuning",
-        "value": n * steps / wall, "unit": "env-frames/s", "n_gpus": 1, "steps": steps,
-        "higher_is_better": True, "data": "synthetic", "dtype": "f32",
-        "config": {"workload": f"apartments-{n}env-rollout-{steps}steps-128x128-depth-semantic",
-                   "envs": n, "resolution": [128, 128],
-                   "triangles_per_frame": int(sum(spec.triangles(e) for e in range(n))),
-                   "sensors": ["linear_depth_f32", "semantic_u32"], "cull": "learned",
-                   "finetune_every": 50},
+        "workload": f"apartments-{n}env-rollout-{steps}steps-128x128-depth-semantic",
+        "value": n * steps / wall, "unit": "env-frames/s", "envs": n, "steps": steps,
+        "triangles_per_frame": int(sum(spec.triangles(e) for e in range(n))),
+        "sensors": ["linear_depth_f32", "semantic_u32"], "cull": "learned+previous-frame",
+        "finetune_every": 50, "setup_s": setup_s,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "proposal_recall": {"mean": float(np.mean(recall)), "min": float(np.min(recall)),
                             "first": recall[0], "last": recall[-1]},
-        "initial_training": {"seconds": train_s, "loss_last": initial["loss_last"]},
+        "initial_training": {"seconds": train_s, "poses": args.train_frames * n,
+                             "loss_last": initial["loss_last"]},
         "fine_tune_loss_last": [f["loss_last"] for f in fits],
+        "diff_pixels_vs_plain_last_frame": diff_pixels(last, plain),
     }
-    print(json.dumps(out), flush=True)
     r.close()
+    return out
+
+
+def learned_proposal(args):
+    """Learned-proposal workload: the uneven batch (`--lp-envs` envs of
+    10k-1M triangles log-uniform, 64/128/256 px viewports, 1-3 sensors) cut
+    into work items of at most 1024 triangles; the visibility model (32 ->
+    256 -> 256 -> 1, camera-pose and chunk-box features) trained on
+    `--lp-train-frames` x envs uniform random poses, then per culling mode
+    `--lp-frames` frames of fresh uniform random poses (every frame a
+    teleport, so the previous frame says nothing).  Reports ms per frame,
+    chunks set up and recall per mode, learned vs hiz (the two-phase method
+    without the model), and the pixels in which each mode's last frame
+    differs from the plain render."""
+    import numpy as np
+    from habitat_sim_amd import RendererConfiguration, RendererStandalone, hip, scenes, visibility
+    from habitat_sim_amd import _native as N
+
+    t_setup = time.perf_counter()
+    spec = scenes.heterogeneous(n_envs=args.lp_envs)
+    gx, gy = spec.grid
+    r = RendererStandalone(RendererConfiguration().set_tile_size_count(spec.tile, (gx, gy)))
+    scenes.populate(r, spec)
+    n = len(spec.envs)
+    proj = np.stack([env.projection for env in spec.envs]).astype(np.float32)
+    rng = np.random.default_rng(2024)
+    frames = [visibility.inverted_rigid(visibility.random_poses(rng, n))
+              for _ in range(args.lp_frames + 2)]
+    setup_s = time.perf_counter() - t_setup
+    t0 = time.perf_counter()
+    fit = visibility.train(r, spec, frames=args.lp_train_frames, steps=args.lp_train_steps,
+                           seed=3, walk_length=1)
+    train_s = time.perf_counter() - t0
+    print(f"bench: learned proposal: {n} uneven envs set up in {setup_s:.1f} s, model trained on "
+          f"{args.lp_train_frames * n} random poses in {train_s:.1f} s", file=sys.stderr, flush=True)
+    stream = r.stream()
+    res, planes = {}, {}
+    for name, mode in MODES:
+        r.set_tuning(cull_mode=mode, proposal_history=0, timing=0)
+        r.update_cameras(0, proj, frames[0])
+        r.autotune(warmup=1, frames=2)
+        for k in range(2):
+            r.update_cameras(0, proj, frames[k])
+            r.draw()
+        r.synchronize()
+        start, stop = hip.Event(), hip.Event()
+        start.record(stream)
+        t1 = time.perf_counter()
+        for k in range(args.lp_frames):
+            r.update_cameras(0, proj, frames[2 + k])
+            r.draw()
+        stop.record(stream)
+        r.synchronize()
+        wall = time.perf_counter() - t1
+        st = r.frame_stats()
+        vis = r.frame_visibility()
+        planes[name] = {k: getattr(r, f"{k}_image")() for k in ("color", "depth", "semantic")}
+        res[name] = {"gpu_ms_per_frame": start.elapsed_ms(stop) / args.lp_frames,
+                     "env_frames_per_s": n * args.lp_frames / wall,
+                     "items_phase1": st["items_phase1"], "items_phase2": st["items_phase2"],
+                     "triangles_setup": st["triangles_setup"], "proposal_recall": vis["recall"],
+                     "diff_pixels_vs_plain": diff_pixels_sets(planes[name], planes["plain"], spec)}
+        print(f"bench: learned proposal workload, {name}: {res[name]['gpu_ms_per_frame']:.3f} ms",
+              file=sys.stderr, flush=True)
+    r.close()
+    return {"workload": f"uneven-{n}env-teleport-64to256px-1to3sensors-1024tri-chunks",
+            "envs": n, "frames": args.lp_frames,
+            "triangles_per_frame": int(sum(spec.triangles(e) for e in range(n))),
+            "work_items": int(st["work_items"]),
+            "training": {"poses": args.lp_train_frames * n, "seconds": train_s,
+                         "steps": args.lp_train_steps, "loss_last": fit["loss_last"]},
+            "modes": res,
+            "learned_vs_hiz": res["hiz"]["gpu_ms_per_frame"] / res["learned"]["gpu_ms_per_frame"],
+            "learned_vs_plain": res["plain"]["gpu_ms_per_frame"] / res["learned"]["gpu_ms_per_frame"]}
+
+
+def diff_pixels_sets(a, b, spec):
+    """Pixels that differ between two framebuffer readbacks of an uneven
+    batch, counting each scene's planes only where its sensor set has them
+    (a plane outside the set holds unspecified values)."""
+    import numpy as np
+    from habitat_sim_amd import _native as N
+    gx, gy = spec.grid
+    tw, th = spec.tile
+    bits = {"color": N.PLANE_COLOR, "depth": N.PLANE_DEPTH, "semantic": N.PLANE_SEMANTIC}
+    sets = np.zeros(gx * gy, np.uint32)
+    sets[:len(spec.envs)] = [e.planes or 0xF for e in spec.envs]
+    total = 0
+    d = np.zeros((gy, gx, th, tw), bool)
+    for k, bit in bits.items():
+        x = a[k] != b[k]
+        if x.ndim == 3:
+            x = x.any(-1)
+        x = x.reshape(gy, th, gx, tw).transpose(0, 2, 1, 3)
+        d |= x & ((sets & bit) != 0).reshape(gy, gx)[:, :, None, None]
+    total = int(d.sum())
+    return total
 
 
 def diff_pixels(a, b):
@@ -302,10 +463,15 @@ def main():
     ap.add_argument("--modes", default=",".join(m for m, _ in MODES),
                     help="culling modes to time, comma-separated (profiling runs time "
                          "fewer; the speedup needs plain and the headline mode)")
-    ap.add_argument("--rollout", type=int, default=0, metavar="ENVS",
-                    help="instead of the benchmark, a trajectory rollout of ENVS envs with "
-                         "online fine-tuning (its own JSON line)")
+    ap.add_argument("--rollout-envs", type=int, default=4096, metavar="ENVS",
+                    help="trajectory rollout of ENVS envs with online fine-tuning (0: skip)")
     ap.add_argument("--rollout-steps", type=int, default=500)
+    ap.add_argument("--lp-envs", type=int, default=1024,
+                    help="learned-proposal workload on the uneven batch of this many envs (0: skip)")
+    ap.add_argument("--lp-train-frames", type=int, default=1024,
+                    help="frames of uniform random poses the proposal model trains on")
+    ap.add_argument("--lp-train-steps", type=int, default=3000)
+    ap.add_argument("--lp-frames", type=int, default=10)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (the reference kernel "
                          "for scaling hardware byte counters) and exit")
@@ -322,9 +488,6 @@ def main():
     __graft_entry__.ensure_built()
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
-        return
-    if args.rollout:
-        rollout(args)
         return
     import numpy as np
     from habitat_sim_amd import RendererConfiguration, RendererStandalone, hip, scenes
@@ -361,8 +524,6 @@ def main():
     base = runs.get("plain")
     headline = HEADLINE if HEADLINE in runs else modes[-1][0]
     head = runs[headline]
-    n_items = head["stats"]["work_items"]
-    n_nodes = sum(2 + len(env.instances) for env in spec.envs)  # roots + children (upper bound)
 
     cull = {}
     for name, _ in modes:
@@ -388,17 +549,12 @@ def main():
     # stream): the frame formula's bytes over the frame's GPU time
     stages = head["stages"]
     dom = max(stages, key=lambda k: stages[k])
-    learned = headline == "learned"
-    bytes_frame = sum(stage_bytes(k, spec, head["stats"], n_items, n_nodes, learned)
-                      for k in stages)
+    bytes_frame = frame_bytes(spec, head["stats"], headline)
     frame_ms = head["gpu_ms"]
     achieved = bytes_frame / (frame_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_apartments512.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        traffic = tj.get("frame_bytes", {}).get(headline)
+    prof, prof_source = profile_of_this_build("counters_apartments512.json")
+    traffic = prof["frame_bytes"].get(headline) if prof else None
+    valu = prof.get("valu_busy", {}).get(headline) if prof else None
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(spec, args.cpu_seconds)
     value = n * args.steps / head["wall"]
@@ -422,8 +578,9 @@ def main():
                    "cull": headline, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": prof_source,
                      "kernel": "frame", "algorithmic_bytes_per_launch": bytes_frame,
-                     "launch_ms": frame_ms,
+                     "launch_ms": frame_ms, "valu_busy": valu,
                      "dominant_stage": {"name": dom, "ms": stages[dom],
                                         "share": stages[dom] / max(sum(stages.values()), 1e-9)},
                      "alu": {"pixels_tested_per_s": head["stats"]["raster_pixels"] / (frame_ms * 1e-3),
@@ -433,14 +590,20 @@ def main():
         "cpu_baseline": cpu,
         "batch_steps_per_s": value / n,
         "learned_cull_speedup": cull["learned"]["speedup_vs_plain"] if "learned" in cull else None,
+        "learned_vs_hiz": ({"learned_ms": cull["learned"]["gpu_ms_per_frame"],
+                            "hiz_ms": cull["hiz"]["gpu_ms_per_frame"],
+                            "speedup": runs["hiz"]["wall"] / runs["learned"]["wall"]}
+                           if "learned" in cull and "hiz" in cull else None),
         "vis_training": training,
         "cull": cull,
         "frame_stats": head["stats"],
     }
     if cpu:
         line["gpu_over_cpu"] = value / cpu["value"]
-    print(json.dumps(line))
     r.close()
+    line["learned_proposal"] = learned_proposal(args) if args.lp_envs else None
+    line["rollout"] = rollout(args) if args.rollout_envs else None
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -15,6 +15,13 @@ Restated from src/esp/gfx_batch/Renderer.cpp (habitat-sim):
     and tile -> scene mapping sceneId = y*tx + x ........... :1298-1303
 and AbstractReplayRenderer::environmentGridSize()
 (src/esp/sim/AbstractReplayRenderer.cpp:26-31).
+
+Parity unpinned: generate_mipmaps(), the stand-in for the GL driver's
+glGenerateMipmap() (Renderer.cpp:476-477,526-527), in how it rounds a 2x2
+mean that falls between two 8-bit values (GL 4.6 section 8.14.4 leaves the
+filter to the implementation).  Its level count and sizes, and its output on
+constant, 2x2-block-constant and linear-ramp images, are pinned by closed
+forms (tests/test_oracle_mip.py).
 """
 from __future__ import annotations
 

@@ -178,3 +178,63 @@ def test_full_size_uneven_batch_1024_envs():
                 continue
             np.testing.assert_array_equal(_tile(plain[name], spec, s), planes[key[name]][k],
                                           err_msg=f"env {s} {name}")
+
+
+def test_full_size_uneven_batch_learned_proposal():
+    """The learned-proposal workload at full size: the uneven batch of 1024
+    envs (10k-1M triangles, 64/128/256 px, 1-3 sensors) in work items of at
+    most 1024 triangles, the pose-feature model trained on uniform random
+    poses, learned mode with graph replay.  On teleported cameras and on the
+    envs' own views every produced plane equals the unculled render, and a
+    sample of every resolution is bit-exact with the oracle."""
+    from habitat_sim_amd import math3d as m3
+    spec = scenes.heterogeneous(n_envs=1024)
+    n = len(spec.envs)
+    plain = _renderer(spec)
+    learned = _renderer(spec)
+    assert learned.tuning().chunk_triangles == 1024
+    rep = visibility.train(learned, spec, frames=16, steps=300, seed=5, walk_length=1)
+    assert rep["loss_last"] < rep["loss_first"]
+    learned.set_tuning(cull_mode=N.CULL_LEARNED, graph=1)
+    assert learned.tuning().graph == 1
+    P = np.stack([e.projection for e in spec.envs])
+    rng = np.random.default_rng(77)
+    frames = [visibility.inverted_rigid(visibility.random_poses(rng, n)) for _ in range(2)]
+    frames.append(np.stack([m3.inverted_rigid(e.camera_pose) for e in spec.envs]))
+    recall = []
+    for k, views in enumerate(frames):
+        for r in (plain, learned):
+            r.update_cameras(0, P, views)
+            r.draw()
+        _same_sets(plain, learned, spec, tag=f"frame {k}")
+        recall.append(learned.frame_visibility()["recall"])
+    st = learned.frame_stats()
+    assert st["items_phase1"] > 0 and st["overflowed_bins"] >= 0
+    print(f"learned proposal, 1024 uneven envs: recall per frame {recall}")
+    final = {name: getattr(learned, f"{name}_image")() for name in PLANE_OF}
+    plain.close()
+    learned.close()
+    rng = np.random.default_rng(31)
+    by_res = {}
+    for s, env in enumerate(spec.envs):
+        by_res.setdefault(env.viewport, []).append(s)
+    sample = sorted(int(rng.choice(v)) for v in by_res.values())
+    gx, gy = spec.grid
+    o = OracleRenderer(spec.tile, (gx, gy))
+    o.add_file_data(spec.file)
+    for s in sample:
+        env = spec.envs[s]
+        for inst in env.instances:
+            o.add_node_hierarchy(s, inst.template, inst.bake, inst.semantic)
+        o.update_camera(s, env.projection, m3.inverted_rigid(env.camera_pose))
+        if env.viewport is not None:
+            o.set_scene_viewport(s, env.viewport)
+    planes = o.render(NTHREADS, scenes=sample)
+    key = {"color": "color", "depth": "depth", "linear_depth": "lindepth", "semantic": "semantic"}
+    for k, s in enumerate(sample):
+        env = spec.envs[s]
+        for name, bit in PLANE_OF.items():
+            if env.planes and not env.planes & bit:
+                continue
+            np.testing.assert_array_equal(_tile(final[name], spec, s), planes[key[name]][k],
+                                          err_msg=f"env {s} {name}")

@@ -9,7 +9,6 @@ import pytest
 
 from habitat_sim_amd import RendererConfiguration, RendererStandalone, scenes
 from habitat_sim_amd import _native as N
-from habitat_sim_amd import math3d as m3
 from habitat_sim_amd import visibility
 from oracle import mlp_ref as M
 
@@ -138,7 +137,7 @@ def test_learned_proposal_without_history(trained):
     rng = np.random.default_rng(1234)
     frames = []
     for _ in range(3):
-        frames.append([m3.inverted_rigid(visibility.random_pose(rng)) for _ in spec.envs])
+        frames.append(visibility.inverted_rigid(visibility.random_poses(rng, len(spec.envs))))
     # each teleport follows a frame at other poses, so history is stale
     res = {}
     for mode in (N.CULL_FRUSTUM, N.CULL_HIZ, N.CULL_LEARNED):
@@ -155,7 +154,13 @@ def test_learned_proposal_without_history(trained):
         res[mode] = (items, planes)
     for k in range(len(frames)):
         np.testing.assert_array_equal(res[N.CULL_LEARNED][1][k], res[N.CULL_FRUSTUM][1][k])
+        np.testing.assert_array_equal(res[N.CULL_HIZ][1][k], res[N.CULL_FRUSTUM][1][k])
+    print(f"work items set up over {len(frames)} teleports: frustum {res[N.CULL_FRUSTUM][0]}, "
+          f"hiz {res[N.CULL_HIZ][0]}, learned {res[N.CULL_LEARNED][0]}")
+    # the model's proposal beats both the method without culling by
+    # occlusion and the two-phase method without the model
     assert res[N.CULL_LEARNED][0] < res[N.CULL_FRUSTUM][0]
+    assert res[N.CULL_LEARNED][0] < res[N.CULL_HIZ][0]
 
 
 def test_online_fine_tuning_walk_stays_exact():
@@ -178,7 +183,7 @@ def test_online_fine_tuning_walk_stays_exact():
     w0 = learned.vis_weights().copy()
 
     def check(k):
-        plain.update_cameras(0, P, np.stack(walk[k]))
+        plain.update_cameras(0, P, walk[k])
         plain.draw()
         for name in ("color", "depth", "semantic"):
             a = getattr(plain, f"{name}_image")()
