@@ -212,6 +212,9 @@ enum : int32_t { kFeaturesPose = 0, kFeaturesHistory = 1 };
 
 // depth pyramid: level 0 = max depth per 8x8 pixels, level 1 per 32x32
 constexpr int kHiZ0 = 3, kHiZ1 = 5;  // log2 block edges
+constexpr int kHiZPer = 1 << (kHiZ1 - kHiZ0);  // level-0 blocks per level-1 block edge
+// blocks a depth-pyramid test may read at level 0 before it uses level 1
+constexpr int kHiZTiles = 16;
 
 
 // One (scene, bin) of the raster and resolve grids, with the scene fields
@@ -302,6 +305,7 @@ struct FrameParams {
   uint32_t* bin_heavy;          // [raster items] 1 = listed (cleared by raster)
   uint32_t* active;             // [raster items] bins with phase-2 entries
   uint32_t heavy_entries;       // 0: no heavy list
+  int32_t fuse_resolve;         // the final raster phase resolves its bins in place
   uint32_t clipped_cap, sub_cap;
   uint32_t queue_grain;         // setup work items fetched per queue atomic
   uint32_t n_items;             // work items of the frame

@@ -20,15 +20,19 @@
 //   k_raster_lane per (scene, bin): one binned triangle per lane, large
 //                 rectangles from an LDS queue by the whole workgroup; depth
 //                 test as a 64-bit LDS atomicMin of (window depth bits,
-//                 primitive order); visibility keys to HBM (k_raster_pairs:
+//                 primitive order); visibility keys to HBM, or, in the
+//                 frame's last raster phase of an unlit renderer, the bin
+//                 resolved in place from its LDS keys (k_raster_pairs:
 //                 selectable pair-parallel form; k_raster_full: exact rescan
 //                 of overflowed bins)
 //   k_hiz, k_cull_hiz  two-phase culling: depth pyramid of the phase-1 keys,
 //                 then phase-2 candidates not hidden by it go through setup
 //                 and raster again
-//   k_resolve     per bin: each pixel once, key -> record -> perspective-
+//   resolve_tile  per bin: each pixel once, key -> record -> perspective-
 //                 correct attributes, mip-mapped texture sampling, RGBA8 /
 //                 window depth / linear depth / semantic written once to HBM
+//                 (inside the last raster phase, or k_resolve from the
+//                 stored keys for lit renderers and k_raster_pairs)
 //   k_gather      framebuffer-layout readback of a rectangle
 //
 // Semantics follow OpenGL as driven by the reference (Renderer.cpp:307-308,
@@ -738,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
               // fragment's lower bound) lies behind the phase-1 pyramid over
               // its rectangle cannot change a pixel
               if (active && hiz_test &&
-                  hiz_hidden(p, wi.scene, pr.x0, pr.y0, pr.x1, pr.y1, min_depth(Z), 16))
+                  hiz_hidden(p, wi.scene, pr.x0, pr.y0, pr.x1, pr.y1, min_depth(Z), kHiZTiles))
                 active = false;
             }
           } else {
@@ -1197,14 +1201,21 @@ __device__ __forceinline__ AxisWrap axis_wrap(int size, int mode) {
   return a;
 }
 
+// min(max(x, 0), hi) in one instruction
+__device__ __forceinline__ int clamp0(int x, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(x), "v"(hi));
+  return r;
+}
+
 __device__ __forceinline__ int wrap_coord(const AxisWrap& a, int i) {
-  if (__builtin_expect(a.fast, 1)) return min(max(i & a.mask, 0), a.size - 1);
+  if (__builtin_expect(a.fast, 1)) return clamp0(i & a.mask, a.size - 1);
   return wrap_coord_slow(i, a.size, a.mode);
 }
 
+// floor of u clamped to [-2^30, 2^30] (the clamp as one median; u is finite)
 __device__ __forceinline__ int floor_to_int(float u) {
-  if (u > 1073741824.0f) u = 1073741824.0f;
-  if (u < -1073741824.0f) u = -1073741824.0f;
+  u = __builtin_amdgcn_fmed3f(u, -1073741824.0f, 1073741824.0f);
   return (int)floorf(u);
 }
 
@@ -1234,8 +1245,11 @@ __device__ __forceinline__ void fetch(const float* lut, const uint8_t* base, uin
   out[3] = lut[texel >> 24];
 }
 
-__device__ __forceinline__ uint32_t texel_row(int layer, int h, int j, int w) {
-  return ((uint32_t)layer * (uint32_t)h + (uint32_t)j) * (uint32_t)w;
+// (layer * h + j) * w mod 2^32 as layer_off + j * w, layer_off = layer * h * w
+// mod 2^32 once per level; j < h and w below 2^24 (checked at import), so
+// the product is a full-rate 24-bit multiply
+__device__ __forceinline__ uint32_t texel_row(uint32_t layer_off, int j, int w) {
+  return layer_off + __umul24((uint32_t)j, (uint32_t)w);
 }
 
 __device__ void sample_level(const float* lut, const TexHeader& t, const uint8_t* base, int level,
@@ -1244,10 +1258,11 @@ __device__ void sample_level(const float* lut, const TexHeader& t, const uint8_t
   const AxisWrap ws = axis_wrap(w, t.wrap_s), wt = axis_wrap(h, t.wrap_t);
   float u = s * (float)w;
   float v = tc * (float)h;
+  const uint32_t layer_off = (uint32_t)layer * (uint32_t)h * (uint32_t)w;
   if (!linear) {
     int i = wrap_coord(ws, floor_to_int(u));
     int j = wrap_coord(wt, floor_to_int(v));
-    fetch(lut, base, texel_row(layer, h, j, w), i, out);
+    fetch(lut, base, texel_row(layer_off, j, w), i, out);
     return;
   }
   float uu = u - 0.5f;
@@ -1260,7 +1275,7 @@ __device__ void sample_level(const float* lut, const TexHeader& t, const uint8_t
   int j1 = wrap_coord(wt, j0 + 1);
   i0 = wrap_coord(ws, i0);
   j0 = wrap_coord(wt, j0);
-  const uint32_t r0 = texel_row(layer, h, j0, w), r1 = texel_row(layer, h, j1, w);
+  const uint32_t r0 = texel_row(layer_off, j0, w), r1 = texel_row(layer_off, j1, w);
   float t00[4], t10[4], t01[4], t11[4];
   fetch(lut, base, r0, i0, t00);
   fetch(lut, base, r0, i1, t10);
@@ -1387,9 +1402,9 @@ __device__ __forceinline__ float interp3(const float* b, float a0, float a1, flo
   return v + v2;
 }
 
+// clamp(c, 0, 1) (NaN to 0) as one median, then round to 8 bits
 __device__ __forceinline__ uint32_t to_unorm8(float c) {
-  if (!(c > 0.0f)) c = 0.0f;
-  if (c > 1.0f) c = 1.0f;
+  c = __builtin_amdgcn_fmed3f(c, 0.0f, 1.0f);
   float v = c * 255.0f;
   v = v + 0.5f;
   return (uint32_t)(int)v;
@@ -1868,29 +1883,32 @@ __device__ __forceinline__ void add_raster_stats(const FrameParams& p, uint32_t 
 }
 
 // Depth pyramid of one 32x32 bin (s_vis rows of kMaxBin keys) from its final
-// phase-1 keys in LDS: max
-// depth per 8x8 pixels (level 0) and over the bin (level 1), the same values
-// k_hiz computes from the stored keys.
+// phase-1 keys in LDS: max depth per level-0 block (2^kHiZ0 pixels square)
+// and over the bin (level 1), the same values k_hiz computes from the
+// stored keys.
 __device__ void bin_hiz(const FrameParams& p, const BinInfo& b, const unsigned long long* s_vis) {
-  __shared__ uint32_t s_max[16];
-  if (threadIdx.x < 16) s_max[threadIdx.x] = 0u;
+  constexpr int kN = kHiZPer * kHiZPer;
+  __shared__ uint32_t s_max[kN];
+  if (threadIdx.x < kN) s_max[threadIdx.x] = 0u;
   __syncthreads();
   const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
   for (int l = threadIdx.x; l < kMaxBin * kMaxBin; l += kBlock) {
     const int lx = l & 31, ly = l >> 5;
     if (lx < bw && ly < bh)
-      atomicMax(&s_max[(ly >> 3) * 4 + (lx >> 3)], (uint32_t)(s_vis[ly * kMaxBin + lx] >> 32));
+      atomicMax(&s_max[(ly >> kHiZ0) * kHiZPer + (lx >> kHiZ0)],
+                (uint32_t)(s_vis[ly * kMaxBin + lx] >> 32));
   }
   __syncthreads();
   const int rx = b.bin.x0 >> kHiZ1, ry = b.bin.y0 >> kHiZ1;
   float* h = p.hiz + (size_t)b.scene * p.hiz_stride;
-  if (threadIdx.x < 16) {
-    const int bx = rx * 4 + (int)(threadIdx.x & 3), by = ry * 4 + (int)(threadIdx.x >> 2);
+  if (threadIdx.x < kN) {
+    const int bx = rx * kHiZPer + (int)(threadIdx.x % kHiZPer),
+              by = ry * kHiZPer + (int)(threadIdx.x / kHiZPer);
     if (bx < p.hiz_w0 && by < p.hiz_h0) h[by * p.hiz_w0 + bx] = __uint_as_float(s_max[threadIdx.x]);
   }
   if (threadIdx.x == 0) {
     uint32_t m = 0;
-    for (int k = 0; k < 16; ++k) m = max(m, s_max[k]);
+    for (int k = 0; k < kN; ++k) m = max(m, s_max[k]);
     h[p.hiz_w0 * p.hiz_h0 + ry * p.hiz_w1 + rx] = __uint_as_float(m);
   }
 }
@@ -1909,9 +1927,26 @@ __device__ void bin_hiz(const FrameParams& p, const BinInfo& b, const unsigned l
 struct RasterLds {
   unsigned long long vis[kMaxBin * kMaxBin];
   BigTri big[kBigQueue];
-  int nbig[2];  // queue fill of even / odd batches
+  float lut[256];  // unorm8 -> float, for resolving in place
+  int nbig[2];     // queue fill of even / odd batches
 };
 
+template <bool kLights>
+__device__ void resolve_tile(const FrameParams& p, const RasterItem& it,
+                             const unsigned long long* tile, int B, const float* s_lut);
+
+// the raster phase after which no pass changes the keys
+__host__ __device__ __forceinline__ bool final_phase(const FrameParams& p, int phase) {
+  return p.cull_mode >= kCullHiZ ? phase == 2 : phase == 1;
+}
+
+// whether the lane rasteriser resolves this phase's bins in place
+__host__ __device__ __forceinline__ bool resolves_in_place(const FrameParams& p, int phase) {
+  return p.fuse_resolve && final_phase(p, phase);
+}
+
+// kResolve: the phase resolves its bins in place (resolves_in_place)
+template <bool kResolve>
 __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int phase, RasterLds& L,
                            uint32_t& pairs, uint32_t& frags) {
   const BinInfo b = bin_info(item);
@@ -1921,7 +1956,9 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
   const uint32_t e_first = threadIdx.x < b.bin_cap ? entries[threadIdx.x] : 0u;
   const uint32_t n = p.bin_count[b.gbin];
   if (n > b.bin_cap) return;  // overflowed: k_raster_full rescans this bin
-  if (phase == 2 && n == 0) return;
+  // with kResolve every bin is resolved here, also those without new
+  // entries; else the keys go to HBM for the next pass
+  if (phase == 2 && n == 0 && !kResolve) return;
   const int B = p.bin_size;
   RasterCtx rc;
   rc.vis = L.vis;
@@ -1984,15 +2021,22 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
     raster_big_all(rc, L.big, min(L.nbig[parity], kBigQueue));
     __syncthreads();
   }
-  store_vis(p, b, L.vis, B);
-  if (phase == 1 && p.hiz_fused) bin_hiz(p, b, L.vis);
+  if (kResolve) {
+    resolve_tile<false>(p, item, L.vis, B, L.lut);
+  } else {
+    store_vis(p, b, L.vis, B);
+    if (phase == 1 && p.hiz_fused) bin_hiz(p, b, L.vis);
+  }
   if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
 }
 
-// at least 5 waves per SIMD (96 VGPRs, no spills): the bins' dependent
-// global loads are latency-bound, and 5 resident workgroups per CU
-// rasterise phase 1 ~8% faster than the 4 an unbounded allocation allows
-__global__ __launch_bounds__(kBlock, 5) void k_raster_lane(FrameParams p,
+// Without resolve, at least 5 waves per SIMD (96 VGPRs, no spills): the
+// bins' dependent global loads are latency-bound, and 5 resident workgroups
+// per CU rasterise phase 1 ~8% faster than the 4 an unbounded allocation
+// allows.  The instance that also resolves runs at 4 (128 VGPRs): at 5 the
+// shading spills to scratch.
+template <bool kResolve>
+__global__ __launch_bounds__(kBlock, kResolve ? 4 : 5) void k_raster_lane(FrameParams p,
                                                         const RasterItem* __restrict__ items,
                                                         uint32_t n_ritems, int phase) {
   __shared__ RasterLds L;
@@ -2000,9 +2044,12 @@ __global__ __launch_bounds__(kBlock, 5) void k_raster_lane(FrameParams p,
   const uint32_t n_heavy =
       min(p.counters[(phase == 2 ? kCtrHeavy2 : kCtrHeavy1) * kCtrStride], n_ritems);
   // phase 1 visits every bin (each gets its keys and pyramid); phase 2 only
-  // the bins that received entries
+  // the bins that received entries, unless it resolves (then every bin)
+  const bool active_only = phase == 2 && !kResolve;
   const uint32_t n_rest =
-      phase == 2 ? min(p.counters[kCtrActive2 * kCtrStride], n_ritems) : n_ritems;
+      active_only ? min(p.counters[kCtrActive2 * kCtrStride], n_ritems) : n_ritems;
+  static_assert(kBlock == 256, "one LUT entry per thread");
+  if (kResolve) L.lut[threadIdx.x] = p.unorm_lut[threadIdx.x];
   uint32_t* const queue = &p.counters[(phase == 2 ? kCtrRaster2 : kCtrRaster1) * kCtrStride];
   uint32_t pairs = 0, frags = 0;
   for (;;) {
@@ -2012,7 +2059,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_raster_lane(FrameParams p,
       if (q < n_heavy) {
         claim = p.heavy[q];
       } else if (q - n_heavy < n_rest) {
-        claim = phase == 2 ? p.active[q - n_heavy] : q - n_heavy;
+        claim = active_only ? p.active[q - n_heavy] : q - n_heavy;
         // a listed bin was (or is being) rasterised from the heavy list
         if (p.bin_heavy[claim]) {
           p.bin_heavy[claim] = 0u;
@@ -2027,7 +2074,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_raster_lane(FrameParams p,
     __syncthreads();  // s_claim is rewritten by the next claim; the LDS tile is free
     if (claim == ~0u) break;
     if (claim & 0x80000000u) continue;
-    raster_bin(p, items[claim], phase, L, pairs, frags);
+    raster_bin<kResolve>(p, items[claim], phase, L, pairs, frags);
   }
   add_raster_stats(p, pairs, frags);
 }
@@ -2155,8 +2202,10 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
   __shared__ int s_nbig;
   __shared__ uint32_t s_over[kFullGroup];
   __shared__ int s_nover;
+  __shared__ float s_lut[256];
 
   if (threadIdx.x == 0) s_nover = 0;
+  if (p.fuse_resolve) s_lut[threadIdx.x] = p.unorm_lut[threadIdx.x];  // (kBlock == 256)
   __syncthreads();
   const uint32_t first = blockIdx.x * kFullGroup;
   if (threadIdx.x < kFullGroup && first + threadIdx.x < n) {
@@ -2199,8 +2248,12 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
       if (threadIdx.x == 0) s_nbig = 0;
       __syncthreads();
     }
-    store_vis(p, b, s_vis, B);
-    if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
+    if (resolves_in_place(p, phase)) {
+      resolve_tile<false>(p, it, s_vis, B, s_lut);
+    } else {
+      store_vis(p, b, s_vis, B);
+      if (phase == 1 && p.hiz_fused) bin_hiz(p, b, s_vis);
+    }
     if (threadIdx.x == 0) {
       add_stat(p.stats, 1, 1ull);
       p.bin_count[b.gbin] = 0;
@@ -2215,16 +2268,14 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
 // linear depth, semantic id; every output written once
 // --------------------------------------------------------------------------
 
-// kLights: the renderer has lights enabled (a separate instance, so that
-// renderers without lights keep the smaller register footprint)
+// The outputs of one bin from its final visibility keys: `tile` holds them
+// in rows of B keys (an LDS tile), s_lut the unorm8 -> float table.  Called
+// by every thread of the workgroup.  kLights: the renderer has lights
+// enabled (a separate instance, so that renderers without lights keep the
+// smaller register footprint).
 template <bool kLights>
-__global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
-                                                    const RasterItem* __restrict__ items) {
-  __shared__ float s_lut[256];
-  static_assert(kBlock >= 256, "one LUT entry per thread");
-  if (threadIdx.x < 256) s_lut[threadIdx.x] = p.unorm_lut[threadIdx.x];
-  __syncthreads();
-  const RasterItem it = items[blockIdx.x];
+__device__ void resolve_tile(const FrameParams& p, const RasterItem& it,
+                             const unsigned long long* tile, int B, const float* s_lut) {
   const BinInfo b = bin_info(it);
   const uint32_t frame = *p.frame;
   const float un0 = p.unprojections[2 * it.scene], un1 = p.unprojections[2 * it.scene + 1];
@@ -2246,10 +2297,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
   for (int base = 0; base < npx; base += kBlock) {
     const int l = base + (int)threadIdx.x;
     const bool valid = l < npx;
-    const int ly = div_small(l, inv_bw);
-    const int px = b.bin.x0 + (l - ly * bw), py = b.bin.y0 + ly;
-    const unsigned long long key =
-        valid ? p.vis[(size_t)b.plane_offset + (size_t)py * b.stride + px] : kClearKey;
+    const int ly = div_small(l, inv_bw), lx = l - ly * bw;
+    const int px = b.bin.x0 + lx, py = b.bin.y0 + ly;
+    const unsigned long long key = valid ? tile[ly * B + lx] : kClearKey;
     const bool drawn = key != kClearKey;
     const uint32_t prim = (uint32_t)key;
     ResolveRec rs;
@@ -2303,8 +2353,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     const float z = __uint_as_float((uint32_t)(key >> 32));
     if (want_color)
-      p.out_color[o] = shade_pixel<kLights>(p, s_lut, hw, hh, ihw, ihh, A, bc, s, t, textured, q, d, prim, it.scene,
-                                    px, py);
+      p.out_color[o] = shade_pixel<kLights>(p, s_lut, hw, hh, ihw, ihh, A, bc, s, t, textured, q, d, prim,
+                                            it.scene, px, py);
     if (want_depth) p.out_depth[o] = z;
     if (want_lin) {
       const float lin = un1 / (z + un0);
@@ -2312,6 +2362,45 @@ __global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
     }
     if (want_sem) p.out_semantic[o] = d.semantic;
   }
+}
+
+// a bin's stored keys into an LDS tile of rows of B keys
+__device__ __forceinline__ void load_vis(const FrameParams& p, const BinInfo& b,
+                                         unsigned long long* tile, int B) {
+  constexpr int kPer = kMaxBin * kMaxBin / kBlock;
+  const int bw = b.bin.x1 - b.bin.x0 + 1, bh = b.bin.y1 - b.bin.y0 + 1;
+  unsigned long long keys[kPer];
+  const float inv_bw = 1.0f / (float)bw;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int l = threadIdx.x + q * kBlock;
+    const int ly = div_small(l, inv_bw), lx = l - ly * bw;
+    keys[q] = l < bw * bh
+                  ? p.vis[(size_t)b.plane_offset + (size_t)(b.bin.y0 + ly) * b.stride + b.bin.x0 + lx]
+                  : 0ull;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int l = threadIdx.x + q * kBlock;
+    const int ly = div_small(l, inv_bw), lx = l - ly * bw;
+    if (l < bw * bh) tile[ly * B + lx] = keys[q];
+  }
+}
+
+// k_resolve: per bin, the stored visibility keys -> colour, window depth,
+// linear depth, semantic id (the renderers whose final raster pass does not
+// resolve in place: lights, the pair rasteriser)
+template <bool kLights>
+__global__ __launch_bounds__(kBlock) void k_resolve(FrameParams p,
+                                                    const RasterItem* __restrict__ items) {
+  __shared__ float s_lut[256];
+  __shared__ unsigned long long s_tile[kMaxBin * kMaxBin];
+  static_assert(kBlock >= 256, "one LUT entry per thread");
+  if (threadIdx.x < 256) s_lut[threadIdx.x] = p.unorm_lut[threadIdx.x];
+  const RasterItem it = items[blockIdx.x];
+  load_vis(p, bin_info(it), s_tile, p.bin_size);
+  __syncthreads();
+  resolve_tile<kLights>(p, it, s_tile, p.bin_size, s_lut);
 }
 
 // --------------------------------------------------------------------------
@@ -2707,12 +2796,13 @@ __global__ __launch_bounds__(kBlock) void k_collect(FrameParams p, uint16_t* __r
 // depth pyramid from the phase-1 visibility keys: one workgroup per 32x32
 // region of a scene
 __global__ __launch_bounds__(kBlock) void k_hiz(FrameParams p, uint32_t regions_per_scene) {
-  __shared__ uint32_t s_max[16];
+  constexpr int kN = kHiZPer * kHiZPer;
+  __shared__ uint32_t s_max[kN];
   const uint32_t scene = blockIdx.x / regions_per_scene;
   const uint32_t reg = blockIdx.x % regions_per_scene;
   const SceneDesc sc = p.scenes[scene];
   const int rx = (int)(reg % (uint32_t)p.hiz_w1), ry = (int)(reg / (uint32_t)p.hiz_w1);
-  if (threadIdx.x < 16) s_max[threadIdx.x] = 0u;
+  if (threadIdx.x < kN) s_max[threadIdx.x] = 0u;
   __syncthreads();
   const int lx = threadIdx.x & 31;
   const int px = rx * 32 + lx;
@@ -2722,18 +2812,19 @@ __global__ __launch_bounds__(kBlock) void k_hiz(FrameParams p, uint32_t regions_
     const int py = ry * 32 + ly;
     if (px < sc.width && py < sc.height) {
       const unsigned long long key = p.vis[(size_t)sc.plane_offset + (size_t)py * sc.stride + px];
-      atomicMax(&s_max[(ly >> 3) * 4 + (lx >> 3)], (uint32_t)(key >> 32));  // depth >= 0
+      atomicMax(&s_max[(ly >> kHiZ0) * kHiZPer + (lx >> kHiZ0)], (uint32_t)(key >> 32));  // depth >= 0
     }
   }
   __syncthreads();
   float* h = p.hiz + (size_t)scene * p.hiz_stride;
-  if (threadIdx.x < 16) {
-    const int bx = rx * 4 + (int)(threadIdx.x & 3), by = ry * 4 + (int)(threadIdx.x >> 2);
+  if (threadIdx.x < kN) {
+    const int bx = rx * kHiZPer + (int)(threadIdx.x % kHiZPer),
+              by = ry * kHiZPer + (int)(threadIdx.x / kHiZPer);
     if (bx < p.hiz_w0 && by < p.hiz_h0) h[by * p.hiz_w0 + bx] = __uint_as_float(s_max[threadIdx.x]);
   }
   if (threadIdx.x == 0) {
     uint32_t m = 0;
-    for (int k = 0; k < 16; ++k) m = max(m, s_max[k]);
+    for (int k = 0; k < kN; ++k) m = max(m, s_max[k]);
     h[p.hiz_w0 * p.hiz_h0 + ry * p.hiz_w1 + rx] = __uint_as_float(m);
   }
 }
@@ -2748,7 +2839,7 @@ __global__ __launch_bounds__(kListBlock) void k_cull_hiz(FrameParams p) {
   if (i < n) {
     item = p.cand[i];
     const ItemCull ic = p.cull[item];
-    keep = !hiz_hidden(p, ic.pad[0], ic.x0, ic.y0, ic.x1, ic.y1, ic.zmin, 16);
+    keep = !hiz_hidden(p, ic.pad[0], ic.x0, ic.y0, ic.x1, ic.y1, ic.zmin, kHiZTiles);
   }
   uint32_t* const ctr[1] = {&p.counters[kCtrList2 * kCtrStride]};
   const bool flag[1] = {keep};
@@ -2829,10 +2920,12 @@ hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream
   return hipGetLastError();
 }
 
-int raster_blocks_per_cu() {
+int raster_blocks_per_cu(bool resolve) {
   int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane, kBlock, 0) != hipSuccess || b < 1)
-    b = 4;
+  const hipError_t e =
+      resolve ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane<true>, kBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane<false>, kBlock, 0);
+  if (e != hipSuccess || b < 1) b = 4;
   return b;
 }
 
@@ -2843,9 +2936,12 @@ hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t
   if (n == 0) return hipSuccess;
   if (p.raster_mode == 1)
     hipLaunchKernelGGL(k_raster_pairs, dim3(n), dim3(kBlock), 0, s, p, items, phase);
+  else if (resolves_in_place(p, phase))
+    hipLaunchKernelGGL(k_raster_lane<true>, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items,
+                       n, phase);
   else
-    hipLaunchKernelGGL(k_raster_lane, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items, n,
-                       phase);
+    hipLaunchKernelGGL(k_raster_lane<false>, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items,
+                       n, phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_raster_full, dim3((n + kFullGroup - 1) / kFullGroup), dim3(kBlock), 0, s, p,

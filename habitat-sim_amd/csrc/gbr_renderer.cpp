@@ -39,7 +39,7 @@ hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream
 int setup_blocks_per_cu(uint32_t lds_bins);
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
                         uint32_t queue_ctr, uint32_t grid, hipStream_t s);
-int raster_blocks_per_cu();
+int raster_blocks_per_cu(bool resolve);
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
                          uint32_t grid,
                          hipStream_t s);
@@ -163,7 +163,7 @@ struct gbr_renderer {
   int device = 0;
   int cu_count = 256;
   int setup_bpc = 4;      // setup workgroups per CU (occupancy with lds_bins)
-  int raster_bpc = 4;     // persistent raster workgroups per CU (occupancy)
+  int raster_bpc[2] = {4, 4};  // persistent raster workgroups per CU (occupancy), without / with resolve
   uint32_t lds_bins = 0;  // bins per scene counted in setup's LDS (0: none)
   hipStream_t stream = nullptr;
 
@@ -406,7 +406,8 @@ gbr_status rebuild_structure(gbr_renderer* r) {
     if (nb <= (uint32_t)kMaxSceneBins) r->lds_bins = std::max(r->lds_bins, nb);
   }
   r->setup_bpc = setup_blocks_per_cu(r->lds_bins);
-  r->raster_bpc = raster_blocks_per_cu();
+  r->raster_bpc[0] = raster_blocks_per_cu(false);
+  r->raster_bpc[1] = raster_blocks_per_cu(true);
 
   std::vector<int32_t> parents;
   std::vector<float> local;
@@ -769,6 +770,9 @@ gbr_status do_draw(gbr_renderer* r) {
   p.bin_heavy = r->d_bin_heavy.ptr;
   p.active = r->d_active.ptr;
   p.heavy_entries = r->tun.raster_mode == 0 ? (uint32_t)r->tun.heavy_entries : 0u;
+  // the lane rasteriser resolves its final phase in place (lit renderers
+  // keep the separate resolve and its larger register footprint)
+  p.fuse_resolve = (r->tun.raster_mode == 0 && r->cfg.max_light_count == 0) ? 1 : 0;
   p.bin_entries = r->d_bin_entries.ptr;
   p.rrecs = r->d_rrecs.ptr;
   p.srecs = r->d_srecs.ptr;
@@ -807,8 +811,8 @@ gbr_status do_draw(gbr_renderer* r) {
   p.lane_pixels = r->tun.lane_pixels;
   {
     const int W = r->cfg.tile_size[0], H = r->cfg.tile_size[1];
-    p.hiz_w0 = (W + 7) >> 3;
-    p.hiz_h0 = (H + 7) >> 3;
+    p.hiz_w0 = (W + (1 << kHiZ0) - 1) >> kHiZ0;
+    p.hiz_h0 = (H + (1 << kHiZ0) - 1) >> kHiZ0;
     p.hiz_w1 = (W + 31) >> 5;
     p.hiz_h1 = (H + 31) >> 5;
     p.hiz_stride = (uint32_t)(p.hiz_w0 * p.hiz_h0 + p.hiz_w1 * p.hiz_h1);
@@ -887,7 +891,10 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
   }
   const uint32_t setup_grid = (uint32_t)(r->cu_count * r->setup_bpc);
   const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
-  const uint32_t raster_grid = (uint32_t)(r->cu_count * r->raster_bpc);
+  // the lane rasteriser's phase that resolves in place runs at lower occupancy
+  const bool fuse2 = p.fuse_resolve && p.cull_mode >= kCullHiZ;
+  const uint32_t raster_grid1 = (uint32_t)(r->cu_count * r->raster_bpc[p.fuse_resolve && !fuse2]);
+  const uint32_t raster_grid2 = (uint32_t)(r->cu_count * r->raster_bpc[fuse2]);
   const bool culled = p.cull_mode != kCullNone;
   if (culled || r->collecting) {  // (unculled frames run it for the feature rows only)
     // phase 1: items that pass the frustum test and are proposed visible
@@ -912,7 +919,7 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
   }
   {
     StageTimer t(r, GBR_TIMING_RASTER1);
-    HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 1, raster_grid, r->stream));
+    HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 1, raster_grid1, r->stream));
   }
   if (p.cull_mode >= kCullHiZ) {
     // phase 2: candidates not hidden behind the phase-1 depth pyramid
@@ -931,10 +938,10 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
     }
     {
       StageTimer t(r, GBR_TIMING_RASTER2);
-      HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 2, raster_grid, r->stream));
+      HIP_TRY(launch_raster(p, r->d_ritems.ptr, r->n_ritems, 2, raster_grid2, r->stream));
     }
   }
-  {
+  if (!p.fuse_resolve) {
     StageTimer t(r, GBR_TIMING_RESOLVE);
     HIP_TRY(launch_resolve(p, r->d_ritems.ptr, r->n_ritems, r->stream));
   }
@@ -1247,7 +1254,10 @@ gbr_status gbr_add_file_data(gbr_renderer* r, const gbr_file_desc* f, uint32_t f
       const gbr_texture_desc& t = f->textures[i];
       if (!t.texels || t.width < 1 || t.height < 1 || t.layers < 1 || t.levels < 1 || t.levels > 16)
         return fail(GBR_ERR_IO, "addFile(): can't import texture " + std::to_string(i) + " of " + fname);
-      // the kernels index a level's texels with 32 bits
+      // the kernels index a level's texels with 32 bits (rows with 24-bit products)
+      if (t.width >= (1u << 24) || t.height >= (1u << 24))
+        return fail(GBR_ERR_SIZE, "addFile(): texture " + std::to_string(i) + " of " + fname +
+                                      " is 2^24 or more texels wide or high");
       if ((uint64_t)t.width * t.height * t.layers >= (1ull << 32))
         return fail(GBR_ERR_SIZE, "addFile(): texture " + std::to_string(i) + " of " + fname +
                                       " has 2^32 or more texels per level");
