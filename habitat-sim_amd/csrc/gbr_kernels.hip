@@ -905,78 +905,69 @@ __device__ __forceinline__ void make_big(const RasterCtx& rc, const int32_t* X, 
   b.narrow = narrow ? 1 : 0;
 }
 
-// The queued triangles, by the whole workgroup.  Each thread owns the pixels
-// (lx, ly0 + k * step), k < kOwn, of the bin: it loads their keys from the
-// LDS tile, keeps the nearest in registers across the queued triangles and
-// stores them back once.  While this runs no lane writes the tile (the
-// callers put barriers around it) and each pixel has one owner, so neither
-// side needs atomics.  The edge values are stepped by additions (exact:
-// 32-bit when narrow).
-constexpr int kOwn = kMaxBin * kMaxBin / kBlock;
-
-__device__ void raster_big_all(const RasterCtx& rc, const BigTri* big, int nb) {
-  const int B = rc.B;
-  const int lx = (int)threadIdx.x % B, ly0 = (int)threadIdx.x / B, step = kBlock / B;
+// The queued triangles, a wave at a time each: a wave claims the next
+// triangle (LDS counter `next`) and spreads its pixel rectangle over its
+// lanes, pixel p = lane + 64 i of the rectangle in row-major order; each
+// lane evaluates the edge functions at its pixel directly (exact: 32-bit
+// when narrow) and merges a covered pixel with an LDS atomicMin, so the
+// waves need no ownership of pixels and a triangle costs
+// ceil(pixels / 64) wave steps.
+__device__ void raster_big_waves(const RasterCtx& rc, const BigTri* big, int nb, int* next) {
+  const int lane = (int)threadIdx.x & 63;
   if (threadIdx.x == 0)
     for (int q = 0; q < nb; ++q) *rc.pairs += big[q].npx;
-  if (ly0 >= B || nb == 0) return;  // (8-pixel bins: the first 64 threads cover the bin)
-  unsigned long long keys[kOwn];
-#pragma unroll
-  for (int k = 0; k < kOwn; ++k) {
-    const int ly = ly0 + k * step;
-    keys[k] = ly < B ? rc.vis[ly * B + lx] : 0ull;
-  }
-  const int32_t Px = (rc.bin.x0 + lx) * 256 + 128;
-  for (int q = 0; q < nb; ++q) {
+  for (;;) {
+    int q = 0;
+    if (lane == 0) q = atomicAdd(next, 1);
+    q = __builtin_amdgcn_readfirstlane(q);
+    if (q >= nb) break;
     const BigTri& t = big[q];
-    if (lx < t.x0 || lx > t.x1) continue;
-    const int ylo = t.y0, yhi = t.y1;
+    const int x0 = t.x0, y0 = t.y0, w = t.x1 - t.x0 + 1, npx = t.npx;
+    const float inv_w = 1.0f / (float)w;
+    const int32_t X0 = t.X0, Y0 = t.Y0;
+    const float z0 = t.z0, dzdx = t.dzdx, dzdy = t.dzdy, zlo = t.zlo;
+    const uint32_t prim = t.prim;
     if (t.narrow) {
-      uint32_t e[3], d[3];
+      uint32_t e[3], sx[3], sy[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        e[i] = (uint32_t)t.e[i] + (uint32_t)t.sx[i] * (uint32_t)lx + (uint32_t)t.sy[i] * (uint32_t)ly0;
-        d[i] = (uint32_t)t.sy[i] * (uint32_t)step;
+        e[i] = (uint32_t)t.e[i];
+        sx[i] = (uint32_t)t.sx[i];
+        sy[i] = (uint32_t)t.sy[i];
       }
+      for (int pi = lane; pi < npx; pi += 64) {
+        const int ry = div_small(pi, inv_w);
+        const int lx = x0 + (pi - ry * w), ly = y0 + ry;
+        uint32_t v[3];
 #pragma unroll
-      for (int k = 0; k < kOwn; ++k) {
-        const int ly = ly0 + k * step;
-        if (ly < B && ly >= ylo && ly <= yhi && (int32_t)(e[0] | e[1] | e[2]) >= 0) {
+        for (int i = 0; i < 3; ++i) v[i] = e[i] + sx[i] * (uint32_t)lx + sy[i] * (uint32_t)ly;
+        if ((int32_t)(v[0] | v[1] | v[2]) >= 0) {
           ++*rc.frags;
-          const float z = depth_at(t.z0, t.dzdx, t.dzdy, t.zlo, t.X0, t.Y0, Px,
+          const float z = depth_at(z0, dzdx, dzdy, zlo, X0, Y0, (rc.bin.x0 + lx) * 256 + 128,
                                    (rc.bin.y0 + ly) * 256 + 128);
-          const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | t.prim;
-          if (z < 1.0f && key < keys[k]) keys[k] = key;
+          if (z < 1.0f)
+            atomicMin(&rc.vis[ly * rc.B + lx],
+                      ((unsigned long long)__float_as_uint(z) << 32) | prim);
         }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) e[i] += d[i];
       }
     } else {
-      long long e[3], d[3];
+      for (int pi = lane; pi < npx; pi += 64) {
+        const int ry = div_small(pi, inv_w);
+        const int lx = x0 + (pi - ry * w), ly = y0 + ry;
+        long long v[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        e[i] = t.e[i] + (long long)t.sx[i] * lx + (long long)t.sy[i] * ly0;
-        d[i] = (long long)t.sy[i] * step;
-      }
-#pragma unroll
-      for (int k = 0; k < kOwn; ++k) {
-        const int ly = ly0 + k * step;
-        if (ly < B && ly >= ylo && ly <= yhi && (e[0] | e[1] | e[2]) >= 0) {
+        for (int i = 0; i < 3; ++i)
+          v[i] = t.e[i] + (long long)t.sx[i] * lx + (long long)t.sy[i] * ly;
+        if ((v[0] | v[1] | v[2]) >= 0) {
           ++*rc.frags;
-          const float z = depth_at(t.z0, t.dzdx, t.dzdy, t.zlo, t.X0, t.Y0, Px,
+          const float z = depth_at(z0, dzdx, dzdy, zlo, X0, Y0, (rc.bin.x0 + lx) * 256 + 128,
                                    (rc.bin.y0 + ly) * 256 + 128);
-          const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | t.prim;
-          if (z < 1.0f && key < keys[k]) keys[k] = key;
+          if (z < 1.0f)
+            atomicMin(&rc.vis[ly * rc.B + lx],
+                      ((unsigned long long)__float_as_uint(z) << 32) | prim);
         }
-#pragma unroll
-        for (int i = 0; i < 3; ++i) e[i] += d[i];
       }
     }
-  }
-#pragma unroll
-  for (int k = 0; k < kOwn; ++k) {
-    const int ly = ly0 + k * step;
-    if (ly < B) rc.vis[ly * B + lx] = keys[k];
   }
 }
 
@@ -1929,6 +1920,7 @@ struct RasterLds {
   BigTri big[kBigQueue];
   float lut[256];  // unorm8 -> float, for resolving in place
   int nbig[2];     // queue fill of even / odd batches
+  int next[2];     // next queue entry to claim, even / odd batches
 };
 
 template <bool kLights>
@@ -1994,7 +1986,7 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
   } else {
     for (int i = threadIdx.x; i < B * B; i += kBlock) L.vis[i] = kClearKey;
   }
-  if (threadIdx.x == 0) L.nbig[0] = L.nbig[1] = 0;
+  if (threadIdx.x == 0) L.nbig[0] = L.nbig[1] = L.next[0] = L.next[1] = 0;
   __syncthreads();
   // per batch of kBlock entries: lanes append large rectangles to the queue
   // (fill counter of this batch's parity; the other one, read before the
@@ -2004,7 +1996,7 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
   for (uint32_t base = 0; base < n; base += kBlock, parity ^= 1) {
     const uint32_t k = base + threadIdx.x;
     rc.nbig = &L.nbig[parity];
-    if (threadIdx.x == 0) L.nbig[parity ^ 1] = 0;
+    if (threadIdx.x == 0) L.nbig[parity ^ 1] = L.next[parity ^ 1] = 0;
     if (k < n) {
       int32_t X[3], Y[3];
       float z0, dzdx, dzdy, zlo;
@@ -2018,7 +2010,7 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
       raster_planed(rc, X, Y, z0, dzdx, dzdy, zlo, prim);
     }
     __syncthreads();
-    raster_big_all(rc, L.big, min(L.nbig[parity], kBigQueue));
+    raster_big_waves(rc, L.big, min(L.nbig[parity], kBigQueue), &L.next[parity]);
     __syncthreads();
   }
   if (kResolve) {
@@ -2199,7 +2191,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
   __shared__ unsigned long long s_vis[kMaxBin * kMaxBin];
   __shared__ BigTri s_big[kBigQueue];
   __shared__ uint32_t s_base[kDrawCache];
-  __shared__ int s_nbig;
+  __shared__ int s_nbig, s_next;
   __shared__ uint32_t s_over[kFullGroup];
   __shared__ int s_nover;
   __shared__ float s_lut[256];
@@ -2235,7 +2227,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
     if (sc.draw_count <= (uint32_t)kDrawCache)
       for (uint32_t i = threadIdx.x; i < sc.draw_count; i += kBlock)
         s_base[i] = p.draws[sc.draw_begin + i].prim_base;
-    if (threadIdx.x == 0) s_nbig = 0;
+    if (threadIdx.x == 0) s_nbig = s_next = 0;
     float PV[16];
     load_mat(p.cameras + (size_t)it.scene * 16, PV);
     __syncthreads();
@@ -2243,9 +2235,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
       const uint32_t k = base + threadIdx.x;
       if (k < sc.prim_count) raster_prim(p, sc, PV, rc, s_base, k);
       __syncthreads();
-      raster_big_all(rc, s_big, min(s_nbig, kBigQueue));
+      raster_big_waves(rc, s_big, min(s_nbig, kBigQueue), &s_next);
       __syncthreads();
-      if (threadIdx.x == 0) s_nbig = 0;
+      if (threadIdx.x == 0) s_nbig = s_next = 0;
       __syncthreads();
     }
     if (resolves_in_place(p, phase)) {
