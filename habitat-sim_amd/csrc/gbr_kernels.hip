@@ -90,6 +90,15 @@ __device__ __forceinline__ float4 vertex_clip(const float* M, const float* PV, f
   return make_float4(c[0], c[1], c[2], c[3]);
 }
 
+// a * b for operands that fit 24 signed bits, as one full-rate
+// v_mul_i32_i24 (the compiler would otherwise emit the quarter-rate 32-bit
+// multiply where it cannot prove the ranges)
+__device__ __forceinline__ int32_t imul24(int32_t a, int32_t b) {
+  int32_t r;
+  asm("v_mul_i32_i24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // l / n for 0 <= l < 4096 and 1 <= n <= 64 from a float reciprocal
 // inv = 1/n: (l + 0.5) / n lies at least 0.5 / n >= 2^-7 away from an
 // integer and the float product errs by less than 2^-10 there, so the
@@ -834,7 +843,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
 // down; narrow = every value over the bin (and a bin past it) fits in 32 bits.
 struct BigTri {
   long long e[3];
-  int32_t sx[3], sy[3];  // -dy * 256, dx * 256
+  int32_t dx[3], dy[3];  // edge vectors (|d| < 2^23); per pixel right -dy * 256, down dx * 256
   float z0, dzdx, dzdy, zlo;
   int32_t X0, Y0;        // vertex 0, the depth plane's origin
   uint32_t prim;
@@ -868,7 +877,7 @@ __device__ __forceinline__ void depth_key_min(const RasterCtx& rc, float z0, flo
                            py * 256 + 128);
   if (z < 1.0f) {
     const unsigned long long key = ((unsigned long long)__float_as_uint(z) << 32) | prim;
-    atomicMin(&rc.vis[(py - rc.bin.y0) * rc.B + (px - rc.bin.x0)], key);
+    atomicMin(&rc.vis[__mul24(py - rc.bin.y0, rc.B) + (px - rc.bin.x0)], key);
   }
 }
 
@@ -888,8 +897,8 @@ __device__ __forceinline__ void make_big(const RasterCtx& rc, const int32_t* X, 
     // window coordinates stay within the guard band (|X|, |Y| < 2^22), so
     // |d| * 256 < 2^31
     b.e[i] = e;
-    b.sx[i] = -dy * 256;
-    b.sy[i] = dx * 256;
+    b.dx[i] = dx;
+    b.dy[i] = dy;
     const long long reach = (llabs((long long)dx) + llabs((long long)dy)) * 256 * 64;
     narrow = narrow && llabs(e) + reach < (1ll << 31);
   }
@@ -927,45 +936,41 @@ __device__ void raster_big_waves(const RasterCtx& rc, const BigTri* big, int nb,
     const int32_t X0 = t.X0, Y0 = t.Y0;
     const float z0 = t.z0, dzdx = t.dzdx, dzdy = t.dzdy, zlo = t.zlo;
     const uint32_t prim = t.prim;
-    if (t.narrow) {
-      uint32_t e[3], sx[3], sy[3];
+    int32_t dx[3], dy[3];
+    long long e[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        e[i] = (uint32_t)t.e[i];
-        sx[i] = (uint32_t)t.sx[i];
-        sy[i] = (uint32_t)t.sy[i];
-      }
-      for (int pi = lane; pi < npx; pi += 64) {
-        const int ry = div_small(pi, inv_w);
-        const int lx = x0 + (pi - ry * w), ly = y0 + ry;
+    for (int i = 0; i < 3; ++i) {
+      dx[i] = t.dx[i];
+      dy[i] = t.dy[i];
+      e[i] = t.e[i];
+    }
+    // integer products below are of a pixel offset (< 64) and an edge
+    // vector component (|d| < 2^23) or a bin dimension: 24-bit multiplies
+    const bool narrow = t.narrow;
+    for (int pi = lane; pi < npx; pi += 64) {
+      const int ry = div_small(pi, inv_w);
+      const int lx = x0 + (pi - __mul24(ry, w)), ly = y0 + ry;
+      bool in;
+      if (narrow) {
         uint32_t v[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) v[i] = e[i] + sx[i] * (uint32_t)lx + sy[i] * (uint32_t)ly;
-        if ((int32_t)(v[0] | v[1] | v[2]) >= 0) {
-          ++*rc.frags;
-          const float z = depth_at(z0, dzdx, dzdy, zlo, X0, Y0, (rc.bin.x0 + lx) * 256 + 128,
-                                   (rc.bin.y0 + ly) * 256 + 128);
-          if (z < 1.0f)
-            atomicMin(&rc.vis[ly * rc.B + lx],
-                      ((unsigned long long)__float_as_uint(z) << 32) | prim);
-        }
-      }
-    } else {
-      for (int pi = lane; pi < npx; pi += 64) {
-        const int ry = div_small(pi, inv_w);
-        const int lx = x0 + (pi - ry * w), ly = y0 + ry;
+        for (int i = 0; i < 3; ++i)
+          v[i] = (uint32_t)e[i] + (uint32_t)(imul24(dx[i], ly) - imul24(dy[i], lx)) * 256u;
+        in = (int32_t)(v[0] | v[1] | v[2]) >= 0;
+      } else {
         long long v[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-          v[i] = t.e[i] + (long long)t.sx[i] * lx + (long long)t.sy[i] * ly;
-        if ((v[0] | v[1] | v[2]) >= 0) {
-          ++*rc.frags;
-          const float z = depth_at(z0, dzdx, dzdy, zlo, X0, Y0, (rc.bin.x0 + lx) * 256 + 128,
-                                   (rc.bin.y0 + ly) * 256 + 128);
-          if (z < 1.0f)
-            atomicMin(&rc.vis[ly * rc.B + lx],
-                      ((unsigned long long)__float_as_uint(z) << 32) | prim);
-        }
+          v[i] = e[i] + (long long)(imul24(dx[i], ly) - imul24(dy[i], lx)) * 256;
+        in = (v[0] | v[1] | v[2]) >= 0;
+      }
+      if (in) {
+        ++*rc.frags;
+        const float z = depth_at(z0, dzdx, dzdy, zlo, X0, Y0, ((rc.bin.x0 + lx) << 8) | 128,
+                                 ((rc.bin.y0 + ly) << 8) | 128);
+        if (z < 1.0f)
+          atomicMin(&rc.vis[__mul24(ly, rc.B) + lx],
+                    ((unsigned long long)__float_as_uint(z) << 32) | prim);
       }
     }
   }
