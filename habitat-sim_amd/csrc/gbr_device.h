@@ -163,12 +163,7 @@ enum : uint32_t {
   kCtrList2 = 6,     // work items set up in phase 2
   kCtrClipMark = 7,  // clipped primitives handled by the phase-1 clip pass
   kCtrProp = 8,      // work items with a visibility-feature row this frame
-  kCtrHeavy1 = 9,    // bins listed as heavy by phase-1 setup (raster takes them first)
-  kCtrHeavy2 = 10,   // the same for phase 2
-  kCtrRaster1 = 11,  // raster work queue of phase 1 (claims by persistent workgroups)
-  kCtrRaster2 = 12,  // and of phase 2
-  kCtrActive2 = 13,  // bins that received entries in phase 2 (the only ones its raster visits)
-  kCtrCount = 14
+  kCtrCount = 9
 };
 constexpr uint32_t kCtrStride = 32;
 // The setup work queue of each phase is split into kQueueStripes contiguous
@@ -177,7 +172,20 @@ constexpr uint32_t kCtrStride = 32;
 // thousands of claims would cost more than the claims save.
 constexpr uint32_t kQueueStripes = 16;
 constexpr uint32_t kCtrStripes = kCtrCount;                       // [phase][stripe]
-constexpr uint32_t kCtrTotal = kCtrStripes + 2 * kQueueStripes;   // counters reset per frame
+// The raster queue of each phase is split into kRasterParts contiguous
+// ranges of the bins (raster items), one per XCD: a persistent workgroup
+// drains the range of its own XCD (blockIdx mod 8) first, so that the bins
+// of a range, whose scenes share triangle records, are rasterised out of
+// one L2, and then moves on to the other ranges.  Per phase and range:
+// bins listed as heavy by setup (taken first), bins with phase-2 entries,
+// and the claim counter.
+constexpr uint32_t kRasterParts = 8;
+enum : uint32_t { kPartHeavy = 0, kPartActive = 1, kPartClaim = 2, kPartKinds = 3 };
+constexpr uint32_t kCtrRasterParts = kCtrStripes + 2 * kQueueStripes;  // [phase][kind][range]
+__host__ __device__ constexpr uint32_t raster_ctr(int phase, uint32_t kind, uint32_t part) {
+  return kCtrRasterParts + ((uint32_t)(phase - 1) * kPartKinds + kind) * kRasterParts + part;
+}
+constexpr uint32_t kCtrTotal = kCtrRasterParts + 2 * kPartKinds * kRasterParts;  // reset per frame
 
 // frame statistics (64-bit): bin entries, overflowed bins, then triangles and
 // vertex-range entries set up in phase 1, the same for phase 2, then pixels
@@ -301,9 +309,11 @@ struct FrameParams {
   // heavy bins (entries >= heavy_entries in a phase): listed by setup when
   // a bin's count crosses the threshold, flagged per bin; the persistent
   // raster takes the list first (longest first), then every other bin
-  uint32_t* heavy;              // [raster items] bins listed this phase
+  uint32_t* heavy;              // [raster items] bins listed this phase, by range
   uint32_t* bin_heavy;          // [raster items] 1 = listed (cleared by raster)
-  uint32_t* active;             // [raster items] bins with phase-2 entries
+  uint32_t* active;             // [raster items] bins with phase-2 entries, by range
+  uint32_t n_ritems;            // raster items (bins of every scene)
+  uint32_t part_bins;           // bins per raster queue range: ceil(n_ritems / kRasterParts)
   uint32_t heavy_entries;       // 0: no heavy list
   int32_t fuse_resolve;         // the final raster phase resolves its bins in place
   uint32_t clipped_cap, sub_cap;

@@ -553,13 +553,17 @@ __device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counte
 // per list and no list exceeds the raster items).
 __device__ __forceinline__ void note_heavy(const FrameParams& p, uint32_t gbin, uint32_t old,
                                            uint32_t n, int phase) {
-  if (phase == 2 && old == 0u && n != 0u)
-    p.active[atomicAdd(&p.counters[kCtrActive2 * kCtrStride], 1u)] = gbin;
+  // the bin's raster queue range and that range's section of the lists
+  const bool active = phase == 2 && old == 0u && n != 0u;
   const uint32_t t = p.heavy_entries;
-  if (t != 0u && old < t && n >= t - old) {
-    const uint32_t slot =
-        atomicAdd(&p.counters[(phase == 2 ? kCtrHeavy2 : kCtrHeavy1) * kCtrStride], 1u);
-    p.heavy[slot] = gbin;
+  const bool heavy = t != 0u && old < t && n >= t - old;
+  if (!active && !heavy) return;
+  const uint32_t part = gbin / p.part_bins, first = part * p.part_bins;
+  if (active)
+    p.active[first + atomicAdd(&p.counters[raster_ctr(2, kPartActive, part) * kCtrStride], 1u)] = gbin;
+  if (heavy) {
+    p.heavy[first + atomicAdd(&p.counters[raster_ctr(phase, kPartHeavy, part) * kCtrStride], 1u)] =
+        gbin;
     p.bin_heavy[gbin] = 1u;
   }
 }
@@ -2038,29 +2042,39 @@ __global__ __launch_bounds__(kBlock, kResolve ? 4 : 5) void k_raster_lane(FrameP
                                                         uint32_t n_ritems, int phase) {
   __shared__ RasterLds L;
   __shared__ uint32_t s_claim;
-  const uint32_t n_heavy =
-      min(p.counters[(phase == 2 ? kCtrHeavy2 : kCtrHeavy1) * kCtrStride], n_ritems);
   // phase 1 visits every bin (each gets its keys and pyramid); phase 2 only
   // the bins that received entries, unless it resolves (then every bin)
   const bool active_only = phase == 2 && !kResolve;
-  const uint32_t n_rest =
-      active_only ? min(p.counters[kCtrActive2 * kCtrStride], n_ritems) : n_ritems;
   static_assert(kBlock == 256, "one LUT entry per thread");
   if (kResolve) L.lut[threadIdx.x] = p.unorm_lut[threadIdx.x];
-  uint32_t* const queue = &p.counters[(phase == 2 ? kCtrRaster2 : kCtrRaster1) * kCtrStride];
   uint32_t pairs = 0, frags = 0;
+  // thread 0's position in the queue ranges: its own XCD's range first
+  // (workgroups are dispatched to the XCDs round robin), then the next ones
+  const uint32_t home = blockIdx.x % kRasterParts;
+  uint32_t tried = 0;
   for (;;) {
     if (threadIdx.x == 0) {
-      const uint32_t q = atomicAdd(queue, 1u);
-      uint32_t claim = ~0u;  // the queue is drained
-      if (q < n_heavy) {
-        claim = p.heavy[q];
-      } else if (q - n_heavy < n_rest) {
-        claim = active_only ? p.active[q - n_heavy] : q - n_heavy;
-        // a listed bin was (or is being) rasterised from the heavy list
-        if (p.bin_heavy[claim]) {
-          p.bin_heavy[claim] = 0u;
-          claim |= 0x80000000u;
+      uint32_t claim = ~0u;  // every range is drained
+      for (; tried < kRasterParts; ++tried) {
+        const uint32_t part = (home + tried) % kRasterParts;
+        const uint32_t first = part * p.part_bins;
+        const uint32_t size = min(first + p.part_bins, n_ritems) - min(first, n_ritems);
+        const uint32_t q = atomicAdd(&p.counters[raster_ctr(phase, kPartClaim, part) * kCtrStride], 1u);
+        const uint32_t n_heavy = min(p.counters[raster_ctr(phase, kPartHeavy, part) * kCtrStride], size);
+        const uint32_t n_rest =
+            active_only ? min(p.counters[raster_ctr(2, kPartActive, part) * kCtrStride], size) : size;
+        if (q < n_heavy) {
+          claim = p.heavy[first + q];
+          break;
+        }
+        if (q - n_heavy < n_rest) {
+          claim = active_only ? p.active[first + q - n_heavy] : first + q - n_heavy;
+          // a listed bin was (or is being) rasterised from the heavy list
+          if (p.bin_heavy[claim]) {
+            p.bin_heavy[claim] = 0u;
+            claim |= 0x80000000u;
+          }
+          break;
         }
       }
       s_claim = claim;

@@ -769,6 +769,8 @@ gbr_status do_draw(gbr_renderer* r) {
   p.heavy = r->d_heavy.ptr;
   p.bin_heavy = r->d_bin_heavy.ptr;
   p.active = r->d_active.ptr;
+  p.n_ritems = r->n_ritems;
+  p.part_bins = std::max<uint32_t>((r->n_ritems + kRasterParts - 1) / kRasterParts, 1u);
   p.heavy_entries = r->tun.raster_mode == 0 ? (uint32_t)r->tun.heavy_entries : 0u;
   // the lane rasteriser resolves its final phase in place (lit renderers
   // keep the separate resolve and its larger register footprint)
