@@ -145,10 +145,20 @@ __host__ __device__ inline void unpack_xy(const uint32_t* xy, int32_t* X, int32_
 
 // A primitive that needs clipping (crosses the near/far plane or the guard band).
 struct alignas(16) PrimRef {
-  uint32_t draw;  // global draw index
-  uint32_t prim;  // scene-local primitive order
-  uint32_t item;  // work item (chunk) the primitive belongs to
-  uint32_t pad;
+  uint32_t draw;   // global draw index
+  uint32_t prim;   // scene-local primitive order
+  uint32_t item;   // work item (chunk) the primitive belongs to
+  uint32_t fast;   // 1: setup wrote the resolve record and the ClipFast entry of the same slot
+};
+
+// Beside the first clip_fast_cap list slots: what the clip pass needs of a
+// primitive whose vertices were in setup's LDS cache, so that it reads the
+// scene, the node transform and the three positions all at once instead of
+// through the draw and the index buffer.
+struct alignas(16) ClipFast {
+  uint32_t scene, node;
+  uint32_t v[3];   // vertex indices
+  uint32_t pad[3];
 };
 
 // device-side counters, reset by k_nodes at the start of every frame; each
@@ -304,6 +314,7 @@ struct FrameParams {
   RasterRec* sub_recs;          // clipped fan triangles
   uint32_t* sub_prims;          // their primitive order
   PrimRef* clipped;             // primitives that need clipping
+  ClipFast* clip_fast;          // [clip_fast_cap] their scene, node and vertices
   uint32_t* counters;           // kCtr*
   unsigned long long* vis;      // per-pixel visibility keys, scene planes
   // heavy bins (entries >= heavy_entries in a phase): listed by setup when
@@ -316,7 +327,7 @@ struct FrameParams {
   uint32_t part_bins;           // bins per raster queue range: ceil(n_ritems / kRasterParts)
   uint32_t heavy_entries;       // 0: no heavy list
   int32_t fuse_resolve;         // the final raster phase resolves its bins in place
-  uint32_t clipped_cap, sub_cap;
+  uint32_t clipped_cap, sub_cap, clip_fast_cap;
   uint32_t queue_grain;         // setup work items fetched per queue atomic
   uint32_t n_items;             // work items of the frame
   // culling

@@ -532,7 +532,7 @@ __device__ __forceinline__ T uniform_load(const T* ptr, uint32_t i) {
 }
 
 __device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counter, bool flag,
-                                           PrimRef ref, PrimRef* list, uint32_t cap, int lane,
+                                           PrimRef ref, const ClipFast& fast, int lane,
                                            unsigned long long lt_mask) {
   const unsigned long long m = __ballot(flag);
   if (m == 0ull) return;
@@ -542,7 +542,11 @@ __device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counte
   base = __shfl(base, leader);
   if (flag) {
     const uint32_t slot = base + (uint32_t)__popcll(m & lt_mask);
-    if (slot < cap) list[slot] = ref;
+    if (ref.fast && slot < p.clip_fast_cap)
+      p.clip_fast[slot] = fast;
+    else
+      ref.fast = 0u;
+    if (slot < p.clipped_cap) p.clipped[slot] = ref;
   }
 }
 
@@ -582,7 +586,9 @@ __device__ __forceinline__ void note_heavy(const FrameParams& p, uint32_t gbin, 
 //     the bin; 4. entries are scattered to base + local slot
 // --------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
+// at least 5 waves per SIMD (96 VGPRs): one register more costs a fifth
+// of the resident waves and ~15% of the setup time
+__global__ __launch_bounds__(kBlock, 5) void k_setup_bin(FrameParams p,
                                                       const uint32_t* __restrict__ list,
                                                       uint32_t list_ctr, uint32_t queue_ctr) {
   __shared__ int32_t s_x[kVertexCache];
@@ -769,6 +775,10 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
           rr.z[1] = Z[1];
           rr.z[2] = Z[2];
           p.rrecs[(size_t)sc.rec_base + prim] = rr;
+        }
+        // the resolve record, also of a primitive the clip pass takes (its
+        // fan triangles carry their own raster records)
+        if (active || clip_me) {
           ResolveRec rec;
           const uint32_t l0 = vi[0] - wi.vmin, l1 = vi[1] - wi.vmin, l2 = vi[2] - wi.vmin;
           {
@@ -782,6 +792,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
           rec.s2 = s_s[l2]; rec.t2 = s_t[l2];
           rec.item = item;
           p.srecs[(size_t)sc.rec_base + prim] = rec;
+        }
+        if (active) {
           // count into the LDS per-bin counters; pairs beyond the LDS list
           // go straight to the global bin
           for (int by = pr.y0 >> sh; by <= (pr.y1 >> sh); ++by)
@@ -800,8 +812,11 @@ __global__ __launch_bounds__(kBlock) void k_setup_bin(FrameParams p,
             }
         }
       }
-      append_ref(p, kCtrClipped * kCtrStride, clip_me, PrimRef{wi.draw, prim, item, 0u}, p.clipped,
-                 p.clipped_cap, lane, lt_mask);
+      // (a primitive of a cached item has its resolve record written above
+      // and its vertex indices in ix)
+      append_ref(p, kCtrClipped * kCtrStride, clip_me, PrimRef{wi.draw, prim, item, cached ? 1u : 0u},
+                 ClipFast{wi.scene, wi.node, {ix[k][0], ix[k][1], ix[k][2]}, {0u, 0u, 0u}}, lane,
+                 lt_mask);
     }
     have_pre = prefetch;
     __syncthreads();
@@ -1591,11 +1606,35 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
 // each binned on its own (entry = kSubTri | record).
 // --------------------------------------------------------------------------
 
-// The bins of a rectangle in groups of 8 whose counter atomics are all in
-// flight together (each returns the entry's slot; taken one at a time, their
-// round trips add up along a thread).
-__device__ __forceinline__ void bin_rect_atomic(const FrameParams& p, const SceneDesc& sc,
-                                                const PixRect& r, uint32_t entry, int phase) {
+
+// forces the exact rescan for every bin the rectangle touches
+__device__ __forceinline__ void overflow_rect(const FrameParams& p, const SceneDesc& sc,
+                                              const PixRect& r) {
+  const int sh = p.bin_shift;
+  for (int by = r.y0 >> sh; by <= (r.y1 >> sh); ++by)
+    for (int bx = r.x0 >> sh; bx <= (r.x1 >> sh); ++bx)
+      atomicAdd(&p.bin_count[sc.bin_base + (uint32_t)(by * sc.bins_x + bx)], sc.bin_cap + 1u);
+}
+
+// phase 1 takes the clipped list from its start and leaves its end in
+// kCtrClipMark; phase 2 continues from there.  One wave per workgroup, one
+// thread per primitive; its polygon and projected fan vertices live in LDS.
+// The wave's fan triangles are then binned by all of its lanes, one fan
+// each, so that a primitive with several fans does not bin them in turn.
+constexpr int kClipBlock = 64;
+constexpr int kClipFans = kClipBlock * (kMaxPoly - 2);  // fan triangles of one wave at most
+
+// the scene fields the binning of a fan triangle needs
+struct ClipBins {
+  int32_t bins_x;
+  uint32_t bin_base, bin_cap, pad;
+  unsigned long long entry_base;
+};
+
+// the bins of a fan triangle's rectangle, in groups of 8 whose counter
+// atomics are all in flight together (each returns the entry's slot)
+__device__ __forceinline__ void bin_fan(const FrameParams& p, const ClipBins& sc, const PixRect& r,
+                                        uint32_t entry, int phase) {
   const int sh = p.bin_shift;
   const int bx0 = r.x0 >> sh, by0 = r.y0 >> sh;
   const int nx = (r.x1 >> sh) - bx0 + 1, nb = nx * ((r.y1 >> sh) - by0 + 1);
@@ -1615,68 +1654,100 @@ __device__ __forceinline__ void bin_rect_atomic(const FrameParams& p, const Scen
   }
 }
 
-// forces the exact rescan for every bin the rectangle touches
-__device__ __forceinline__ void overflow_rect(const FrameParams& p, const SceneDesc& sc,
-                                              const PixRect& r) {
-  const int sh = p.bin_shift;
-  for (int by = r.y0 >> sh; by <= (r.y1 >> sh); ++by)
-    for (int bx = r.x0 >> sh; bx <= (r.x1 >> sh); ++bx)
-      atomicAdd(&p.bin_count[sc.bin_base + (uint32_t)(by * sc.bins_x + bx)], sc.bin_cap + 1u);
-}
-
-// phase 1 takes the clipped list from its start and leaves its end in
-// kCtrClipMark; phase 2 continues from there.  One thread per primitive;
-// its polygon and projected fan vertices live in LDS.
-constexpr int kClipBlock = 64;
-
 __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int phase) {
   __shared__ float4 s_poly[2][kMaxPoly * kClipBlock];
   __shared__ int32_t s_px[kMaxPoly * kClipBlock], s_py[kMaxPoly * kClipBlock];
   __shared__ float s_pz[kMaxPoly * kClipBlock];
+  __shared__ PixRect s_frect[kClipFans];  // the wave's fan triangles: pixel rectangle,
+  __shared__ uint32_t s_fentry[kClipFans];  // bin entry (~0u: no record slot left),
+  __shared__ uint8_t s_fown[kClipFans];     // and the lane whose primitive it is
+  __shared__ ClipBins s_bins[kClipBlock];
+  const int tid = threadIdx.x;
+  // phase 1 reads its first references beside the list length (the slots
+  // exist whatever the length; the ones past it are not used)
+  const uint32_t i_first = blockIdx.x * kClipBlock + tid;
+  PrimRef ref_first = {};
+  ClipFast cf_first = {};
+  if (phase == 1) {
+    if (i_first < p.clipped_cap) ref_first = p.clipped[i_first];
+    if (i_first < p.clip_fast_cap) cf_first = p.clip_fast[i_first];
+  }
   const uint32_t n = min(p.counters[kCtrClipped * kCtrStride], p.clipped_cap);
   const uint32_t start = phase == 1 ? 0u : p.counters[kCtrClipMark * kCtrStride];
   if (phase == 1 && blockIdx.x == 0 && threadIdx.x == 0) p.counters[kCtrClipMark * kCtrStride] = n;
-  const int tid = threadIdx.x;
   int32_t* PX = s_px + tid;
   int32_t* PY = s_py + tid;
   float* PZ = s_pz + tid;
-  // wave-uniform trip count: the fan triangles of all lanes take their
-  // records with one counter atomic per wave
+  // wave-uniform trip count.  The fan triangles' records take fixed slots:
+  // the (kMaxPoly - 2) of list slot i start at (kMaxPoly - 2) * i, so no
+  // counter is waited for (slots past sub_cap fall back to the exact rescan)
   for (uint32_t base = start + blockIdx.x * kClipBlock; base < n; base += gridDim.x * kClipBlock) {
     const uint32_t i = base + tid;
     uint32_t fans = 0;  // bit q: fan triangle (0, q, q+1) is front-facing and covers a pixel
     PrimRef ref = {};
-    DrawDesc d = {};
     SceneDesc sc = {};
-    float4 c[3] = {};
-    const uint32_t* ip = nullptr;
     if (i < n) {
-      ref = p.clipped[i];
-      d = p.draws[ref.draw];
-      sc = p.scenes[d.scene];
+      // the reference and its fast entry (the entry is used only when the
+      // reference says it was written)
+      const bool first = phase == 1 && i == i_first;
+      ref = first ? ref_first : p.clipped[i];
+      ClipFast cf = first ? cf_first : ClipFast{};
+      if (!first && i < p.clip_fast_cap) cf = p.clip_fast[i];
+      const bool full = ref.fast == 0u;  // indices and resolve record left to this pass
+      DrawDesc d = {};
+      uint32_t scene = cf.scene, node = cf.node;
+      uint32_t v[3] = {cf.v[0], cf.v[1], cf.v[2]};
+      if (full) {
+        d = p.draws[ref.draw];
+        scene = d.scene;
+        node = d.node;
+        const uint32_t* ip = p.indices + d.index_base + 3 * (ref.prim - d.prim_base);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v[q] = ip[q];
+      }
+      sc = p.scenes[scene];
       float M[16], PV[16];
-      load_mat(p.abs_nodes + (size_t)d.node * 16, M);
-      load_mat(p.cameras + (size_t)d.scene * 16, PV);
+      load_mat(p.abs_nodes + (size_t)node * 16, M);
+      load_mat(p.cameras + (size_t)scene * 16, PV);
       const float hw = (float)sc.width * 0.5f, hh = (float)sc.height * 0.5f;
       const float gx = kGuardPx / hw, gy = kGuardPx / hh;
-      ip = p.indices + d.index_base + 3 * (ref.prim - d.prim_base);
       float4* poly = s_poly[0] + tid;
+      float4 c[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const uint32_t v = ip[q];
-        c[q] = vertex_clip(M, PV, p.pos_x[v], p.pos_y[v], p.pos_z[v]);
+        c[q] = vertex_clip(M, PV, p.pos_x[v[q]], p.pos_y[v[q]], p.pos_z[v[q]]);
         poly[q * kClipBlock] = c[q];
+      }
+      if (full) {
+        // resolve record of the primitive (its fan triangles carry their own
+        // raster records)
+        const MaterialDesc& mat = p.materials[d.material];
+        float tm[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) tm[q] = mat.texmat[q];
+        ResolveRec rec;
+        rec.item = ref.item;
+        {
+          const float cx[3] = {c[0].x, c[1].x, c[2].x};
+          const float cy[3] = {c[0].y, c[1].y, c[2].y};
+          const float cw[3] = {c[0].w, c[1].w, c[2].w};
+          bary_rows_of(cx, cy, cw, rec.e);
+        }
+        vertex_uv(p, d.flags, v[0], tm, rec.s0, rec.t0);
+        vertex_uv(p, d.flags, v[1], tm, rec.s1, rec.t1);
+        vertex_uv(p, d.flags, v[2], tm, rec.s2, rec.t2);
+        p.srecs[(size_t)sc.rec_base + ref.prim] = rec;
       }
       const int nv = clip_poly_lds<kClipBlock>(poly, s_poly[1] + tid, 3, gx, gy, &poly);
       bool ok = nv >= 3;
       for (int q = 0; q < nv && ok; ++q) {
-        const float4 v = poly[q * kClipBlock];
-        if (!(v.w > 0.0f)) {
+        const float4 pv = poly[q * kClipBlock];
+        if (!(pv.w > 0.0f)) {
           ok = false;
         } else {
           int32_t x, y;
           float z;
-          project(v, hw, hh, x, y, z);
+          project(pv, hw, hh, x, y, z);
           PX[q * kClipBlock] = x;
           PY[q * kClipBlock] = y;
           PZ[q * kClipBlock] = z;
@@ -1693,6 +1764,7 @@ __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int ph
         fans |= 1u << q;
       }
     }
+    s_bins[tid] = ClipBins{sc.bins_x, sc.bin_base, sc.bin_cap, 0u, sc.entry_base};
     // exclusive wave prefix of the fan counts, one atomic for the wave
     const uint32_t cnt = (uint32_t)__popc(fans);
     uint32_t incl = cnt;
@@ -1702,17 +1774,19 @@ __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int ph
       if (tid >= o) incl += t;
     }
     const uint32_t total = __shfl(incl, 63);
-    uint32_t wbase = 0;
-    if (tid == 63 && total) wbase = atomicAdd(&p.counters[kCtrSubTris * kCtrStride], total);
-    uint32_t slot = __shfl(wbase, 63) + incl - cnt;
+    if (tid == 63 && total) atomicAdd(&p.counters[kCtrSubTris * kCtrStride], total);  // (statistics)
+    uint32_t k = incl - cnt;  // this lane's first fan within the wave
+    unsigned long long slot = (unsigned long long)(kMaxPoly - 2) * base + k;
     if (fans) {
       const PixRect tile = {0, 0, sc.width - 1, sc.height - 1};
-      for (uint32_t f = fans; f; f &= f - 1u, ++slot) {
+      for (uint32_t f = fans; f; f &= f - 1u, ++slot, ++k) {
         const int q = __ffs(f) - 1;
         const int a = q * kClipBlock, b = (q + 1) * kClipBlock;
         const int32_t tx[3] = {PX[0], PX[a], PX[b]};
         const int32_t ty[3] = {PY[0], PY[a], PY[b]};
         const PixRect r = tri_pixels(tx, ty, tile);
+        s_frect[k] = r;
+        s_fown[k] = (uint8_t)tid;
         if (slot < p.sub_cap) {
           RasterRec rr;
           pack_xy(tx, ty, rr.xy);
@@ -1721,30 +1795,20 @@ __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int ph
           rr.z[2] = PZ[b];
           p.sub_recs[slot] = rr;
           p.sub_prims[slot] = ref.prim;
-          bin_rect_atomic(p, sc, r, kSubTri | slot, phase);
+          s_fentry[k] = kSubTri | (uint32_t)slot;
         } else {
+          s_fentry[k] = ~0u;
           overflow_rect(p, sc, r);
         }
       }
-      // resolve record of the primitive (its fan triangles carry their own
-      // raster records)
-      const MaterialDesc& mat = p.materials[d.material];
-      float tm[9];
-#pragma unroll
-      for (int q = 0; q < 9; ++q) tm[q] = mat.texmat[q];
-      ResolveRec rec;
-      rec.item = ref.item;
-      {
-        const float cx[3] = {c[0].x, c[1].x, c[2].x};
-        const float cy[3] = {c[0].y, c[1].y, c[2].y};
-        const float cw[3] = {c[0].w, c[1].w, c[2].w};
-        bary_rows_of(cx, cy, cw, rec.e);
-      }
-      vertex_uv(p, d.flags, ip[0], tm, rec.s0, rec.t0);
-      vertex_uv(p, d.flags, ip[1], tm, rec.s1, rec.t1);
-      vertex_uv(p, d.flags, ip[2], tm, rec.s2, rec.t2);
-      p.srecs[(size_t)sc.rec_base + ref.prim] = rec;
     }
+    __syncthreads();  // the wave's fans are staged (the workgroup is one wave)
+    // the wave's fan triangles into their bins, one per lane
+    for (uint32_t f = (uint32_t)tid; f < total; f += kClipBlock) {
+      const uint32_t e = s_fentry[f];
+      if (e != ~0u) bin_fan(p, s_bins[s_fown[f]], s_frect[f], e, phase);
+    }
+    __syncthreads();  // LDS is rewritten by the next primitives
   }
 }
 

@@ -240,9 +240,10 @@ struct gbr_renderer {
   DevBuf<RasterRec> d_sub_recs;
   DevBuf<uint32_t> d_sub_prims;
   DevBuf<PrimRef> d_clipped;
+  DevBuf<ClipFast> d_clip_fast;
   DevBuf<uint32_t> d_counters;
   DevBuf<unsigned long long> d_vis;
-  uint32_t clipped_cap = 0, sub_cap = 0;
+  uint32_t clipped_cap = 0, sub_cap = 0, clip_fast_cap = 0;
   DevBuf<unsigned long long> d_stats;
   DevBuf<uint32_t> d_color, d_sem;
   DevBuf<float> d_depth, d_lin;
@@ -565,11 +566,16 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   r->entries_cap = entries;
   TRY(r->d_rrecs.reserve(std::max<uint64_t>(tris, 1)));
   TRY(r->d_srecs.reserve(std::max<uint64_t>(tris, 1)));
-  // every primitive is listed at most once; the sub-triangle pool is bounded
-  // (bins of an unstored fan fall back to the exact rescan)
+  // every primitive is listed at most once; the sub-triangle pool gives list
+  // slot i the fan slots [7 i, 7 i + 7) and is bounded (bins of a fan past
+  // it fall back to the exact rescan)
   r->clipped_cap = (uint32_t)std::max<uint64_t>(tris, 1);
-  r->sub_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(tris, 1024), 1u << 21);
+  r->sub_cap = (uint32_t)std::min<uint64_t>(7 * std::max<uint64_t>(tris, 1024), 7u << 20);
+  // the clip pass's fast entries cover the first slots of the list (beyond
+  // them it reads a primitive through its draw)
+  r->clip_fast_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(tris, 1), 1u << 20);
   TRY(r->d_clipped.reserve(r->clipped_cap));
+  TRY(r->d_clip_fast.reserve(r->clip_fast_cap));
   TRY(r->d_sub_recs.reserve(r->sub_cap));
   TRY(r->d_sub_prims.reserve(r->sub_cap));
   HIP_TRY(hipStreamSynchronize(s));
@@ -781,9 +787,11 @@ gbr_status do_draw(gbr_renderer* r) {
   p.sub_recs = r->d_sub_recs.ptr;
   p.sub_prims = r->d_sub_prims.ptr;
   p.clipped = r->d_clipped.ptr;
+  p.clip_fast = r->d_clip_fast.ptr;
   p.counters = r->d_counters.ptr;
   p.vis = r->d_vis.ptr;
   p.clipped_cap = r->clipped_cap;
+  p.clip_fast_cap = r->clip_fast_cap;
   p.queue_grain = (uint32_t)r->tun.queue_grain;
   p.n_items = r->n_items;
   p.items = r->d_items.ptr;
@@ -1170,6 +1178,7 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   r->d_sub_prims.release();
   r->d_sub_recs.release();
   r->d_clipped.release();
+  r->d_clip_fast.release();
   r->d_counters.release();
   r->d_vis.release();
   r->d_scenes.release();
