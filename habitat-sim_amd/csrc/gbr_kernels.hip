@@ -2095,13 +2095,13 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
   if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
 }
 
-// Without resolve, at least 5 waves per SIMD (96 VGPRs, no spills): the
-// bins' dependent global loads are latency-bound, and 5 resident workgroups
-// per CU rasterise phase 1 ~8% faster than the 4 an unbounded allocation
-// allows.  The instance that also resolves runs at 4 (128 VGPRs): at 5 the
-// shading spills to scratch.
+// Without resolve, at least 6 waves per SIMD (80 VGPRs, no spills): the
+// bins' chains of dependent global loads are latency-bound, and 6 resident
+// workgroups per CU rasterise the first phase ~7% faster than 5 (at 7 the
+// walk spills).  The instance that also resolves runs at 4 (128 VGPRs): at
+// 5 the shading spills to scratch.
 template <bool kResolve>
-__global__ __launch_bounds__(kBlock, kResolve ? 4 : 5) void k_raster_lane(FrameParams p,
+__global__ __launch_bounds__(kBlock, kResolve ? 4 : 6) void k_raster_lane(FrameParams p,
                                                         const RasterItem* __restrict__ items,
                                                         uint32_t n_ritems, int phase) {
   __shared__ RasterLds L;

@@ -1065,9 +1065,9 @@ gbr_status gbr_tuning_init(gbr_tuning* t) {
   t->min_bin_capacity = 256;
   t->cull_mode = GBR_CULL_NONE;
   t->queue_grain = 4;
-  t->lane_pixels = 96;
+  t->lane_pixels = 64;
   t->graph = 1;
-  t->heavy_entries = 512;
+  t->heavy_entries = 256;
   return GBR_OK;
 }
 
@@ -1906,7 +1906,9 @@ gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
         float ms = 0.0f;
         TRY(time_tuning(r, t, *params, &ms));
         ++report->candidates;
-        if (ms < best_ms) {
+        // a candidate has to beat the best by more than the frame-to-frame
+        // noise (~1%) to replace it
+        if (ms < best_ms * 0.99f) {
           best_ms = ms;
           best = t;
         }

@@ -122,8 +122,9 @@ typedef struct gbr_tuning {
   float proposal_threshold;   /* GBR_CULL_LEARNED: phase 1 = chunks whose logit
                                  exceeds this */
   int32_t lane_pixels;        /* raster_mode 0: triangles covering more than this
-                                 many pixels of a bin are rasterised by the whole
-                                 workgroup, the rest by one lane each: 1..1024 */
+                                 many pixels of a bin are rasterised by a whole
+                                 wave (the rectangle spread over its lanes), the
+                                 rest by one lane each: 1..1024; default 64 */
   int32_t graph;              /* 1: gbr_draw() replays its kernel sequence as a
                                  captured HIP graph while nothing but cameras and
                                  transforms changed (timing and row collection
@@ -134,7 +135,7 @@ typedef struct gbr_tuning {
                                  the right choice when cameras jump) */
   int32_t heavy_entries;      /* raster_mode 0: bins that receive at least this many
                                  entries in a phase are rasterised first (0: bins in
-                                 order); default 512 */
+                                 order); default 256 */
 } gbr_tuning;
 
 /* stages timed when gbr_tuning.timing is set (gbr_frame_timings) */
@@ -390,8 +391,9 @@ gbr_status gbr_frame_visibility(gbr_renderer* r, uint64_t* visible, uint64_t* vi
    candidate launch parameters, one parameter at a time (bin size, per-bin
    capacity factor, lane/workgroup raster split, setup claim size, raster
    mode, heavy-bin threshold, and with GBR_CULL_LEARNED the proposal
-   threshold; each from the best values found so far), and leaves the
-   fastest set (every candidate renders the same pixels).  cull_mode,
+   threshold; each from the best values found so far; a candidate replaces
+   the best only when it is more than 1% faster), and leaves the fastest set
+   (every candidate renders the same pixels).  cull_mode,
    timing, graph, chunk_triangles and proposal_history are kept. */
 typedef struct gbr_autotune_params {
   int32_t warmup; /* untimed draws per candidate (>= 1; absorbs rebuilds) */

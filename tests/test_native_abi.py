@@ -51,7 +51,8 @@ def test_tuning_defaults():
     N.check(lib.gbr_tuning_init(ctypes.byref(t)))
     assert (t.bin_size, t.chunk_triangles, t.queue_grain) == (32, 1024, 4)
     assert (t.cull_mode, t.raster_mode, t.timing) == (N.CULL_NONE, 0, 0)
-    assert t.lane_pixels == 96 and t.proposal_threshold == 0.0 and t.graph == 1
+    assert t.lane_pixels == 64 and t.proposal_threshold == 0.0 and t.graph == 1
+    assert t.heavy_entries == 256 and t.proposal_history == 0
 
 
 def test_null_arguments_are_errors_not_crashes():
