@@ -2095,13 +2095,16 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
   if (threadIdx.x == 0) p.bin_count[b.gbin] = 0;  // ready for the next frame
 }
 
-// Without resolve, at least 6 waves per SIMD (80 VGPRs, no spills): the
-// bins' chains of dependent global loads are latency-bound, and 6 resident
-// workgroups per CU rasterise the first phase ~7% faster than 5 (at 7 the
-// walk spills).  The instance that also resolves runs at 4 (128 VGPRs): at
-// 5 the shading spills to scratch.
-template <bool kResolve>
-__global__ __launch_bounds__(kBlock, kResolve ? 4 : 6) void k_raster_lane(FrameParams p,
+// kWaves: the waves per SIMD the register allocation must allow.  Without
+// resolve 6 (80 VGPRs, no spills): the bins' chains of dependent global
+// loads are latency-bound, and 6 resident workgroups per CU rasterise the
+// first phase ~7% faster than 5 (at 7 the walk spills).  With the resolve,
+// 5 when the resolving phase is the only one (every bin rasterised: a few
+// spilled registers cost less than the lost occupancy, plain frame -4%)
+// and 4 (no spills) after the depth-pyramid phase, whose bins mostly only
+// resolve.
+template <bool kResolve, int kWaves>
+__global__ __launch_bounds__(kBlock, kWaves) void k_raster_lane(FrameParams p,
                                                         const RasterItem* __restrict__ items,
                                                         uint32_t n_ritems, int phase) {
   __shared__ RasterLds L;
@@ -2995,11 +2998,14 @@ hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream
   return hipGetLastError();
 }
 
-int raster_blocks_per_cu(bool resolve) {
+// the lane rasteriser's workgroups per CU: kind 0 stores the keys, 1
+// resolves a frame's only raster phase, 2 resolves after the depth pyramid
+int raster_blocks_per_cu(int kind) {
   int b = 0;
   const hipError_t e =
-      resolve ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane<true>, kBlock, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane<false>, kBlock, 0);
+      kind == 1   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane<true, 5>, kBlock, 0)
+      : kind == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane<true, 4>, kBlock, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_lane<false, 6>, kBlock, 0);
   if (e != hipSuccess || b < 1) b = 4;
   return b;
 }
@@ -3011,12 +3017,15 @@ hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t
   if (n == 0) return hipSuccess;
   if (p.raster_mode == 1)
     hipLaunchKernelGGL(k_raster_pairs, dim3(n), dim3(kBlock), 0, s, p, items, phase);
+  else if (resolves_in_place(p, phase) && phase == 1)
+    hipLaunchKernelGGL((k_raster_lane<true, 5>), dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p,
+                       items, n, phase);
   else if (resolves_in_place(p, phase))
-    hipLaunchKernelGGL(k_raster_lane<true>, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items,
-                       n, phase);
+    hipLaunchKernelGGL((k_raster_lane<true, 4>), dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p,
+                       items, n, phase);
   else
-    hipLaunchKernelGGL(k_raster_lane<false>, dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p, items,
-                       n, phase);
+    hipLaunchKernelGGL((k_raster_lane<false, 6>), dim3(grid < n ? grid : n), dim3(kBlock), 0, s, p,
+                       items, n, phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_raster_full, dim3((n + kFullGroup - 1) / kFullGroup), dim3(kBlock), 0, s, p,

@@ -39,7 +39,7 @@ hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream
 int setup_blocks_per_cu(uint32_t lds_bins);
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
                         uint32_t queue_ctr, uint32_t grid, hipStream_t s);
-int raster_blocks_per_cu(bool resolve);
+int raster_blocks_per_cu(int kind);
 hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t n, int phase,
                          uint32_t grid,
                          hipStream_t s);
@@ -163,7 +163,9 @@ struct gbr_renderer {
   int device = 0;
   int cu_count = 256;
   int setup_bpc = 4;      // setup workgroups per CU (occupancy with lds_bins)
-  int raster_bpc[2] = {4, 4};  // persistent raster workgroups per CU (occupancy), without / with resolve
+  // persistent raster workgroups per CU (occupancy): storing, resolving the
+  // only phase, resolving after the depth pyramid
+  int raster_bpc[3] = {4, 4, 4};
   uint32_t lds_bins = 0;  // bins per scene counted in setup's LDS (0: none)
   hipStream_t stream = nullptr;
 
@@ -407,8 +409,7 @@ gbr_status rebuild_structure(gbr_renderer* r) {
     if (nb <= (uint32_t)kMaxSceneBins) r->lds_bins = std::max(r->lds_bins, nb);
   }
   r->setup_bpc = setup_blocks_per_cu(r->lds_bins);
-  r->raster_bpc[0] = raster_blocks_per_cu(false);
-  r->raster_bpc[1] = raster_blocks_per_cu(true);
+  for (int k = 0; k < 3; ++k) r->raster_bpc[k] = raster_blocks_per_cu(k);
 
   std::vector<int32_t> parents;
   std::vector<float> local;
@@ -903,8 +904,8 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
   const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
   // the lane rasteriser's phase that resolves in place runs at lower occupancy
   const bool fuse2 = p.fuse_resolve && p.cull_mode >= kCullHiZ;
-  const uint32_t raster_grid1 = (uint32_t)(r->cu_count * r->raster_bpc[p.fuse_resolve && !fuse2]);
-  const uint32_t raster_grid2 = (uint32_t)(r->cu_count * r->raster_bpc[fuse2]);
+  const uint32_t raster_grid1 = (uint32_t)(r->cu_count * r->raster_bpc[p.fuse_resolve && !fuse2 ? 1 : 0]);
+  const uint32_t raster_grid2 = (uint32_t)(r->cu_count * r->raster_bpc[fuse2 ? 2 : 0]);
   const bool culled = p.cull_mode != kCullNone;
   if (culled || r->collecting) {  // (unculled frames run it for the feature rows only)
     // phase 1: items that pass the frustum test and are proposed visible
