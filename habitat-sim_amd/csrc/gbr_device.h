@@ -173,7 +173,9 @@ enum : uint32_t {
   kCtrList2 = 6,     // work items set up in phase 2
   kCtrClipMark = 7,  // clipped primitives handled by the phase-1 clip pass
   kCtrProp = 8,      // work items with a visibility-feature row this frame
-  kCtrCount = 9
+  kCtrOverflow1 = 9,   // bins whose entries overflowed in phase 1 (exact rescan)
+  kCtrOverflow2 = 10,  // and in phase 2
+  kCtrCount = 11
 };
 constexpr uint32_t kCtrStride = 32;
 // The setup work queue of each phase is split into kQueueStripes contiguous
@@ -323,6 +325,7 @@ struct FrameParams {
   uint32_t* heavy;              // [raster items] bins listed this phase, by range
   uint32_t* bin_heavy;          // [raster items] 1 = listed (cleared by raster)
   uint32_t* active;             // [raster items] bins with phase-2 entries, by range
+  uint32_t* overflow;           // [raster items] bins whose entries overflowed this phase
   uint32_t n_ritems;            // raster items (bins of every scene)
   uint32_t part_bins;           // bins per raster queue range: ceil(n_ritems / kRasterParts)
   uint32_t heavy_entries;       // 0: no heavy list

@@ -556,12 +556,18 @@ __device__ __forceinline__ void append_ref(const FrameParams& p, uint32_t counte
 // active (counts only grow within a phase, so a bin is listed at most once
 // per list and no list exceeds the raster items).
 __device__ __forceinline__ void note_heavy(const FrameParams& p, uint32_t gbin, uint32_t old,
-                                           uint32_t n, int phase) {
-  // the bin's raster queue range and that range's section of the lists
+                                           uint32_t n, int phase, uint32_t cap) {
   const bool active = phase == 2 && old == 0u && n != 0u;
   const uint32_t t = p.heavy_entries;
   const bool heavy = t != 0u && old < t && n >= t - old;
+  // the reservation that takes the count past the bin's capacity lists the
+  // bin for the exact rescan
+  const bool over = old <= cap && n > cap - old;
+  if (over)
+    p.overflow[atomicAdd(&p.counters[(phase == 2 ? kCtrOverflow2 : kCtrOverflow1) * kCtrStride], 1u)] =
+        gbin;
   if (!active && !heavy) return;
+  // the bin's raster queue range and that range's section of the lists
   const uint32_t part = gbin / p.part_bins, first = part * p.part_bins;
   if (active)
     p.active[first + atomicAdd(&p.counters[raster_ctr(2, kPartActive, part) * kCtrStride], 1u)] = gbin;
@@ -805,7 +811,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_setup_bin(FrameParams p,
                 s_pair[pi] = (b << 21) | (slot << 10) | j;
               } else {
                 const uint32_t slot = atomicAdd(&p.bin_count[sc.bin_base + b], 1u);
-                note_heavy(p, sc.bin_base + b, slot, 1u, phase);
+                note_heavy(p, sc.bin_base + b, slot, 1u, phase, sc.bin_cap);
                 if (slot < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b * sc.bin_cap + slot] = prim;
               }
               ++written;
@@ -826,7 +832,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_setup_bin(FrameParams p,
       if (n) {
         const uint32_t old = atomicAdd(&p.bin_count[sc.bin_base + b], n);
         s_bbase[b] = old;
-        note_heavy(p, sc.bin_base + b, old, n, phase);
+        note_heavy(p, sc.bin_base + b, old, n, phase, sc.bin_cap);
       }
     }
     __syncthreads();
@@ -1609,11 +1615,14 @@ __device__ uint32_t shade_pixel(const FrameParams& p, const float* lut, float hw
 
 // forces the exact rescan for every bin the rectangle touches
 __device__ __forceinline__ void overflow_rect(const FrameParams& p, const SceneDesc& sc,
-                                              const PixRect& r) {
+                                              const PixRect& r, int phase) {
   const int sh = p.bin_shift;
   for (int by = r.y0 >> sh; by <= (r.y1 >> sh); ++by)
-    for (int bx = r.x0 >> sh; bx <= (r.x1 >> sh); ++bx)
-      atomicAdd(&p.bin_count[sc.bin_base + (uint32_t)(by * sc.bins_x + bx)], sc.bin_cap + 1u);
+    for (int bx = r.x0 >> sh; bx <= (r.x1 >> sh); ++bx) {
+      const uint32_t gbin = sc.bin_base + (uint32_t)(by * sc.bins_x + bx);
+      const uint32_t old = atomicAdd(&p.bin_count[gbin], sc.bin_cap + 1u);
+      note_heavy(p, gbin, old, sc.bin_cap + 1u, phase, sc.bin_cap);
+    }
 }
 
 // phase 1 takes the clipped list from its start and leaves its end in
@@ -1648,7 +1657,7 @@ __device__ __forceinline__ void bin_fan(const FrameParams& p, const ClipBins& sc
     }
 #pragma unroll
     for (int g = 0; g < 8; ++g) {
-      if (j0 + g < nb) note_heavy(p, sc.bin_base + b[g], slot[g], 1u, phase);
+      if (j0 + g < nb) note_heavy(p, sc.bin_base + b[g], slot[g], 1u, phase, sc.bin_cap);
       if (slot[g] < sc.bin_cap) p.bin_entries[sc.entry_base + (size_t)b[g] * sc.bin_cap + slot[g]] = entry;
     }
   }
@@ -1798,7 +1807,7 @@ __global__ __launch_bounds__(kClipBlock) void k_setup_clip(FrameParams p, int ph
           s_fentry[k] = kSubTri | (uint32_t)slot;
         } else {
           s_fentry[k] = ~0u;
-          overflow_rect(p, sc, r);
+          overflow_rect(p, sc, r, phase);
         }
       }
     }
@@ -2266,10 +2275,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_pairs(FrameParams p,
   add_raster_stats(p, pairs, frags);
 }
 
-// exact fallback for bins whose list overflowed: every primitive of the scene
-// One workgroup checks kFullGroup bins and rescans the overflowed ones among
-// them in turn (overflow is rare: most workgroups exit after the check).
-constexpr int kFullGroup = 64;
+// exact fallback for bins whose list overflowed: every primitive of the
+// scene.  Setup lists the overflowed bins of the phase (the reservation that
+// crosses a bin's capacity appends it); kFullBlocks workgroups take them in
+// turn (overflow is rare: the workgroups exit after reading the count).
+constexpr int kFullBlocks = 64;
 
 __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
                                                         const RasterItem* __restrict__ items,
@@ -2278,26 +2288,17 @@ __global__ __launch_bounds__(kBlock) void k_raster_full(FrameParams p,
   __shared__ BigTri s_big[kBigQueue];
   __shared__ uint32_t s_base[kDrawCache];
   __shared__ int s_nbig, s_next;
-  __shared__ uint32_t s_over[kFullGroup];
-  __shared__ int s_nover;
   __shared__ float s_lut[256];
 
-  if (threadIdx.x == 0) s_nover = 0;
+  const uint32_t n_over =
+      min(p.counters[(phase == 2 ? kCtrOverflow2 : kCtrOverflow1) * kCtrStride], n);
+  if (blockIdx.x >= n_over) return;  // (whole workgroup)
   if (p.fuse_resolve) s_lut[threadIdx.x] = p.unorm_lut[threadIdx.x];  // (kBlock == 256)
   __syncthreads();
-  const uint32_t first = blockIdx.x * kFullGroup;
-  if (threadIdx.x < kFullGroup && first + threadIdx.x < n) {
-    const RasterItem it = items[first + threadIdx.x];
-    if (p.bin_count[it.gbin] > it.bin_cap)
-      s_over[atomicAdd(&s_nover, 1)] = first + threadIdx.x;
-  }
-  __syncthreads();
-  const int nover = s_nover;
-  if (nover == 0) return;  // (uniform: read after the barrier)
   const int B = p.bin_size;
   uint32_t pairs = 0, frags = 0;
-  for (int o = 0; o < nover; ++o) {
-    const RasterItem it = items[s_over[o]];
+  for (uint32_t o = blockIdx.x; o < n_over; o += gridDim.x) {
+    const RasterItem it = items[p.overflow[o]];
     const BinInfo b = bin_info(it);
     const SceneDesc sc = p.scenes[it.scene];
     RasterCtx rc;
@@ -3028,7 +3029,8 @@ hipError_t launch_raster(const FrameParams& p, const RasterItem* items, uint32_t
                        items, n, phase);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_raster_full, dim3((n + kFullGroup - 1) / kFullGroup), dim3(kBlock), 0, s, p,
+  hipLaunchKernelGGL(k_raster_full, dim3(n < (uint32_t)kFullBlocks ? n : (uint32_t)kFullBlocks),
+                     dim3(kBlock), 0, s, p,
                      items, n, phase);
   return hipGetLastError();
 }

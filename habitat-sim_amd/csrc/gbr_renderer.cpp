@@ -237,6 +237,7 @@ struct gbr_renderer {
   DevBuf<uint32_t> d_bin_count, d_bin_entries;
   DevBuf<uint32_t> d_heavy, d_bin_heavy;  // heavy-bin list and per-bin flags (raster items)
   DevBuf<uint32_t> d_active;              // bins with phase-2 entries (raster items)
+  DevBuf<uint32_t> d_overflow;            // bins whose entries overflowed this phase
   DevBuf<RasterRec> d_rrecs;
   DevBuf<ResolveRec> d_srecs;
   DevBuf<RasterRec> d_sub_recs;
@@ -562,6 +563,7 @@ gbr_status rebuild_structure(gbr_renderer* r) {
   TRY(r->d_heavy.reserve(std::max<size_t>(ritems.size(), 1)));
   TRY(r->d_bin_heavy.reserve(std::max<size_t>(ritems.size(), 1)));
   TRY(r->d_active.reserve(std::max<size_t>(ritems.size(), 1)));
+  TRY(r->d_overflow.reserve(std::max<size_t>(ritems.size(), 1)));
   HIP_TRY(hipMemsetAsync(r->d_bin_heavy.ptr, 0, r->d_bin_heavy.cap * sizeof(uint32_t), s));
   TRY(r->d_bin_entries.reserve(std::max<uint64_t>(entries, 1)));
   r->entries_cap = entries;
@@ -776,6 +778,7 @@ gbr_status do_draw(gbr_renderer* r) {
   p.heavy = r->d_heavy.ptr;
   p.bin_heavy = r->d_bin_heavy.ptr;
   p.active = r->d_active.ptr;
+  p.overflow = r->d_overflow.ptr;
   p.n_ritems = r->n_ritems;
   p.part_bins = std::max<uint32_t>((r->n_ritems + kRasterParts - 1) / kRasterParts, 1u);
   p.heavy_entries = r->tun.raster_mode == 0 ? (uint32_t)r->tun.heavy_entries : 0u;
@@ -1166,7 +1169,7 @@ gbr_status gbr_destroy(gbr_renderer* r) {
   for (auto* b : fb) b->release();
   DevBuf<uint32_t>* ub[] = {&r->d_uv16, &r->d_col8, &r->d_indices, &r->d_bin_count,
                             &r->d_bin_entries, &r->d_color, &r->d_sem, &r->d_heavy,
-                            &r->d_bin_heavy, &r->d_active};
+                            &r->d_bin_heavy, &r->d_active, &r->d_overflow};
   for (auto* b : ub) b->release();
   r->d_uv32.release();
   r->d_col32.release();
