@@ -7,11 +7,14 @@ seeded random walk), with every input already resident in HBM; outputs stay
 in HBM (colour RGBA8, linear depth FP32, semantic uint32 planes).  The walk is
 run once per culling mode (plain = every triangle set up, frustum, hiz =
 two-phase depth-pyramid occlusion with the previous frame as proposal,
-learned = the same with the visibility MLP as proposal, trained first on
-other walks); each mode's launch parameters come from the library's
-autotuner (gbr_autotune) on the walk's first view; the headline value is the
-learned mode, and the final frame of every mode is compared pixel by pixel
-with the plain one.
+learned = the same with the visibility MLP's proposal together with the
+previous frame's visible chunks, the proposal for a camera walk, and
+learned_mlp = the MLP's proposal alone; the MLP is trained first on other
+walks); each mode's launch parameters come from the library's autotuner
+(gbr_autotune) on the walk's first view; the headline value is the learned
+mode, and the final frame of every mode is compared pixel by pixel with the
+plain one.  The learned-proposal workload (teleports, no history) times the
+MLP's proposal alone against Hi-Z.
 
 Prints ONE JSON line (rank 0).  ``roofline`` is the headline mode's frame
 (its kernels back to back on the renderer's stream): the frozen formula
@@ -149,7 +152,8 @@ def copy_probe(nbytes: int) -> None:
     b.close()
 
 
-MODES = (("plain", 0), ("frustum", 1), ("hiz", 2), ("learned", 3))
+# (name, gbr cull mode, proposal_history)
+MODES = (("plain", 0, 0), ("frustum", 1, 0), ("hiz", 2, 0), ("learned", 3, 1), ("learned_mlp", 3, 0))
 HEADLINE = "learned"
 
 
@@ -171,9 +175,9 @@ def frame_bytes(spec, st, mode, sensors=3):
     b += envs * (sensors * w * h * 4 + 8 * w * h)
     if mode != "plain":
         b += 24 * st["work_items"]
-    if mode in ("hiz", "learned"):
+    if mode == "hiz" or mode.startswith("learned"):
         b += envs * 2 * (4 * w * h * 4 // 3)
-    if mode == "learned":
+    if mode.startswith("learned"):
         from habitat_sim_amd import _native as N
         b += 64 * (st["items_phase1"] + st["items_candidates"]) + 2 * N.VIS_PARAMS
     return int(b)
@@ -205,14 +209,14 @@ def profile_of_this_build(name):
     return prof, f"profiles/{name}"
 
 
-def run_mode(r, mode, spec, views, proj, warmup, steps):
+def run_mode(r, mode, spec, views, proj, warmup, steps, history=0):
     """Times `steps` camera-update + draw() steps after `warmup` untimed
     ones, with stage timing off (its events sit between kernels); then
     replays the same views with stage timing on for the per-stage split,
     and once more one frame at a time for the p50/p99 frame latency."""
     import numpy as np
     from habitat_sim_amd import hip
-    r.set_tuning(cull_mode=mode, timing=0)
+    r.set_tuning(cull_mode=mode, proposal_history=history, timing=0)
     stream = r.stream()
     # launch parameters for this mode from the autotuner, on the walk's
     # first views (every candidate renders the same pixels)
@@ -377,7 +381,7 @@ def learned_proposal(args):
           f"{args.lp_train_frames * n} random poses in {train_s:.1f} s", file=sys.stderr, flush=True)
     stream = r.stream()
     res, planes = {}, {}
-    for name, mode in MODES:
+    for name, mode, _ in MODES[:4]:
         r.set_tuning(cull_mode=mode, proposal_history=0, timing=0)
         r.update_cameras(0, proj, frames[0])
         r.autotune(warmup=1, frames=2)
@@ -460,7 +464,7 @@ def main():
     ap.add_argument("--train-frames", type=int, default=64,
                     help="frames of training rows for the visibility model")
     ap.add_argument("--train-steps", type=int, default=600)
-    ap.add_argument("--modes", default=",".join(m for m, _ in MODES),
+    ap.add_argument("--modes", default=",".join(m for m, _, _ in MODES),
                     help="culling modes to time, comma-separated (profiling runs time "
                          "fewer; the speedup needs plain and the headline mode)")
     ap.add_argument("--rollout-envs", type=int, default=4096, metavar="ENVS",
@@ -476,9 +480,9 @@ def main():
                     help="only copy MIB MiB device-to-device 3 times (the reference kernel "
                          "for scaling hardware byte counters) and exit")
     args = ap.parse_args()
-    modes = [(m, c) for m, c in MODES if m in args.modes.split(",")]
+    modes = [(m, c, h) for m, c, h in MODES if m in args.modes.split(",")]
     if not modes:
-        ap.error(f"--modes: none of {[m for m, _ in MODES]}")
+        ap.error(f"--modes: none of {[m for m, _, _ in MODES]}")
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "environments would shard across GPUs with no exchange between them", file=sys.stderr)
@@ -506,7 +510,7 @@ def main():
     # the visibility model trains on its own walks (random starts, other
     # seeds) before the learned mode is timed
     training = None
-    if any(m == "learned" for m, _ in modes):
+    if any(m.startswith("learned") for m, _, _ in modes):
         from habitat_sim_amd import visibility
         t0 = time.perf_counter()
         training = visibility.train(r, spec, frames=args.train_frames, steps=args.train_steps,
@@ -518,15 +522,15 @@ def main():
         print(f"bench: visibility model trained in {training['seconds']:.1f} s", file=sys.stderr,
               flush=True)
     runs = {}
-    for name, mode in modes:
-        runs[name] = run_mode(r, mode, spec, views, proj, args.warmup, args.steps)
+    for name, mode, history in modes:
+        runs[name] = run_mode(r, mode, spec, views, proj, args.warmup, args.steps, history)
         print(f"bench: {name} {runs[name]['gpu_ms']:.3f} ms/frame", file=sys.stderr, flush=True)
     base = runs.get("plain")
     headline = HEADLINE if HEADLINE in runs else modes[-1][0]
     head = runs[headline]
 
     cull = {}
-    for name, _ in modes:
+    for name, _, _ in modes:
         x = runs[name]
         cull[name] = {
             "env_frames_per_s": n * args.steps / x["wall"],
@@ -575,7 +579,9 @@ def main():
                    "envs": n, "resolution": [args.res, args.res],
                    "triangles_per_frame": int(head["stats"]["triangles"]),
                    "sensors": ["rgba8", "linear_depth_f32", "semantic_u32"],
-                   "cull": headline, "parallelism": "single-gpu"},
+                   "cull": headline + (" (MLP proposal + previous frame's visible chunks)"
+                                       if headline == "learned" else ""),
+                   "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": prof_source,
@@ -591,8 +597,11 @@ def main():
         "batch_steps_per_s": value / n,
         "learned_cull_speedup": cull["learned"]["speedup_vs_plain"] if "learned" in cull else None,
         "learned_vs_hiz": ({"learned_ms": cull["learned"]["gpu_ms_per_frame"],
+                            "learned_mlp_ms": (cull["learned_mlp"]["gpu_ms_per_frame"]
+                                               if "learned_mlp" in cull else None),
                             "hiz_ms": cull["hiz"]["gpu_ms_per_frame"],
-                            "speedup": runs["hiz"]["wall"] / runs["learned"]["wall"]}
+                            "speedup": runs["hiz"]["wall"] / runs["learned"]["wall"],
+                            "teleports": "learned_proposal.learned_vs_hiz"}
                            if "learned" in cull and "hiz" in cull else None),
         "vis_training": training,
         "cull": cull,
