@@ -349,6 +349,7 @@ struct FrameParams {
   uint32_t hiz_stride;
   float proposal_threshold;
   int32_t proposal_history;  // learned mode: phase 1 also takes last frame's visible items
+  int32_t collecting;        // visibility rows are collected this frame (every chunk gets its row)
   int32_t raster_mode;  // 0: a lane per triangle, 1: a lane per (triangle, pixel)
   int32_t lane_pixels;  // mode 0: rectangles above this go to the workgroup queue
   uint32_t lds_bins;    // setup: scenes with at most this many bins count in LDS

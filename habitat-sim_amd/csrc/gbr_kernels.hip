@@ -2789,14 +2789,21 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
           ic.zmin = (zmin - ez) * 0.5f + 0.5f;
           ic.pad[0] = wi.scene;
           ic.pad[1] = 0u;
-          if (p.features) {
+          // learned mode with history: a chunk visible in the previous frame
+          // goes to phase 1 without a score (unless rows are collected, which
+          // needs every chunk's row)
+          const bool seen = p.cull_mode == kCullLearned && p.proposal_history && !p.collecting &&
+                            p.item_vis[item] + 1u == *p.frame;
+          if (p.features && !seen) {
             feat = true;
             if (p.feature_set == kFeaturesHistory)
               item_features_history(p, wi, sc, bb, M, PV, ic, wmin, wmax, item, frow);
             else
               item_features_pose(p, wi, bb, M, frow);
           }
-          if (p.cull_mode == kCullLearned) {
+          if (seen) {
+            to1 = true;
+          } else if (p.cull_mode == kCullLearned) {
             p.cull[item] = ic;  // k_propose decides from the model's score
           } else {
             const bool proposed =

@@ -812,6 +812,7 @@ gbr_status do_draw(gbr_renderer* r) {
   p.frame = r->d_frame.ptr;
   p.proposal_threshold = r->tun.proposal_threshold;
   p.proposal_history = r->tun.proposal_history;
+  p.collecting = r->collecting ? 1 : 0;
   p.cam_info = r->d_caminfo.ptr;
   const bool learned = p.cull_mode == kCullLearned;
   if (learned || r->collecting) {
