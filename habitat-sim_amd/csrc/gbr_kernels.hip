@@ -1987,14 +1987,16 @@ __device__ void bin_hiz(const FrameParams& p, const BinInfo& b, const unsigned l
 }
 
 // --------------------------------------------------------------------------
-// k_raster_lane: persistent workgroups draining the phase's raster queue:
-// first the bins setup listed as heavy (most entries), then every other bin
-// in order, so that the longest bins start first and the frame's tail is
-// short.  Per bin, one binned triangle per lane walking its clipped
-// rectangle; rectangles above the lane limit (gbr_tuning.lane_pixels) are
-// rasterised by the whole workgroup from an LDS queue.  Phase 1 starts every
-// bin from the clear key; phase 2 merges into the phase-1 keys and leaves
-// bins without new entries untouched.
+// k_raster_lane: persistent workgroups draining the phase's raster queue,
+// split into one range of bins per XCD: within a range first the bins setup
+// listed as heavy (most entries), then every other bin in order, so that
+// the longest bins start first and the frame's tail is short.  Per bin, one
+// binned triangle per lane walking its clipped rectangle; rectangles above
+// the lane limit (gbr_tuning.lane_pixels) go to an LDS queue whose
+// triangles the waves take one at a time, each spread over a wave's lanes.
+// Phase 1 starts every bin from the clear key; phase 2 merges into the
+// phase-1 keys and leaves bins without new entries untouched (or, when it
+// resolves in place, resolves them from their phase-1 keys).
 // --------------------------------------------------------------------------
 
 struct RasterLds {
@@ -2071,9 +2073,9 @@ __device__ void raster_bin(const FrameParams& p, const RasterItem& item, int pha
   if (threadIdx.x == 0) L.nbig[0] = L.nbig[1] = L.next[0] = L.next[1] = 0;
   __syncthreads();
   // per batch of kBlock entries: lanes append large rectangles to the queue
-  // (fill counter of this batch's parity; the other one, read before the
-  // previous batch's second barrier, is cleared for the next batch), then
-  // the whole workgroup rasterises the queue, each thread its own pixels
+  // (fill and claim counters of this batch's parity; the other ones, read
+  // before the previous batch's second barrier, are cleared for the next
+  // batch), then the waves take the queued triangles one at a time
   int parity = 0;
   for (uint32_t base = 0; base < n; base += kBlock, parity ^= 1) {
     const uint32_t k = base + threadIdx.x;
