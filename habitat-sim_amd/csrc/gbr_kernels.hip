@@ -301,24 +301,11 @@ __device__ void mat4_mul(const float* a, const float* b, float* out) {
   for (int i = 0; i < 16; ++i) out[i] = t[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ parents,
-                                                  const float* __restrict__ local,
-                                                  float* __restrict__ abs_out, uint32_t n,
-                                                  uint32_t* __restrict__ counters,
-                                                  unsigned long long* __restrict__ stats,
-                                                  uint32_t* __restrict__ frame,
-                                                  float* __restrict__ nrm_out,
-                                                  const uint32_t* __restrict__ dirty) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  // first kernel of every frame: reset the frame's counters, advance the
-  // frame number (kept on the device, so a captured frame replays as is)
-  if (i < kCtrTotal) counters[i * kCtrStride] = 0u;
-  if (i == 0) *frame += 1u;
-  for (uint32_t j = i; j < kStatCount * kStatStripes; j += gridDim.x * kBlock) stats[j] = 0ull;
-  // the host uploads *dirty with the cameras of every frame: 0 when no local
-  // transform and no node list changed since the previous frame, whose
-  // absolute transforms (and normal matrices) are still in place
-  if (i >= n || *dirty == 0u) return;
+// absolute transform (and normal matrix) of node i
+__device__ __forceinline__ void node_transform(const int32_t* __restrict__ parents,
+                                               const float* __restrict__ local,
+                                               float* __restrict__ abs_out,
+                                               float* __restrict__ nrm_out, uint32_t i) {
   int depth = 0;
   for (int32_t q = parents[i]; q >= 0 && depth < 1024; q = parents[q]) ++depth;
   float acc[16];
@@ -356,6 +343,30 @@ __global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ pa
     nm[1] = c10 / det; nm[4] = c11 / det; nm[7] = c12 / det;
     nm[2] = c20 / det; nm[5] = c21 / det; nm[8] = c22 / det;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_nodes(const int32_t* __restrict__ parents,
+                                                  const float* __restrict__ local,
+                                                  float* __restrict__ abs_out, uint32_t n,
+                                                  uint32_t* __restrict__ counters,
+                                                  unsigned long long* __restrict__ stats,
+                                                  uint32_t* __restrict__ frame,
+                                                  float* __restrict__ nrm_out,
+                                                  const uint32_t* __restrict__ dirty) {
+  const uint32_t i0 = blockIdx.x * kBlock + threadIdx.x;
+  // first kernel of every frame: reset the frame's counters, advance the
+  // frame number (kept on the device, so a captured frame replays as is)
+  if (i0 < kCtrTotal) counters[i0 * kCtrStride] = 0u;
+  if (i0 == 0) *frame += 1u;
+  for (uint32_t j = i0; j < kStatCount * kStatStripes; j += gridDim.x * kBlock) stats[j] = 0ull;
+  // the host uploads *dirty with the cameras of every frame: 0 when no local
+  // transform and no node list changed since the previous frame, whose
+  // absolute transforms (and normal matrices) are still in place.  The
+  // grid is capped (a frame without changes then dispatches few
+  // workgroups); the nodes are taken grid-stride.
+  if (*dirty == 0u) return;
+  for (uint32_t i = i0; i < n; i += gridDim.x * kBlock)
+    node_transform(parents, local, abs_out, nrm_out, i);
 }
 
 // --------------------------------------------------------------------------
@@ -2977,8 +2988,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g) {
 
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
                         uint32_t* counters, unsigned long long* stats, uint32_t* frame,
-                        float* nrm_out, const uint32_t* dirty, hipStream_t s) {
-  const uint32_t blocks = n == 0 ? 1 : (n + kBlock - 1) / kBlock;  // always runs: resets counters
+                        float* nrm_out, const uint32_t* dirty, uint32_t max_blocks, hipStream_t s) {
+  // always runs (it resets the counters): at least one workgroup
+  uint32_t blocks = n == 0 ? 1 : (n + kBlock - 1) / kBlock;
+  if (max_blocks > 0 && blocks > max_blocks) blocks = max_blocks;
   hipLaunchKernelGGL(k_nodes, dim3(blocks), dim3(kBlock), 0, s, parents, local, abs_out, n,
                      counters, stats, frame, nrm_out, dirty);
   return hipGetLastError();

@@ -34,7 +34,7 @@
 namespace gbr {
 hipError_t launch_nodes(const int32_t* parents, const float* local, float* abs_out, uint32_t n,
                         uint32_t* counters, unsigned long long* stats, uint32_t* frame,
-                        float* nrm_out, const uint32_t* dirty, hipStream_t s);
+                        float* nrm_out, const uint32_t* dirty, uint32_t max_blocks, hipStream_t s);
 hipError_t launch_clip(const FrameParams& p, uint32_t grid, int phase, hipStream_t s);
 int setup_blocks_per_cu(uint32_t lds_bins);
 hipError_t launch_setup(const FrameParams& p, const uint32_t* list, uint32_t list_ctr,
@@ -902,7 +902,7 @@ gbr_status enqueue_frame(gbr_renderer* r, const FrameParams& p) {
     StageTimer t(r, GBR_TIMING_NODES);
     HIP_TRY(launch_nodes(r->d_parents.ptr, r->d_local.ptr, r->d_abs.ptr, r->n_nodes, p.counters,
                          p.stats, p.frame, const_cast<float*>(p.normal_mats),
-                         r->d_nodes_dirty.ptr, r->stream));
+                         r->d_nodes_dirty.ptr, (uint32_t)r->cu_count * 2u, r->stream));
   }
   const uint32_t setup_grid = (uint32_t)(r->cu_count * r->setup_bpc);
   const uint32_t clip_grid = (uint32_t)r->cu_count * 4u;
