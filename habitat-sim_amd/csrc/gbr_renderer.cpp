@@ -1889,13 +1889,14 @@ gbr_status gbr_autotune(gbr_renderer* r, const gbr_autotune_params* params,
   const int lanes[] = {32, 64, 96, 128, 192};
   const int grains[] = {1, 2, 4, 8, 16};
   const int modes[] = {0, 1};
-  const int heavies[] = {0, 256, 512, 1024};
+  const int heavies[] = {0, 64, 128, 256, 512};
   const float thresholds[] = {-2.0f, -1.0f, 0.0f, 1.0f};
   // the proposal threshold only matters for the learned proposal
   const int nparams = best.cull_mode == GBR_CULL_LEARNED ? 7 : 6;
   for (int pass = 0; pass < params->passes; ++pass) {
     for (int param = 0; param < nparams; ++param) {
-      const int n = param == 0 ? 3 : param == 1 ? 4 : param == 2 ? 5 : param == 3 ? 5 : param == 4 ? 2 : 4;
+      const int n = param == 0 ? 3 : param == 1 ? 4 : param == 2 ? 5 : param == 3 ? 5 : param == 4 ? 2
+                    : param == 5 ? 5 : 4;
       for (int k = 0; k < n; ++k) {
         gbr_tuning t = best;
         switch (param) {
