@@ -8,9 +8,9 @@ in HBM (colour RGBA8, linear depth FP32, semantic uint32 planes).  The walk is
 run once per culling mode (plain = every triangle set up, frustum, hiz =
 two-phase depth-pyramid occlusion with the previous frame as proposal,
 learned = the same with the visibility MLP's proposal together with the
-previous frame's visible chunks, the proposal for a camera walk, and
-learned_mlp = the MLP's proposal alone; the MLP is trained first on other
-walks); each mode's launch parameters come from the library's autotuner
+previous frame's visible chunks, the proposal for a camera walk, where the
+MLP scores the chunks that come into view, and learned_mlp = the MLP's
+proposal alone; the MLP is trained first on other walks); each mode's launch parameters come from the library's autotuner
 (gbr_autotune) on the walk's first view; the headline value is the learned
 mode, and the final frame of every mode is compared pixel by pixel with the
 plain one.  The learned-proposal workload (teleports, no history) times the
@@ -179,7 +179,7 @@ def frame_bytes(spec, st, mode, sensors=3):
         b += envs * 2 * (4 * w * h * 4 // 3)
     if mode.startswith("learned"):
         from habitat_sim_amd import _native as N
-        b += 64 * (st["items_phase1"] + st["items_candidates"]) + 2 * N.VIS_PARAMS
+        b += 64 * st["items_scored"] + 2 * N.VIS_PARAMS
     return int(b)
 
 

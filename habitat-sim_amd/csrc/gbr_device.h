@@ -342,6 +342,7 @@ struct FrameParams {
   uint32_t* cand;               // work items awaiting the depth-pyramid test
   uint32_t* list2;              // work items of phase 2
   uint32_t* item_vis;           // frame number in which the item was last visible
+  uint32_t* item_view;          // frame number in which the item last passed the frustum test
   float* hiz;                   // depth pyramid, hiz_stride floats per scene
   uint32_t* frame;              // device frame number (>= 2 while drawing), advanced by k_nodes
   int32_t cull_mode;

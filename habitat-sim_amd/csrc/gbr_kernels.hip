@@ -2782,6 +2782,16 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
         zmin = fminf(zmin, nz); wmin = fminf(wmin, c.w); wmax = fmaxf(wmax, c.w);
       }
     }
+    // learned mode with history: a chunk that passed the frustum test in the
+    // previous frame without being visible was occluded there; it is not
+    // scored and waits for the depth-pyramid test, as in the Hi-Z mode (the
+    // model scores the chunks that come into view)
+    bool occluded_before = false;
+    if (out_all == 0u && p.cull_mode == kCullLearned) {
+      const uint32_t fr = *p.frame;
+      occluded_before = p.proposal_history && !p.collecting && p.item_view[item] + 1u == fr;
+      p.item_view[item] = fr;
+    }
     if (out_all == 0u) {
       if (!inner) {
         to1 = true;  // crosses a clip plane: always drawn in phase 1
@@ -2807,7 +2817,8 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
           // needs every chunk's row)
           const bool seen = p.cull_mode == kCullLearned && p.proposal_history && !p.collecting &&
                             p.item_vis[item] + 1u == *p.frame;
-          if (p.features && !seen) {
+          if (seen) occluded_before = false;
+          if (p.features && !seen && !occluded_before) {
             feat = true;
             if (p.feature_set == kFeaturesHistory)
               item_features_history(p, wi, sc, bb, M, PV, ic, wmin, wmax, item, frow);
@@ -2816,6 +2827,9 @@ __global__ __launch_bounds__(kListBlock) void k_cull(FrameParams p) {
           }
           if (seen) {
             to1 = true;
+          } else if (occluded_before) {
+            p.cull[item] = ic;
+            toc = true;
           } else if (p.cull_mode == kCullLearned) {
             p.cull[item] = ic;  // k_propose decides from the model's score
           } else {

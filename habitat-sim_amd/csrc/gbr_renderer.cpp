@@ -214,7 +214,7 @@ struct gbr_renderer {
   DevBuf<DrawDesc> d_item_draws;  // each work item's draw, indexed by item
   DevBuf<ItemBounds> d_bounds;
   DevBuf<ItemCull> d_cull;
-  DevBuf<uint32_t> d_list1, d_cand, d_list2, d_item_vis;
+  DevBuf<uint32_t> d_list1, d_cand, d_list2, d_item_vis, d_item_view;
   DevBuf<float> d_hiz;
   DevBuf<float> d_nrm, d_nmat;       // vertex normals, per-node normal matrices
   DevBuf<LightDesc> d_lights;
@@ -547,8 +547,10 @@ gbr_status rebuild_structure(gbr_renderer* r) {
     TRY(r->d_cand.reserve(n));
     TRY(r->d_list2.reserve(n));
     TRY(r->d_item_vis.reserve(n));
-    // visibility history refers to the old item numbering
+    TRY(r->d_item_view.reserve(n));
+    // visibility and frustum history refer to the old item numbering
     HIP_TRY(hipMemsetAsync(r->d_item_vis.ptr, 0, n * sizeof(uint32_t), s));
+    HIP_TRY(hipMemsetAsync(r->d_item_view.ptr, 0, n * sizeof(uint32_t), s));
     TRY(r->d_frame.reserve(1));
     HIP_TRY(hipMemsetD32Async(r->d_frame.ptr, 1u, 1, s));
   }
@@ -809,6 +811,7 @@ gbr_status do_draw(gbr_renderer* r) {
   // visibility feedback is kept in every mode, so switching to the
   // previous-frame proposal starts from the last frame drawn
   p.item_vis = r->d_item_vis.ptr;
+  p.item_view = r->d_item_view.ptr;
   p.frame = r->d_frame.ptr;
   p.proposal_threshold = r->tun.proposal_threshold;
   p.proposal_history = r->tun.proposal_history;
@@ -1998,6 +2001,7 @@ gbr_status gbr_frame_stats(gbr_renderer* r, gbr_frame_stats_info* out) {
     out->items_phase2 = r->tun.cull_mode >= GBR_CULL_HIZ ? c[kCtrList2 * kCtrStride] : 0;
     out->triangles_clipped = c[kCtrClipped * kCtrStride];
     out->clip_fan_triangles = c[kCtrSubTris * kCtrStride];
+    out->items_scored = r->tun.cull_mode == GBR_CULL_LEARNED ? c[kCtrProp * kCtrStride] : 0;
   } else {
     out->items_phase1 = r->n_items;
   }

@@ -94,7 +94,8 @@ class FrameStatsInfo(ctypes.Structure):
                 ("triangles_setup", ctypes.c_uint64), ("vertices_setup", ctypes.c_uint64),
                 ("triangles_phase2", ctypes.c_uint64), ("vertices_phase2", ctypes.c_uint64),
                 ("raster_pixels", ctypes.c_uint64), ("fragments", ctypes.c_uint64),
-                ("triangles_clipped", ctypes.c_uint64), ("clip_fan_triangles", ctypes.c_uint64)]
+                ("triangles_clipped", ctypes.c_uint64), ("clip_fan_triangles", ctypes.c_uint64),
+                ("items_scored", ctypes.c_uint64)]
 
 
 class MeshDesc(ctypes.Structure):

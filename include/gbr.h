@@ -130,9 +130,13 @@ typedef struct gbr_tuning {
                                  transforms changed (timing and row collection
                                  draw directly) */
   int32_t proposal_history;   /* GBR_CULL_LEARNED: 1 = phase 1 also takes the
-                                 chunks visible in the previous frame (camera
-                                 walks), 0 = the model's proposal alone (default;
-                                 the right choice when cameras jump) */
+                                 chunks visible in the previous frame, and the
+                                 model scores only the chunks that come into
+                                 view (one that passed the frustum test in the
+                                 previous frame without being visible waits
+                                 for the depth-pyramid test): camera walks;
+                                 0 = the model's proposal alone (default; the
+                                 right choice when cameras jump) */
   int32_t heavy_entries;      /* raster_mode 0: bins that receive at least this many
                                  entries in a phase are rasterised first (0: bins in
                                  order); default 256 */
@@ -169,6 +173,7 @@ typedef struct gbr_frame_stats_info {
   uint64_t fragments;         /* of which inside their triangle */
   uint64_t triangles_clipped; /* triangles through the near/far/guard-band clipper */
   uint64_t clip_fan_triangles; /* front-facing triangles of their clipped fans */
+  uint64_t items_scored;      /* GBR_CULL_LEARNED: work items scored by the model */
 } gbr_frame_stats_info;
 
 /* ---- parsed-file description consumed by gbr_add_file_data() (new) ---- */

@@ -163,6 +163,43 @@ def test_learned_proposal_without_history(trained):
     assert res[N.CULL_LEARNED][0] < res[N.CULL_HIZ][0]
 
 
+def test_history_scores_only_chunks_coming_into_view(trained):
+    """With the previous frame as part of the proposal, a chunk that was in
+    the frustum last frame is not scored: visible, it goes to phase 1,
+    hidden, to the depth-pyramid test.  A camera that does not move brings
+    nothing into view, so nothing is scored; without history the model
+    scores every chunk in view."""
+    spec, r, _ = trained
+    walk = visibility.camera_walk(spec, 2, seed=5)
+
+    def draw(views):
+        for e, v in enumerate(views):
+            r.update_camera(e, spec.envs[e].projection, v)
+        r.draw()
+        return r.frame_stats(), {n: getattr(r, f"{n}_image")() for n in PLANES}
+
+    try:
+        r.set_tuning(cull_mode=N.CULL_NONE)
+        _, plain0 = draw(walk[0])
+        _, plain1 = draw(walk[1])
+        r.set_tuning(cull_mode=N.CULL_LEARNED, proposal_history=0)
+        st_none, _ = draw(walk[0])
+        r.set_tuning(proposal_history=1)
+        draw(walk[0])
+        st_still, still = draw(walk[0])   # the same views again
+        st_moved, moved = draw(walk[1])   # one step of the walk
+        for n in PLANES:
+            np.testing.assert_array_equal(still[n], plain0[n], err_msg=n)
+            np.testing.assert_array_equal(moved[n], plain1[n], err_msg=n)
+        assert st_none["items_scored"] > 0
+        assert st_still["items_scored"] == 0
+        assert st_moved["items_scored"] < st_none["items_scored"]
+        print(f"chunks scored: no history {st_none['items_scored']}, still camera "
+              f"{st_still['items_scored']}, one walk step {st_moved['items_scored']}")
+    finally:
+        r.set_tuning(proposal_history=0)
+
+
 def test_online_fine_tuning_walk_stays_exact():
     # a walk in learned mode with rows collected every frame and a short
     # fine-tune every 5 frames (the 4096-env x 500-step configuration at a
