@@ -527,6 +527,41 @@ __device__ __forceinline__ bool hiz_hidden(const FrameParams& p, uint32_t scene,
   return true;
 }
 
+// hiz_hidden() for a chunk's rectangle, which may cover many tiles: the
+// tiles are read sixteen at a time, their loads issued together, so a
+// hidden chunk costs one round trip per sixteen tiles rather than one per
+// tile (same tiles, same answer).  Rows past the rectangle's last are read
+// as its last row; their values are not used.
+__device__ __forceinline__ bool hiz_hidden_wide(const FrameParams& p, uint32_t scene, int x0, int y0,
+                                                int x1, int y1, float zmin, int max_tiles) {
+  const float* h = p.hiz + (size_t)scene * p.hiz_stride;
+  int w = p.hiz_w0;
+  int bx0 = x0 >> kHiZ0, bx1 = x1 >> kHiZ0, by0 = y0 >> kHiZ0, by1 = y1 >> kHiZ0;
+  if ((bx1 - bx0 + 1) * (by1 - by0 + 1) > max_tiles) {
+    w = p.hiz_w1;
+    h += p.hiz_w0 * p.hiz_h0;
+    bx0 = x0 >> kHiZ1; bx1 = x1 >> kHiZ1; by0 = y0 >> kHiZ1; by1 = y1 >> kHiZ1;
+  }
+  const int nx = bx1 - bx0 + 1, n = nx * (by1 - by0 + 1);
+  int c = 0, r = by0;
+  for (int t0 = 0; t0 < n; t0 += 16) {
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      v[k] = h[min(r, by1) * w + bx0 + c];
+      if (++c == nx) {
+        c = 0;
+        ++r;
+      }
+    }
+    bool hidden = true;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) hidden = hidden && (t0 + k >= n || zmin > v[k]);
+    if (!hidden) return false;
+  }
+  return true;
+}
+
 // A load of data the kernel never writes, at an address uniform over the
 // wave: through the constant address space it becomes a scalar load into
 // SGPRs (no VGPRs, scalar-cache latency).
@@ -2952,7 +2987,7 @@ __global__ __launch_bounds__(kListBlock) void k_cull_hiz(FrameParams p) {
   if (i < n) {
     item = p.cand[i];
     const ItemCull ic = p.cull[item];
-    keep = !hiz_hidden(p, ic.pad[0], ic.x0, ic.y0, ic.x1, ic.y1, ic.zmin, kHiZTiles);
+    keep = !hiz_hidden_wide(p, ic.pad[0], ic.x0, ic.y0, ic.x1, ic.y1, ic.zmin, kHiZTiles);
   }
   uint32_t* const ctr[1] = {&p.counters[kCtrList2 * kCtrStride]};
   const bool flag[1] = {keep};
